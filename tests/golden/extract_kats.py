@@ -1,0 +1,135 @@
+"""Transcribe the reference's inline known-answer tests (KATs) into data fixtures.
+
+The reference keeps its golden vectors inline in Catch2 tests as strings in the
+``ImageFromString`` format ``"<Scalar> <GUINT8|GINT32|GINT64|F32|D64> <rank> <dims...> <values...>"``
+(src/base/image_interface.cpp:407-425,616-650).  This script reads those test
+sources as text, pulls out the input images, the operation called and the expected
+output string, and writes them as plain data to ``kats.json`` next to this file.
+No reference code is executed or copied: only the numbers and the name of the
+operation each case calls are kept.
+
+Sources read (relative to the reference root):
+  test/low_level_morpho/ero_dil_static_se_t.cpp   (t_Dilate/t_Erode by Common_NL_SE)
+  test/low_level_morpho/ero_dil_runtime_se.cpp    (t_Dilate/t_Erode by runtime neighbor_list_t)
+  test/low_level_morpho/ero_dil_line_se.cpp       (t_DilateX/ErodeX/DilateY/ErodeY)
+  test/geodesy/mreconstruct.cpp                   (geo::Reconstruction)
+  test/geodesy/leveling.cpp                       (low_/high_leveling, leveling)
+  test/geodesy/extrema.cpp                        (h_minima/h_maxima/h_concave/h_convex/dynamic_pseudo_*)
+  test/pixel_processing/transpose.cpp             (t_transpose)
+
+Usage: python tests/golden/extract_kats.py [/root/reference]
+"""
+import json
+import os
+import re
+import sys
+
+FILES = [
+    "test/low_level_morpho/ero_dil_static_se_t.cpp",
+    "test/low_level_morpho/ero_dil_runtime_se.cpp",
+    "test/low_level_morpho/ero_dil_line_se.cpp",
+    "test/geodesy/mreconstruct.cpp",
+    "test/geodesy/leveling.cpp",
+    "test/geodesy/extrema.cpp",
+    "test/pixel_processing/transpose.cpp",
+]
+
+OPS = [
+    "t_DilateX", "t_ErodeX", "t_DilateY", "t_ErodeY", "t_Dilate", "t_Erode",
+    "Reconstruction", "low_leveling", "high_leveling", "leveling",
+    "h_minima", "h_maxima", "h_concave", "h_convex",
+    "dynamic_pseudo_closing", "dynamic_pseudo_opening", "t_transpose",
+]
+
+_STR = r'(?:R"\((?P<raw>.*?)\)"|(?P<cooked>"(?:[^"\\]|\\.|\\\n)*"))'
+TOKEN_RE = re.compile(
+    r'(?P<tc>TEST_CASE\("(?P<tcname>[^"]*)")'
+    r'|(?P<ifs>ImageFromString\(\s*' + _STR.replace("raw", "raw1").replace("cooked", "cooked1") + r')'
+    r'|(?P<exp>expected\s*=\s*' + _STR.replace("raw", "raw2").replace("cooked", "cooked2") + r')'
+    r'|(?P<const>static_cast<poutre::p\w+>\((?P<cval>-?\d+)\))'
+    r'|(?P<call>(?:poutre::)?(?:llm::details::|geo::|details::)?(?P<op>' + "|".join(OPS) + r')\((?P<args>[^;]*)\);)',
+    re.S,
+)
+
+
+def _cooked(s):
+    """Decode a C string literal that may use backslash-newline continuations."""
+    body = s[1:-1]
+    body = body.replace("\\\n", "")
+    return body.encode().decode("unicode_escape")
+
+
+def parse_image(text):
+    toks = text.split()
+    compound, ptype, rank = toks[0], toks[1], int(toks[2])
+    shape = [int(t) for t in toks[3:3 + rank]]
+    n = 1
+    for d in shape:
+        n *= d
+    vals = toks[3 + rank:]
+    conv = float if ptype in ("F32", "D64") else int
+    values = [conv(v) for v in vals]
+    return {"compound": compound, "ptype": ptype, "shape": shape,
+            "values": values[:n], "n_listed": len(values)}
+
+
+def extract(ref_root):
+    cases = []
+    for rel in FILES:
+        path = os.path.join(ref_root, rel)
+        src = open(path).read()
+        name = None
+        inputs, call, h = [], None, None
+        sub = 0
+        for m in TOKEN_RE.finditer(src):
+            line = src.count("\n", 0, m.start()) + 1
+            if m.group("tc"):
+                name, inputs, call, h, sub = m.group("tcname"), [], None, None, 0
+            elif m.group("ifs"):
+                raw = m.group("raw1")
+                txt = raw if raw is not None else _cooked(m.group("cooked1"))
+                inputs.append(parse_image(txt))
+            elif m.group("const"):
+                h = int(m.group("cval"))
+            elif m.group("call"):
+                op = m.group("op")
+                args = m.group("args")
+                se = re.search(r"\b(SE\w+)", args)
+                rec = re.search(r"reconstruction_type::(\w+)", args)
+                rr = None
+                if op in ("t_DilateX", "t_ErodeX", "t_DilateY", "t_ErodeY"):
+                    rr = int(args.split(",")[1].strip())
+                call = {"op": op, "se": se.group(1) if se else None,
+                        "rec_type": rec.group(1) if rec else None, "r": rr,
+                        "runtime_se": "Common_NL_SE" not in args and se is not None,
+                        "call_line": line}
+            elif m.group("exp"):
+                raw = m.group("raw2")
+                txt = raw if raw is not None else _cooked(m.group("cooked2"))
+                exp = parse_image(txt)
+                n = 1
+                for d in exp["shape"]:
+                    n *= d
+                case = {"file": rel, "name": name, "sub": sub, "line": line,
+                        "inputs": inputs, "expected": exp,
+                        "expected_complete": exp["n_listed"] >= n}
+                case.update(call or {})
+                if h is not None:
+                    case["h"] = h
+                cases.append(case)
+                sub += 1
+                inputs, call, h = [], None, None
+    return cases
+
+
+def main():
+    ref_root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    cases = extract(ref_root)
+    out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "kats.json")
+    with open(out, "w") as f:
+        json.dump({"source": "ThomasRetornaz/poutre2 inline KATs", "cases": cases}, f, indent=1)
+    print(f"wrote {len(cases)} cases to {out}")
+
+
+if __name__ == "__main__":
+    main()

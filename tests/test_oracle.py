@@ -1,0 +1,233 @@
+"""Pins the CPU oracle before anything is compared against it.
+
+1. Every known-answer test of the reference for this path (tests/golden/kats.json).
+2. Independent checks the mathematics fixes: brute-force window max/min, composition
+   identities of iterated structuring elements, the brute-force fixed point of
+   reconstruction, the closed forms of leveling, saturating arithmetic.
+Each check is chosen so that a plausible mistake in the oracle (off-by-one window,
+wrong neutral, non-causal scan split, missing clamp) fails one of them.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests import kats
+
+
+def _run_kat(c):
+    ins = [kats.image(d) for d in c["inputs"]]
+    op = c["op"]
+    if op in ("t_Dilate", "t_Erode"):
+        return oracle.static(ins[0], c["se"], dilate=(op == "t_Dilate"))
+    if op in ("t_DilateX", "t_ErodeX"):
+        return oracle.line_x(ins[0], c["r"], dilate=(op == "t_DilateX"))
+    if op in ("t_DilateY", "t_ErodeY"):
+        return oracle.line_y(ins[0], c["r"], dilate=(op == "t_DilateY"))
+    if op == "Reconstruction":
+        return oracle.reconstruct(c["rec_type"], ins[0], ins[1], c["se"])
+    if op == "low_leveling":
+        return oracle.low_leveling(ins[0], ins[1], c["se"])
+    if op == "high_leveling":
+        return oracle.high_leveling(ins[0], ins[1], c["se"])
+    if op == "leveling":
+        return oracle.leveling(ins[0], ins[1], c["se"])
+    if op in ("h_minima", "h_maxima", "h_concave", "h_convex", "dynamic_pseudo_closing",
+              "dynamic_pseudo_opening"):
+        return getattr(oracle, op)(ins[0], c["h"], c["se"])
+    if op == "t_transpose":
+        return oracle.transpose(ins[0])
+    raise AssertionError(op)
+
+
+@pytest.mark.parametrize("case", kats.cases(), ids=kats.case_id)
+def test_oracle_matches_reference_kat(case):
+    got = _run_kat(case)
+    exp = kats.image(case["expected"])
+    assert got.shape == exp.shape
+    np.testing.assert_array_equal(got, exp)
+
+
+def test_kat_count():
+    # 20 static + 20 runtime + 6 line + 4 reconstruction + 4 leveling + 6 extrema + 1 transpose
+    assert len(kats.cases()) == 61
+
+
+# ---------------------------------------------------------------- brute-force pins
+def _window_brute(img, ry, rx, dilate):
+    """max/min over the in-bounds part of a (2ry+1)x(2rx+1) window, numpy, no padding trick."""
+    H, W = img.shape
+    out = np.empty_like(img)
+    f = np.max if dilate else np.min
+    for y in range(H):
+        for x in range(W):
+            out[y, x] = f(img[max(0, y - ry):y + ry + 1, max(0, x - rx):x + rx + 1])
+    return out
+
+
+RNG = np.random.default_rng(1234)
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.int32, np.float32, np.int64, np.uint16, np.float64])
+@pytest.mark.parametrize("shape,r", [((7, 13), 1), ((9, 5), 2), ((6, 17), 4), ((3, 4), 9), ((1, 1), 3)])
+def test_line_x_brute(dtype, shape, r):
+    img = (RNG.integers(0, 200, size=shape) - (50 if np.dtype(dtype).kind in "if" else 0)).astype(dtype)
+    for d in (True, False):
+        np.testing.assert_array_equal(oracle.line_x(img, r, d), _window_brute(img, 0, r, d))
+
+
+@pytest.mark.parametrize("n,r", [(11, 2), (12, 5), (3, 7)])
+def test_line_y_brute_and_square_only(n, r):
+    img = RNG.integers(0, 256, size=(n, n)).astype(np.uint8)
+    for d in (True, False):
+        np.testing.assert_array_equal(oracle.line_y(img, r, d), _window_brute(img, r, 0, d))
+    with pytest.raises(oracle.OracleError):
+        oracle.line_y(RNG.integers(0, 256, size=(n, n + 1)).astype(np.uint8), r)
+
+
+def test_line_r0_copies():
+    img = RNG.integers(0, 256, size=(5, 6)).astype(np.uint8)
+    np.testing.assert_array_equal(oracle.line_x(img, 0), img)
+    np.testing.assert_array_equal(oracle.line_x(img, -3, False), img)
+
+
+@pytest.mark.parametrize("se", ["SESquare2D", "SECross2D", "SESegmentX2D", "SESegmentY2D"])
+def test_static_2d_brute(se):
+    img = RNG.integers(0, 256, size=(9, 11)).astype(np.uint8)
+    offs = oracle.se_offsets(se)
+    H, W = img.shape
+    for d in (True, False):
+        exp = np.empty_like(img)
+        for y in range(H):
+            for x in range(W):
+                v = [img[y + dy, x + dx] for dy, dx in offs if 0 <= y + dy < H and 0 <= x + dx < W]
+                exp[y, x] = max(v) if d else min(v)
+        np.testing.assert_array_equal(oracle.static(img, se, d), exp)
+
+
+def test_se_geometry():
+    sq = {tuple(o) for o in oracle.se_offsets("SESquare2D")}
+    assert sq == {(dy, dx) for dy in (-1, 0, 1) for dx in (-1, 0, 1)}
+    cr = {tuple(o) for o in oracle.se_offsets("SECross2D")}
+    assert cr == {(0, 0), (1, 0), (-1, 0), (0, 1), (0, -1)}
+    assert len(oracle.se_offsets("SESquare3D")) == 27
+    assert {tuple(o) for o in oracle.se_offsets("SECross3D")} == {
+        (0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)}
+    assert {tuple(o) for o in oracle.se_offsets("SESegmentZ3D")} == {(0, 0, 0), (1, 0, 0), (-1, 0, 0)}
+    assert {tuple(o) for o in oracle.se_offsets("SESegmentX1D")} == {(0,), (1,), (-1,)}
+
+
+@pytest.mark.parametrize("it", [0, 1, 2, 3, 4])
+def test_static_iter_square_equals_window(it):
+    """Iterated 3x3 square on a box = (2it+1)^2 window (L-inf geodesic distance = L-inf distance)."""
+    img = RNG.integers(0, 256, size=(13, 10)).astype(np.uint8)
+    for d in (True, False):
+        np.testing.assert_array_equal(oracle.static_iter(img, "SESquare2D", it, d), _window_brute(img, it, it, d))
+
+
+@pytest.mark.parametrize("it", [2, 3])
+def test_static_iter_cross_equals_diamond(it):
+    img = RNG.integers(0, 256, size=(11, 12)).astype(np.int32)
+    diamond = [(dy, dx) for dy in range(-it, it + 1) for dx in range(-it, it + 1) if abs(dy) + abs(dx) <= it]
+    for d in (True, False):
+        np.testing.assert_array_equal(oracle.static_iter(img, "SECross2D", it, d), oracle.nl(img, diamond, d))
+
+
+def test_static_iter_negative_rejected():
+    img = np.zeros((3, 3), np.uint8)
+    with pytest.raises(oracle.OracleError):
+        oracle.static_iter(img, "SESquare2D", -1)
+
+
+def test_static_3d_vs_window():
+    img = RNG.integers(0, 256, size=(5, 6, 7)).astype(np.uint8)
+    out = oracle.static(img, "SESquare3D", True)
+    D, H, W = img.shape
+    exp = np.empty_like(img)
+    for z in range(D):
+        for y in range(H):
+            for x in range(W):
+                exp[z, y, x] = img[max(0, z - 1):z + 2, max(0, y - 1):y + 2, max(0, x - 1):x + 2].max()
+    np.testing.assert_array_equal(out, exp)
+
+
+# ---------------------------------------------------------------- reconstruction pins
+def _rand_pair(shape, dtype, rng, hi=50):
+    mask = rng.integers(0, hi, size=shape).astype(dtype)
+    marker = rng.integers(0, hi, size=shape).astype(dtype)
+    return marker, mask
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.int32, np.int64])
+@pytest.mark.parametrize("se,shape", [("SESquare2D", (17, 23)), ("SECross2D", (19, 14)),
+                                      ("SESegmentX1D", (40,)), ("SECross3D", (5, 6, 7)),
+                                      ("SESquare3D", (4, 5, 6))])
+@pytest.mark.parametrize("rec", ["dilate", "erode"])
+def test_reconstruct_hybrid_equals_fixed_point(dtype, se, shape, rec):
+    rng = np.random.default_rng(7)
+    for _ in range(3):
+        marker, mask = _rand_pair(shape, dtype, rng)
+        # sparse marker, like an h-maxima seed
+        if rec == "dilate":
+            marker = np.where(rng.random(shape) < 0.05, marker, 0).astype(dtype)
+        hyb = oracle.reconstruct(rec, marker, mask, se)
+        bf = oracle.reconstruct_bf(rec, marker, mask, se)
+        np.testing.assert_array_equal(hyb, bf)
+        lo = np.minimum(marker, mask) if rec == "dilate" else mask
+        hi = mask if rec == "dilate" else np.maximum(marker, mask)
+        assert (hyb >= lo).all() and (hyb <= hi).all()
+        np.testing.assert_array_equal(oracle.reconstruct(rec, hyb, mask, se), hyb)  # idempotent
+
+
+def test_reconstruct_bf_closed_form_plateau():
+    """A single seed inside a connected plateau floods it and nothing else (8- vs 4-connectivity)."""
+    mask = np.zeros((5, 5), np.uint8)
+    mask[1, 1] = mask[2, 2] = mask[3, 3] = 9  # diagonal chain: 8-connected, not 4-connected
+    marker = np.zeros_like(mask)
+    marker[1, 1] = 9
+    sq = oracle.reconstruct_bf("dilate", marker, mask, "SESquare2D")
+    cr = oracle.reconstruct_bf("dilate", marker, mask, "SECross2D")
+    np.testing.assert_array_equal(sq, mask)
+    assert cr[1, 1] == 9 and cr[2, 2] == 0 and cr[3, 3] == 0
+
+
+# ---------------------------------------------------------------- leveling pins
+@pytest.mark.parametrize("dtype", [np.uint8, np.int32])
+@pytest.mark.parametrize("se,shape", [("SESquare2D", (16, 21)), ("SECross2D", (13, 9)), ("SESegmentX1D", (50,))])
+def test_leveling_priority_queue_equals_closed_forms(dtype, se, shape):
+    """low(r,g) = r ^ R^d_{r v g}(g); high(r,g) = r v R^e_{r ^ g}(g); leveling = low(high(f,g),g)."""
+    rng = np.random.default_rng(11)
+    for _ in range(4):
+        r = rng.integers(0, 30, size=shape).astype(dtype)
+        g = rng.integers(0, 30, size=shape).astype(dtype)
+        low = oracle.low_leveling(r, g, se)
+        np.testing.assert_array_equal(
+            low, np.minimum(r, oracle.reconstruct_bf("dilate", g, np.maximum(r, g), se)))
+        high = oracle.high_leveling(r, g, se)
+        np.testing.assert_array_equal(
+            high, np.maximum(r, oracle.reconstruct_bf("erode", g, np.minimum(r, g), se)))
+        lev = oracle.leveling(r, g, se)
+        np.testing.assert_array_equal(lev, oracle.low_leveling(high, g, se))
+
+
+# ---------------------------------------------------------------- arithmetic pins
+def test_saturating_arith_closed_forms():
+    a = np.arange(256, dtype=np.uint8)
+    np.testing.assert_array_equal(oracle.ssub_const(a, 32), np.maximum(a.astype(int) - 32, 0).astype(np.uint8))
+    np.testing.assert_array_equal(oracle.sadd_const(a, 32), np.minimum(a.astype(int) + 32, 255).astype(np.uint8))
+    b = a[::-1].copy()
+    np.testing.assert_array_equal(oracle.ssub_image(a, b),
+                                  np.maximum(a.astype(int) - b.astype(int), 0).astype(np.uint8))
+    i = np.array([-2**31, -5, 0, 2**31 - 1], np.int32)
+    np.testing.assert_array_equal(oracle.ssub_const(i, 10), np.array([-2**31, -15, -10, 2**31 - 11], np.int32))
+    # float saturated subtraction floors at FLT_MIN (a positive number), as the reference's
+    # numeric_limits<float>::min() does
+    f = np.array([1.0, 0.5, -3.0], np.float32)
+    got = oracle.ssub_const(f, 1.0)
+    assert got[0] == np.finfo(np.float32).tiny and got[1] == np.finfo(np.float32).tiny
+
+
+def test_h_maxima_is_ssub_then_reconstruct():
+    rng = np.random.default_rng(3)
+    f = rng.integers(0, 256, size=(20, 20)).astype(np.uint8)
+    exp = oracle.reconstruct_bf("dilate", np.maximum(f.astype(int) - 32, 0).astype(np.uint8), f, "SESquare2D")
+    np.testing.assert_array_equal(oracle.h_maxima(f, 32, "SESquare2D"), exp)
