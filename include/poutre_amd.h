@@ -1,0 +1,156 @@
+/*
+ * poutre_amd.h -- C ABI of the MI355X (gfx950) morphology / geodesy library.
+ *
+ * Drop-in boundary for Poutre2's hot path.  Each entry point replaces one reference
+ * C++ API function (cited below, paths relative to the Poutre2 source root); the
+ * C++ shim (poutre_amd.hpp) and the Python bindings (poutre2_amd) both call these.
+ *
+ * Conventions
+ *  - Images are described by pm_image: a DEVICE pointer (HBM, from hipMalloc or a
+ *    torch tensor), the reference PType bit value, rank 1..3 and the shape in the
+ *    reference's order ({W}, {H, W}, {D, H, W}; last axis fastest, dense row-major).
+ *  - `stream` is a hipStream_t (NULL = default stream).  Calls are stream-ordered:
+ *    they enqueue work and return without synchronising, except the geodesic
+ *    fixed-point calls, which read a convergence flag back each global round.
+ *  - Return value: PM_OK (0) or a negative pm_status; pm_last_error() then returns
+ *    a thread-local message.  Checks happen in the reference's order: sizes, types,
+ *    aliasing, rank, SE (src/low_level_morpho/ero_dil_static.cpp:229-236).
+ *  - Temporaries come from the stream-ordered HIP allocator; the caller never
+ *    allocates scratch.
+ */
+#ifndef POUTRE_AMD_H
+#define POUTRE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* PType bit values, include/poutre/base/types.hpp:517-527.  PM_PTYPE_U16 has no
+ * reference counterpart (uint16 is an extension of this library). */
+enum pm_ptype {
+  PM_PTYPE_U8 = 1 << 2,
+  PM_PTYPE_I32 = 1 << 3,
+  PM_PTYPE_F32 = 1 << 4,
+  PM_PTYPE_I64 = 1 << 5,
+  PM_PTYPE_D64 = 1 << 6,
+  PM_PTYPE_U16 = 1 << 7,
+};
+
+/* se::Common_NL_SE values, include/poutre/structuring_element/se_types_and_tags.hpp:45-57 */
+enum pm_common_se {
+  PM_SE_UNDEF = 0,
+  PM_SE_SEGMENT_X_1D = 1,
+  PM_SE_SQUARE_2D = 2,
+  PM_SE_CROSS_2D = 3,
+  PM_SE_SEGMENT_X_2D = 4,
+  PM_SE_SEGMENT_Y_2D = 5,
+  PM_SE_SEGMENT_X_3D = 6,
+  PM_SE_SEGMENT_Y_3D = 7,
+  PM_SE_SEGMENT_Z_3D = 8,
+  PM_SE_CROSS_3D = 9,
+  PM_SE_SQUARE_3D = 10,
+};
+
+/* se::Compound_NL_SE, se_types_and_tags.hpp:59-63 */
+enum pm_compound_se { PM_COMPOUND_UNDEF = 0, PM_COMPOUND_OCTAGON = 1, PM_COMPOUND_RHOMBICUBOCTAHEDRON = 2 };
+
+/* geo::reconstruction_type, include/poutre/geodesy/mreconstruct.hpp:30-34 */
+enum pm_reconstruction_type { PM_REC_ERODE = 0, PM_REC_DILATE = 1 };
+
+enum pm_status {
+  PM_OK = 0,
+  PM_E_SIZE = -1,        /* AssertSizesCompatible, image_interface.hpp:95 */
+  PM_E_PTYPE = -2,       /* AssertAsTypesCompatible / unsupported PType */
+  PM_E_ALIAS = -3,       /* AssertImagesAreDifferent */
+  PM_E_RANK = -4,        /* unsupported rank or SE/rank mismatch */
+  PM_E_SE = -5,          /* unknown structuring element */
+  PM_E_ARG = -6,         /* iter < 0, bad offsets, NULL pointers ... */
+  PM_E_HIP = -7,         /* HIP runtime error */
+  PM_E_UNSUPPORTED = -8, /* the reference throws for this combination too */
+  PM_E_NOCONV = -9,      /* geodesic iteration cap reached */
+};
+
+typedef struct pm_image {
+  void *data;
+  int32_t ptype;
+  int32_t rank;
+  int64_t shape[3];
+} pm_image;
+
+/* thread-local message of the last failing call on this thread ("" if none) */
+const char *pm_last_error(void);
+/* library version string */
+const char *pm_version(void);
+
+/* ---- low-level morphology: poutre::Erode/Dilate by Common_NL_SE with iterations
+ * include/poutre/low_level_morpho/ero_dil.hpp:32,35 -> src/low_level_morpho/ero_dil_static.cpp:229-305 */
+int pm_erode_static(const pm_image *in, int se, int iter, pm_image *out, void *stream);
+int pm_dilate_static(const pm_image *in, int se, int iter, pm_image *out, void *stream);
+
+/* poutre::Erode/Dilate by Compound_NL_SE, ero_dil.hpp:38,41 -> src/low_level_morpho/ero_dil_compound_static.cpp:58-188 */
+int pm_erode_compound(const pm_image *in, int compound, int size, pm_image *out, void *stream);
+int pm_dilate_compound(const pm_image *in, int compound, int size, pm_image *out, void *stream);
+
+/* poutre::Erode/Dilate by a runtime neighbour list (se::neighbor_list_t<R>), ero_dil.hpp:44,47 ->
+ * src/low_level_morpho/ero_dil.cpp:59-243.  `offsets` holds n rows of `rank` int64 {dz,dy,dx}. */
+int pm_erode_nl(const pm_image *in, const int64_t *offsets, int n, pm_image *out, void *stream);
+int pm_dilate_nl(const pm_image *in, const int64_t *offsets, int n, pm_image *out, void *stream);
+
+/* poutre::ErodeX/DilateX/ErodeY/DilateY, include/poutre/low_level_morpho/ero_dil_line.hpp:32-47 ->
+ * src/low_level_morpho/ero_dil_line.cpp:80-295.  X: rank 1 and 2; Y: rank 2.  The reference's
+ * Y pass only accepts square images (its transposed temporary keeps the input shape,
+ * ero_dil_line_se_t.hpp:258-259); these entry points accept any H, W and the C++ shim
+ * poutre::ErodeY/DilateY restores the reference's error for H != W. */
+int pm_erode_x(const pm_image *in, int64_t half, pm_image *out, void *stream);
+int pm_dilate_x(const pm_image *in, int64_t half, pm_image *out, void *stream);
+int pm_erode_y(const pm_image *in, int64_t half, pm_image *out, void *stream);
+int pm_dilate_y(const pm_image *in, int64_t half, pm_image *out, void *stream);
+
+/* Flat erosion/dilation by a (2ry+1) x (2rx+1) rectangle (rank 2, or rank 3 slice-wise), the
+ * separable building block of SESquare2D iterations (no reference counterpart of its own:
+ * equals DilateX then DilateY, ero_dil_line_se_t.hpp:192-280). */
+int pm_erode_rect(const pm_image *in, int64_t ry, int64_t rx, pm_image *out, void *stream);
+int pm_dilate_rect(const pm_image *in, int64_t ry, int64_t rx, pm_image *out, void *stream);
+
+/* Opening / closing / morphological gradient by a Common_NL_SE iterated `iter` times.
+ * No reference symbol (only mentioned at include/poutre/low_level_morpho/low_level_morpho.hpp:15):
+ * defined as Dilate(Erode(f)), Erode(Dilate(f)) and the saturated difference Dilate(f) - Erode(f),
+ * each operand computed with pm_*_static semantics. */
+int pm_opening(const pm_image *in, int se, int iter, pm_image *out, void *stream);
+int pm_closing(const pm_image *in, int se, int iter, pm_image *out, void *stream);
+int pm_gradient(const pm_image *in, int se, int iter, pm_image *out, void *stream);
+
+/* ---- geodesy
+ * poutre::geo::Reconstruction, include/poutre/geodesy/mreconstruct.hpp:37-41 ->
+ * src/geodesy/mreconstruct.cpp:48-132 (mreconstruct_t.hpp:75-512). */
+int pm_reconstruct(int rec_type, const pm_image *marker, const pm_image *mask, int se, pm_image *out,
+                   void *stream);
+/* poutre::geo::low_leveling / high_leveling / leveling, include/poutre/geodesy/leveling.hpp:37-52 ->
+ * src/geodesy/leveling.cpp:67-237 */
+int pm_low_leveling(const pm_image *ref, const pm_image *marker, int se, pm_image *out, void *stream);
+int pm_high_leveling(const pm_image *ref, const pm_image *marker, int se, pm_image *out, void *stream);
+int pm_leveling(const pm_image *ref, const pm_image *marker, int se, pm_image *out, void *stream);
+/* poutre::geo::h_minima / h_maxima / h_concave / h_convex / dynamic_pseudo_opening / _closing,
+ * include/poutre/geodesy/extrema.hpp:33-64 -> src/geodesy/extrema.cpp:27-114.  `h` is the
+ * ScalarTypeVariant value converted to double. */
+int pm_h_maxima(const pm_image *in, double h, int se, pm_image *out, void *stream);
+int pm_h_minima(const pm_image *in, double h, int se, pm_image *out, void *stream);
+int pm_h_convex(const pm_image *in, double h, int se, pm_image *out, void *stream);
+int pm_h_concave(const pm_image *in, double h, int se, pm_image *out, void *stream);
+int pm_dynamic_pseudo_opening(const pm_image *in, double h, int se, pm_image *out, void *stream);
+int pm_dynamic_pseudo_closing(const pm_image *in, double h, int se, pm_image *out, void *stream);
+
+/* ---- elementwise helpers on the path (include/poutre/pixel_processing/arith.hpp:31-53):
+ * ArithSupImage / ArithInfImage / ArithSaturatedSubImage / ArithSaturatedSub/AddConstant */
+int pm_arith_sup(const pm_image *a, const pm_image *b, pm_image *out, void *stream);
+int pm_arith_inf(const pm_image *a, const pm_image *b, pm_image *out, void *stream);
+int pm_arith_ssub_image(const pm_image *a, const pm_image *b, pm_image *out, void *stream);
+int pm_arith_ssub_const(const pm_image *in, double c, pm_image *out, void *stream);
+int pm_arith_sadd_const(const pm_image *in, double c, pm_image *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* POUTRE_AMD_H */

@@ -1,0 +1,300 @@
+"""poutre2_amd -- MI355X (gfx950) implementation of Poutre2's morphology / geodesy hot path.
+
+Python entry points with the names and argument order of the reference's nanobind
+module ``pypoutre`` (python/low_level_morpho/ero_dil.cpp:12-27,
+python/geodesy/geodesy.cpp:12-29, python/structuring_element/se_types.cpp:10-36),
+operating on device-resident images: contiguous torch tensors on a ROCm device, of
+dtype uint8 / uint16 / int32 / int64 / float32 / float64, shaped ``(W,)``, ``(H, W)`` or
+``(D, H, W)``.  Every call goes through the C ABI of libpoutre_amd.so on the current
+torch stream; there is no CPU fallback.
+
+Differences from pypoutre, all additive:
+  * ``out`` may be omitted, in which case a new tensor is returned;
+  * ``h_minima`` calls h_minima (the reference binds h_maxima under that name,
+    python/geodesy/geodesy.cpp:19);
+  * ``reconstruction`` is bound (the reference has no binding for it);
+  * ``opening`` / ``closing`` / ``gradient`` / ``erode_rect`` / ``dilate_rect`` are added.
+"""
+import ctypes
+import enum
+
+from . import _capi
+from ._capi import PoutreError
+
+__all__ = ["static_nl_se", "static_compound_se", "RecType", "erode", "dilate", "erodeX", "dilateX", "erodeY",
+           "dilateY", "erode_rect", "dilate_rect", "opening", "closing", "gradient", "reconstruction",
+           "h_minima", "h_maxima", "h_concave", "h_convex", "dynamic_pseudo_closing", "dynamic_pseudo_opening",
+           "high_leveling", "low_leveling", "leveling", "arith_sup", "arith_inf", "arith_ssub_image",
+           "arith_ssub_const", "arith_sadd_const", "PoutreError", "version"]
+
+
+class static_nl_se(enum.IntEnum):
+    """se::Common_NL_SE (se_types_and_tags.hpp:45-57), pypoutre names."""
+    segx1d = 1
+    square2d = 2
+    cross2d = 3
+    segx2d = 4
+    segy2d = 5
+    segx3d = 6
+    segy3d = 7
+    segz3d = 8
+    cross3d = 9
+    square3d = 10
+
+
+class static_compound_se(enum.IntEnum):
+    """se::Compound_NL_SE (se_types_and_tags.hpp:59-63)."""
+    octagon = 1
+    rhombicuboctahedron = 2
+
+
+class RecType(enum.IntEnum):
+    """geo::reconstruction_type (mreconstruct.hpp:30-34)."""
+    erode = 0
+    dilate = 1
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _ptype(t):
+    torch = _torch()
+    m = {torch.uint8: _capi.PTYPE_U8, torch.int32: _capi.PTYPE_I32, torch.float32: _capi.PTYPE_F32,
+         torch.int64: _capi.PTYPE_I64, torch.float64: _capi.PTYPE_D64}
+    if hasattr(torch, "uint16"):
+        m[torch.uint16] = _capi.PTYPE_U16
+    if t.dtype not in m:
+        raise PoutreError(-2, f"Unsupported PType for dtype {t.dtype}")
+    return m[t.dtype]
+
+
+def _desc(t):
+    if not t.is_cuda:
+        raise PoutreError(-6, "images must be device tensors (HBM-resident)")
+    if not t.is_contiguous():
+        raise PoutreError(-6, "images must be contiguous")
+    if t.dim() < 1 or t.dim() > 3:
+        raise PoutreError(-4, f"Unsupported number of dims {t.dim()}")
+    d = _capi.PmImage()
+    d.data = t.data_ptr()
+    d.ptype = _ptype(t)
+    d.rank = t.dim()
+    for i, s in enumerate(t.shape):
+        d.shape[i] = s
+    return d
+
+
+def _stream(t):
+    return ctypes.c_void_p(_torch().cuda.current_stream(t.device).cuda_stream)
+
+
+def _out(img, out):
+    return _torch().empty_like(img) if out is None else out
+
+
+def _call(name, *args):
+    _capi.check(getattr(_capi.lib(), name)(*args))
+
+
+def version():
+    return _capi.lib().pm_version().decode()
+
+
+# ---------------------------------------------------------------- low-level morphology
+def _erodil(prefix, img, se, it, out):
+    out = _out(img, out)
+    di, do = _desc(img), _desc(out)
+    if isinstance(se, static_compound_se):
+        _call(f"pm_{prefix}_compound", ctypes.byref(di), int(se), int(it), ctypes.byref(do), _stream(img))
+    elif isinstance(se, (int, static_nl_se)) and not isinstance(se, bool):
+        _call(f"pm_{prefix}_static", ctypes.byref(di), int(se), int(it), ctypes.byref(do), _stream(img))
+    else:
+        # runtime neighbour list: a sequence of offset tuples ({dy,dx} / {dz,dy,dx}), applied once
+        offs = [int(v) for o in se for v in o]
+        n = len(se)
+        arr = (ctypes.c_int64 * max(1, len(offs)))(*offs)
+        _call(f"pm_{prefix}_nl", ctypes.byref(di), arr, n, ctypes.byref(do), _stream(img))
+    return out
+
+
+def erode(img, se, iter=1, out=None):
+    """poutre::Erode(img, Common_NL_SE | Compound_NL_SE, iter|size, out) (ero_dil.hpp:32-47).
+    A list of offset tuples selects the runtime neighbour-list overload (applied once)."""
+    return _erodil("erode", img, se, iter, out)
+
+
+def dilate(img, se, iter=1, out=None):
+    """poutre::Dilate(img, Common_NL_SE | Compound_NL_SE, iter|size, out) (ero_dil.hpp:32-47)."""
+    return _erodil("dilate", img, se, iter, out)
+
+
+def _line(name, img, r, out):
+    out = _out(img, out)
+    di, do = _desc(img), _desc(out)
+    _call(name, ctypes.byref(di), int(r), ctypes.byref(do), _stream(img))
+    return out
+
+
+def erodeX(img, size_half_segment, out=None):
+    """poutre::ErodeX (ero_dil_line.hpp:32)."""
+    return _line("pm_erode_x", img, size_half_segment, out)
+
+
+def dilateX(img, size_half_segment, out=None):
+    """poutre::DilateX (ero_dil_line.hpp:37)."""
+    return _line("pm_dilate_x", img, size_half_segment, out)
+
+
+def erodeY(img, size_half_segment, out=None):
+    """poutre::ErodeY (ero_dil_line.hpp:42); any H, W (the reference requires H == W)."""
+    return _line("pm_erode_y", img, size_half_segment, out)
+
+
+def dilateY(img, size_half_segment, out=None):
+    """poutre::DilateY (ero_dil_line.hpp:47); any H, W (the reference requires H == W)."""
+    return _line("pm_dilate_y", img, size_half_segment, out)
+
+
+def _rect(name, img, ry, rx, out):
+    out = _out(img, out)
+    di, do = _desc(img), _desc(out)
+    _call(name, ctypes.byref(di), int(ry), int(rx), ctypes.byref(do), _stream(img))
+    return out
+
+
+def erode_rect(img, ry, rx, out=None):
+    """Erosion by a (2ry+1) x (2rx+1) rectangle (ErodeX then ErodeY)."""
+    return _rect("pm_erode_rect", img, ry, rx, out)
+
+
+def dilate_rect(img, ry, rx, out=None):
+    """Dilation by a (2ry+1) x (2rx+1) rectangle (DilateX then DilateY)."""
+    return _rect("pm_dilate_rect", img, ry, rx, out)
+
+
+def _compose(name, img, se, it, out):
+    out = _out(img, out)
+    di, do = _desc(img), _desc(out)
+    _call(name, ctypes.byref(di), int(se), int(it), ctypes.byref(do), _stream(img))
+    return out
+
+
+def opening(img, se, iter=1, out=None):
+    """Dilate(Erode(img, se, iter), se, iter)."""
+    return _compose("pm_opening", img, se, iter, out)
+
+
+def closing(img, se, iter=1, out=None):
+    """Erode(Dilate(img, se, iter), se, iter)."""
+    return _compose("pm_closing", img, se, iter, out)
+
+
+def gradient(img, se, iter=1, out=None):
+    """Dilate(img, se, iter) - Erode(img, se, iter) (saturated)."""
+    return _compose("pm_gradient", img, se, iter, out)
+
+
+# ---------------------------------------------------------------- geodesy
+def reconstruction(rec_type, marker, mask, se, out=None):
+    """geo::Reconstruction(rec_type, marker, mask, se, out) (mreconstruct.hpp:37-41)."""
+    out = _out(mask, out)
+    dm, dk, do = _desc(marker), _desc(mask), _desc(out)
+    _call("pm_reconstruct", int(rec_type), ctypes.byref(dm), ctypes.byref(dk), int(se), ctypes.byref(do),
+          _stream(mask))
+    return out
+
+
+def _lev(name, ref, marker, se, out):
+    out = _out(ref, out)
+    dr, dm, do = _desc(ref), _desc(marker), _desc(out)
+    _call(name, ctypes.byref(dr), ctypes.byref(dm), int(se), ctypes.byref(do), _stream(ref))
+    return out
+
+
+def low_leveling(ref, marker, se, out=None):
+    """geo::low_leveling (leveling.hpp:37)."""
+    return _lev("pm_low_leveling", ref, marker, se, out)
+
+
+def high_leveling(ref, marker, se, out=None):
+    """geo::high_leveling (leveling.hpp:45)."""
+    return _lev("pm_high_leveling", ref, marker, se, out)
+
+
+def leveling(ref, marker, se, out=None):
+    """geo::leveling (leveling.hpp:52)."""
+    return _lev("pm_leveling", ref, marker, se, out)
+
+
+def _ext(name, img, value, se, out):
+    out = _out(img, out)
+    di, do = _desc(img), _desc(out)
+    _call(name, ctypes.byref(di), float(value), int(se), ctypes.byref(do), _stream(img))
+    return out
+
+
+def h_maxima(img, value, se, out=None):
+    """geo::h_maxima (extrema.hpp:36)."""
+    return _ext("pm_h_maxima", img, value, se, out)
+
+
+def h_minima(img, value, se, out=None):
+    """geo::h_minima (extrema.hpp:33)."""
+    return _ext("pm_h_minima", img, value, se, out)
+
+
+def h_convex(img, value, se, out=None):
+    """geo::h_convex (extrema.hpp:43)."""
+    return _ext("pm_h_convex", img, value, se, out)
+
+
+def h_concave(img, value, se, out=None):
+    """geo::h_concave (extrema.hpp:50)."""
+    return _ext("pm_h_concave", img, value, se, out)
+
+
+def dynamic_pseudo_opening(img, value, se, out=None):
+    """geo::dynamic_pseudo_opening (extrema.hpp:57)."""
+    return _ext("pm_dynamic_pseudo_opening", img, value, se, out)
+
+
+def dynamic_pseudo_closing(img, value, se, out=None):
+    """geo::dynamic_pseudo_closing (extrema.hpp:64)."""
+    return _ext("pm_dynamic_pseudo_closing", img, value, se, out)
+
+
+# ---------------------------------------------------------------- elementwise helpers
+def _ew2(name, a, b, out):
+    out = _out(a, out)
+    da, db, do = _desc(a), _desc(b), _desc(out)
+    _call(name, ctypes.byref(da), ctypes.byref(db), ctypes.byref(do), _stream(a))
+    return out
+
+
+def arith_sup(a, b, out=None):
+    return _ew2("pm_arith_sup", a, b, out)
+
+
+def arith_inf(a, b, out=None):
+    return _ew2("pm_arith_inf", a, b, out)
+
+
+def arith_ssub_image(a, b, out=None):
+    return _ew2("pm_arith_ssub_image", a, b, out)
+
+
+def _ewc(name, a, c, out):
+    out = _out(a, out)
+    da, do = _desc(a), _desc(out)
+    _call(name, ctypes.byref(da), float(c), ctypes.byref(do), _stream(a))
+    return out
+
+
+def arith_ssub_const(a, c, out=None):
+    return _ewc("pm_arith_ssub_const", a, c, out)
+
+
+def arith_sadd_const(a, c, out=None):
+    return _ewc("pm_arith_sadd_const", a, c, out)

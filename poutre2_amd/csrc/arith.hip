@@ -1,0 +1,190 @@
+// arith.hip -- elementwise helpers on the path (include/poutre/pixel_processing/arith.hpp:31-53,
+// arith_op_t.hpp:84-445): sup/inf of two images, saturated subtraction of images,
+// saturated addition/subtraction of a constant (clamped at numeric_limits<T>::min()/max(),
+// accumulated in a wider type), plus the opening / closing / gradient compositions.
+#include "arith.hpp"
+#include "erodil.hpp"
+
+namespace pm {
+
+// accumulation type of the saturated operations (TypeTraits<T>::accu_type)
+template <typename T> struct Accu { using type = T; };
+template <> struct Accu<uint8_t> { using type = int32_t; };
+template <> struct Accu<uint16_t> { using type = int32_t; };
+template <> struct Accu<int32_t> { using type = int64_t; };
+template <> struct Accu<float> { using type = double; };
+
+template <typename T> __device__ __forceinline__ T ssub_sat(T a, T b)
+{
+  using A = typename Accu<T>::type;
+  const A r = (A)a - (A)b;
+  return r < (A)Lim<T>::tmin() ? Lim<T>::tmin() : (T)r;
+}
+template <typename T> __device__ __forceinline__ T sadd_sat(T a, T b)
+{
+  using A = typename Accu<T>::type;
+  const A r = (A)a + (A)b;
+  return r > (A)Lim<T>::highest() ? Lim<T>::highest() : (T)r;
+}
+
+template <typename T, int OP>
+__global__ __launch_bounds__(256) void ew2_kernel(const T *__restrict__ a, const T *__restrict__ b, T *__restrict__ o,
+                                                  int64_t n)
+{
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const T x = a[i], y = b[i];
+    T r;
+    if (OP == EW_SUP) r = x > y ? x : y;
+    else if (OP == EW_INF) r = x < y ? x : y;
+    else r = ssub_sat<T>(x, y);
+    o[i] = r;
+  }
+}
+
+template <typename T, int OP>
+__global__ __launch_bounds__(256) void ewc_kernel(const T *__restrict__ a, T c, T *__restrict__ o, int64_t n)
+{
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    o[i] = (OP == EWC_SSUB) ? ssub_sat<T>(a[i], c) : sadd_sat<T>(a[i], c);
+}
+
+// marker of dynamic_pseudo_opening/closing (extrema.cpp:90-93,110-113) in one pass:
+// mask = (ext == tmp) ? img : E;  marker = (ext != E) ? mask : E, E = lowest (opening) / highest
+template <typename T, bool OPENING>
+__global__ __launch_bounds__(256) void dpo_kernel(const T *__restrict__ ext, const T *__restrict__ tmp,
+                                                  const T *__restrict__ img, T *__restrict__ o, int64_t n)
+{
+  const T E = OPENING ? Lim<T>::lowest() : Lim<T>::highest();
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const T e = ext[i];
+    const T m = (e == tmp[i]) ? img[i] : E;
+    o[i] = (e != E) ? m : E;
+  }
+}
+
+template <typename T, int OP> int ew2(const T *a, const T *b, T *o, int64_t n, hipStream_t s)
+{
+  if (n == 0) return PM_OK;
+  hipLaunchKernelGGL((ew2_kernel<T, OP>), dim3(grid1d(n, 256, 8192)), dim3(256), 0, s, a, b, o, n);
+  PM_HIP(hipGetLastError());
+  return PM_OK;
+}
+
+template <typename T, int OP> int ewc(const T *a, T c, T *o, int64_t n, hipStream_t s)
+{
+  if (n == 0) return PM_OK;
+  hipLaunchKernelGGL((ewc_kernel<T, OP>), dim3(grid1d(n, 256, 8192)), dim3(256), 0, s, a, c, o, n);
+  PM_HIP(hipGetLastError());
+  return PM_OK;
+}
+
+template <typename T, bool OPENING> int dpo_marker(const T *ext, const T *tmp, const T *img, T *o, int64_t n, hipStream_t s)
+{
+  if (n == 0) return PM_OK;
+  hipLaunchKernelGGL((dpo_kernel<T, OPENING>), dim3(grid1d(n, 256, 8192)), dim3(256), 0, s, ext, tmp, img, o, n);
+  PM_HIP(hipGetLastError());
+  return PM_OK;
+}
+
+#define PM_INST(T)                                                                                           \
+  template int ew2<T, EW_SUP>(const T *, const T *, T *, int64_t, hipStream_t);                              \
+  template int ew2<T, EW_INF>(const T *, const T *, T *, int64_t, hipStream_t);                              \
+  template int ew2<T, EW_SSUB>(const T *, const T *, T *, int64_t, hipStream_t);                             \
+  template int ewc<T, EWC_SSUB>(const T *, T, T *, int64_t, hipStream_t);                                    \
+  template int ewc<T, EWC_SADD>(const T *, T, T *, int64_t, hipStream_t);                                    \
+  template int dpo_marker<T, true>(const T *, const T *, const T *, T *, int64_t, hipStream_t);              \
+  template int dpo_marker<T, false>(const T *, const T *, const T *, T *, int64_t, hipStream_t);
+PM_INST(uint8_t)
+PM_INST(uint16_t)
+PM_INST(int32_t)
+PM_INST(int64_t)
+PM_INST(float)
+PM_INST(double)
+#undef PM_INST
+
+template <int OP> static int arith2(const pm_image *a, const pm_image *b, pm_image *out, void *stream, const char *what)
+{
+  clear_error();
+  PM_TRY(check_pair(a, b, what));
+  PM_TRY(check_pair(b, out, what));
+  const int64_t n = numel(*a);
+  auto f = [a = a, b = b, o = out, n = n, s = as_stream(stream)](auto tag) -> int {
+      using T = typename decltype(tag)::type; return ew2<T, OP>((const T *)a->data, (const T *)b->data, (T *)o->data, n, s); };
+  return dispatch_ptype(a->ptype, f);
+}
+
+template <int OP> static int arithc(const pm_image *a, double c, pm_image *out, void *stream, const char *what)
+{
+  clear_error();
+  PM_TRY(check_pair(a, out, what));
+  const int64_t n = numel(*a);
+  auto f = [a = a, c = c, o = out, n = n, s = as_stream(stream)](auto tag) -> int {
+      using T = typename decltype(tag)::type; return ewc<T, OP>((const T *)a->data, (T)c, (T *)o->data, n, s); };
+  return dispatch_ptype(a->ptype, f);
+}
+
+// ---------------------------------------------------------------- open / close / gradient
+enum { C_OPEN, C_CLOSE, C_GRAD };
+
+template <int K> static int compose(const pm_image *in, int se, int iter, pm_image *out, void *stream)
+{
+  const char *what = K == C_OPEN ? "Opening" : K == C_CLOSE ? "Closing" : "Gradient";
+  PM_TRY(check_unary(in, out, what));
+  if (iter < 0) PM_FAIL(PM_E_ARG, "%s iter must be >= 0", what);
+  hipStream_t s = as_stream(stream);
+  const int64_t n = numel(*in);
+  if (n == 0) return PM_OK;
+  if (iter == 0) {
+    if (K == C_GRAD) PM_HIP(hipMemsetAsync(out->data, 0, (size_t)n * ptype_size(in->ptype), s));
+    else PM_HIP(hipMemcpyAsync(out->data, in->data, (size_t)n * ptype_size(in->ptype), hipMemcpyDeviceToDevice, s));
+    return PM_OK;
+  }
+  PM_TRY(check_se_rank(se, in->rank));
+  const Dims d = dims3(*in);
+  auto f = [in = in, out = out, d = d, rank = in->rank, se = se, iter = iter, s = s, n = n](auto tag) -> int {
+      using T = typename decltype(tag)::type;
+      const T *src = (const T *)in->data;
+      T *dst = (T *)out->data;
+      Scratch tmp;
+      PM_TRY(tmp.alloc((size_t)n * sizeof(T), s));
+      T *t = (T *)tmp.ptr;
+      if (K == C_OPEN) {
+        PM_TRY((static_iter<T, false>(src, t, d, rank, se, iter, s)));
+        return static_iter<T, true>(t, dst, d, rank, se, iter, s);
+      } else if (K == C_CLOSE) {
+        PM_TRY((static_iter<T, true>(src, t, d, rank, se, iter, s)));
+        return static_iter<T, false>(t, dst, d, rank, se, iter, s);
+      } else {
+        Scratch tmp2;
+        PM_TRY(tmp2.alloc((size_t)n * sizeof(T), s));
+        T *e = (T *)tmp2.ptr;
+        PM_TRY((static_iter<T, true>(src, t, d, rank, se, iter, s)));
+        PM_TRY((static_iter<T, false>(src, e, d, rank, se, iter, s)));
+        return ew2<T, EW_SSUB>(t, e, dst, n, s);
+      }
+    };
+  return dispatch_ptype(in->ptype, f);
+}
+
+}  // namespace pm
+
+using namespace pm;
+
+extern "C" {
+int pm_arith_sup(const pm_image *a, const pm_image *b, pm_image *out, void *stream)
+{ return arith2<EW_SUP>(a, b, out, stream, "ArithSupImage"); }
+int pm_arith_inf(const pm_image *a, const pm_image *b, pm_image *out, void *stream)
+{ return arith2<EW_INF>(a, b, out, stream, "ArithInfImage"); }
+int pm_arith_ssub_image(const pm_image *a, const pm_image *b, pm_image *out, void *stream)
+{ return arith2<EW_SSUB>(a, b, out, stream, "ArithSaturatedSubImage"); }
+int pm_arith_ssub_const(const pm_image *in, double c, pm_image *out, void *stream)
+{ return arithc<EWC_SSUB>(in, c, out, stream, "ArithSaturatedSubConstant"); }
+int pm_arith_sadd_const(const pm_image *in, double c, pm_image *out, void *stream)
+{ return arithc<EWC_SADD>(in, c, out, stream, "ArithSaturatedAddConstant"); }
+int pm_opening(const pm_image *in, int se, int iter, pm_image *out, void *stream)
+{ return compose<C_OPEN>(in, se, iter, out, stream); }
+int pm_closing(const pm_image *in, int se, int iter, pm_image *out, void *stream)
+{ return compose<C_CLOSE>(in, se, iter, out, stream); }
+int pm_gradient(const pm_image *in, int se, int iter, pm_image *out, void *stream)
+{ return compose<C_GRAD>(in, se, iter, out, stream); }
+}

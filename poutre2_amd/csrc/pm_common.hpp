@@ -1,0 +1,145 @@
+// pm_common.hpp -- shared host/device helpers of the gfx950 morphology library:
+// element traits (neutral values of dilation/erosion), the PType dispatch, argument
+// validation in the reference's order, the thread-local error message and the
+// stream-ordered scratch allocation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+#include <cstdio>
+#include <limits>
+#include <string>
+
+#include "../../include/poutre_amd.h"
+
+namespace pm {
+
+// ---------------------------------------------------------------- errors
+void set_error(const char *fmt, ...);
+void clear_error();
+
+struct Status {
+  int code;
+};
+
+#define PM_FAIL(code, ...)                                                                                   \
+  do {                                                                                                       \
+    ::pm::set_error(__VA_ARGS__);                                                                            \
+    return (code);                                                                                           \
+  } while (0)
+
+#define PM_HIP(expr)                                                                                         \
+  do {                                                                                                       \
+    hipError_t e_ = (expr);                                                                                  \
+    if (e_ != hipSuccess) PM_FAIL(PM_E_HIP, "HIP error %s at %s:%d (%s)", hipGetErrorString(e_), __FILE__,   \
+                                  __LINE__, #expr);                                                          \
+  } while (0)
+
+#define PM_TRY(expr)                                                                                         \
+  do {                                                                                                       \
+    int rc_ = (expr);                                                                                        \
+    if (rc_ != PM_OK) return rc_;                                                                            \
+  } while (0)
+
+// ---------------------------------------------------------------- element traits
+// Neutral of dilation = numeric_limits<T>::lowest(), of erosion = max()
+// (ero_dil_line_se_t.hpp:37,44; types_traits.hpp:114-263).  Float neutrals are +-FLT_MAX.
+template <typename T> struct Lim;
+template <> struct Lim<uint8_t> { static constexpr __host__ __device__ uint8_t lowest() { return 0; } static constexpr __host__ __device__ uint8_t highest() { return 255; } static constexpr __host__ __device__ uint8_t tmin() { return 0; } };
+template <> struct Lim<uint16_t> { static constexpr __host__ __device__ uint16_t lowest() { return 0; } static constexpr __host__ __device__ uint16_t highest() { return 65535; } static constexpr __host__ __device__ uint16_t tmin() { return 0; } };
+template <> struct Lim<int32_t> { static constexpr __host__ __device__ int32_t lowest() { return INT32_MIN; } static constexpr __host__ __device__ int32_t highest() { return INT32_MAX; } static constexpr __host__ __device__ int32_t tmin() { return INT32_MIN; } };
+template <> struct Lim<int64_t> { static constexpr __host__ __device__ int64_t lowest() { return INT64_MIN; } static constexpr __host__ __device__ int64_t highest() { return INT64_MAX; } static constexpr __host__ __device__ int64_t tmin() { return INT64_MIN; } };
+template <> struct Lim<float> { static constexpr __host__ __device__ float lowest() { return -FLT_MAX; } static constexpr __host__ __device__ float highest() { return FLT_MAX; } static constexpr __host__ __device__ float tmin() { return FLT_MIN; } };
+template <> struct Lim<double> { static constexpr __host__ __device__ double lowest() { return -DBL_MAX; } static constexpr __host__ __device__ double highest() { return DBL_MAX; } static constexpr __host__ __device__ double tmin() { return DBL_MIN; } };
+
+// The lattice operation of a dilation (sup) or an erosion (inf).  Written as a select
+// so that integer and float types share one definition (`a > b ? a : b` is the
+// reference's scalar form, arith_op_t.hpp:363,414).
+template <typename T, bool DIL> struct Lat {
+  static __host__ __device__ __forceinline__ T op(T a, T b) { return DIL ? (a > b ? a : b) : (a < b ? a : b); }
+  static __host__ __device__ __forceinline__ T neutral() { return DIL ? Lim<T>::lowest() : Lim<T>::highest(); }
+  // "a is strictly better than b" in the direction of the operation
+  static __host__ __device__ __forceinline__ bool better(T a, T b) { return DIL ? (a > b) : (a < b); }
+};
+
+inline size_t ptype_size(int ptype)
+{
+  switch (ptype) {
+  case PM_PTYPE_U8: return 1;
+  case PM_PTYPE_U16: return 2;
+  case PM_PTYPE_I32: return 4;
+  case PM_PTYPE_F32: return 4;
+  case PM_PTYPE_I64: return 8;
+  case PM_PTYPE_D64: return 8;
+  default: return 0;
+  }
+}
+
+template <typename T> struct Tag { using type = T; };
+
+// Calls f(Tag<T>{}) for the image's element type (f is typically a generic lambda).
+template <typename F> int dispatch_ptype(int ptype, F &&f)
+{
+  switch (ptype) {
+  case PM_PTYPE_U8: return f(Tag<uint8_t>{});
+  case PM_PTYPE_U16: return f(Tag<uint16_t>{});
+  case PM_PTYPE_I32: return f(Tag<int32_t>{});
+  case PM_PTYPE_F32: return f(Tag<float>{});
+  case PM_PTYPE_I64: return f(Tag<int64_t>{});
+  case PM_PTYPE_D64: return f(Tag<double>{});
+  default: PM_FAIL(PM_E_PTYPE, "Unsupported PType %d", ptype);
+  }
+}
+
+// ---------------------------------------------------------------- images
+inline int64_t numel(const pm_image &im)
+{
+  int64_t n = 1;
+  for (int a = 0; a < im.rank; a++) n *= im.shape[a];
+  return n;
+}
+
+// Logical 3D view {D, H, W} of a rank-1..3 image (rank 1: D = H = 1).
+struct Dims {
+  int64_t D, H, W;
+};
+inline Dims dims3(const pm_image &im)
+{
+  if (im.rank == 1) return {1, 1, im.shape[0]};
+  if (im.rank == 2) return {1, im.shape[0], im.shape[1]};
+  return {im.shape[0], im.shape[1], im.shape[2]};
+}
+
+// AssertSizesCompatible / AssertAsTypesCompatible / AssertImagesAreDifferent
+// (include/poutre/base/image_interface.hpp:95-102), then rank 1..3.
+int check_pair(const pm_image *a, const pm_image *b, const char *what);
+int check_unary(const pm_image *in, const pm_image *out, const char *what);
+int check_binary(const pm_image *a, const pm_image *b, const pm_image *out, const char *what);
+
+// ---------------------------------------------------------------- scratch
+// Stream-ordered device scratch: allocated with hipMallocAsync on the call's stream and
+// released with hipFreeAsync on the same stream, so that it never races with the kernels
+// of that stream that use it.
+struct Scratch {
+  void *ptr = nullptr;
+  hipStream_t stream = nullptr;
+  Scratch() = default;
+  Scratch(const Scratch &) = delete;
+  Scratch &operator=(const Scratch &) = delete;
+  int alloc(size_t bytes, hipStream_t s);
+  ~Scratch();
+};
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline unsigned grid1d(int64_t n, int block, int64_t cap = 1 << 20)
+{
+  int64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+}  // namespace pm
