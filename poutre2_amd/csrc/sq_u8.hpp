@@ -1,0 +1,14 @@
+// sq_u8.hpp -- fused uint8 square erosion/dilation (sq_u8.hip).
+#pragma once
+#include "pm_common.hpp"
+
+namespace pm {
+
+constexpr int64_t SQ_U8_RMAX = 32;  // largest radius handled by one launch
+
+// True if the fused kernel can run on these buffers (16-byte aligned rows).
+bool sq_u8_applicable(const void *in, const void *out, int64_t H, int64_t W);
+// (2r+1)^2 square on B images of H x W; r > SQ_U8_RMAX runs several launches.
+template <bool DIL> int sq_u8(const uint8_t *in, uint8_t *out, int64_t B, int64_t H, int64_t W, int64_t r, hipStream_t s);
+
+}  // namespace pm

@@ -272,3 +272,32 @@ def test_errors_follow_reference_checks():
     assert e.value.code == -6
     with pytest.raises(pm.PoutreError):
         pm.dilateY(g(rand((10,), np.uint8, 2)), 1)
+
+
+# ----------------------------------------------------------------------------- fused uint8 square kernel
+@pytest.mark.parametrize("shape", [(256, 272), (130, 48), (517, 1008), (16, 16), (300, 16)])
+@pytest.mark.parametrize("r", [1, 2, 3, 4, 7, 8, 15, 16, 31, 32, 33, 70])
+def test_square_u8_fused(shape, r):
+    img = rand(shape, np.uint8, 60 + r)
+    for dil in (True, False):
+        f = pm.dilate if dil else pm.erode
+        exp = oracle.static_iter(oracle.line_x(img, r, dil), "SESegmentY2D", min(r, shape[0]), dil)
+        np.testing.assert_array_equal(h(f(g(img), pm.static_nl_se.square2d, r)), exp)
+
+
+@pytest.mark.parametrize("it", [1, 2, 5])
+def test_square3d_u8_fused(it):
+    img = rand((6, 40, 48), np.uint8, 70)
+    for dil in (True, False):
+        f = pm.dilate if dil else pm.erode
+        np.testing.assert_array_equal(h(f(g(img), pm.static_nl_se.square3d, it)),
+                                      oracle.static_iter(img, "SESquare3D", it, dil))
+
+
+def test_square_u8_unaligned_view_falls_back():
+    """A view whose data pointer is not 16-byte aligned takes the generic path, same bits."""
+    base = torch.from_numpy(rand((65, 64), np.uint8, 71)).to(DEV)
+    view = base.view(-1)[1:1 + 64 * 64].view(64, 64)
+    img = view.cpu().numpy()
+    np.testing.assert_array_equal(h(pm.dilate(view, pm.static_nl_se.square2d, 3)),
+                                  oracle.static_iter(img, "SESquare2D", 3, True))
