@@ -1,0 +1,150 @@
+// api_kats.cpp -- the reference's known-answer tests run through the C++ API (poutre_amd.hpp),
+// the way the reference's own Catch2 tests call it: images built with ImageFromString, the
+// operator called on IInterface references, the result compared with ImageToString.
+// Each case runs twice: on host images (staged through HBM) and on device images.
+//
+// usage: api_kats <cases.txt>   one case per line: op|se|rec|r|h|in1|in2|expected
+#include <cstdio>
+#include <fstream>
+#include <functional>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "poutre_amd.hpp"
+
+using namespace poutre;
+
+static std::vector<std::string> split(const std::string &s, char c)
+{
+  std::vector<std::string> out;
+  std::string cur;
+  for (char ch : s) {
+    if (ch == c) { out.push_back(cur); cur.clear(); }
+    else cur += ch;
+  }
+  out.push_back(cur);
+  return out;
+}
+
+static se::Common_NL_SE se_of(const std::string &n)
+{
+  static const char *names[] = {"Undef", "SESegmentX1D", "SESquare2D", "SECross2D", "SESegmentX2D", "SESegmentY2D",
+                                "SESegmentX3D", "SESegmentY3D", "SESegmentZ3D", "SECross3D", "SESquare3D"};
+  for (int i = 0; i < 11; i++)
+    if (n == names[i]) return (se::Common_NL_SE)i;
+  return se::Common_NL_SE::Undef;
+}
+
+static std::unique_ptr<IInterface> on(bool dev, const std::string &s)
+{
+  auto h = ImageFromString(s);
+  if (!dev) return h;
+  auto d = CreateDevice(h->GetShape(), h->GetPType());
+  CopyInto(*h, *d);
+  return d;
+}
+
+static int run_case(const std::vector<std::string> &f, bool dev)
+{
+  const std::string &op = f[0];
+  const auto se = se_of(f[1]);
+  auto a = on(dev, f[5]);
+  std::unique_ptr<IInterface> b = f[6].empty() ? nullptr : on(dev, f[6]);
+  auto out = CloneGeometry(*a);
+  const int r = f[3].empty() ? 0 : std::stoi(f[3]);
+  const ScalarTypeVariant h = (pINT64)(f[4].empty() ? 0 : std::stoll(f[4]));
+  if (op == "t_Dilate") Dilate(*a, se, 1, *out);
+  else if (op == "t_Erode") Erode(*a, se, 1, *out);
+  else if (op == "rt_Dilate" || op == "rt_Erode") {
+    static const se::neighbor_list *nls[] = {nullptr, &se::SESegmentX1D, &se::SESquare2D, &se::SECross2D,
+                                             &se::SESegmentX2D, &se::SESegmentY2D, &se::SESegmentX3D, &se::SESegmentY3D,
+                                             &se::SESegmentZ3D, &se::SECross3D, &se::SESquare3D};
+    if (op == "rt_Dilate") Dilate(*a, *nls[(int)se], *out);
+    else Erode(*a, *nls[(int)se], *out);
+  }
+  else if (op == "t_DilateX") DilateX(*a, r, *out);
+  else if (op == "t_ErodeX") ErodeX(*a, r, *out);
+  else if (op == "t_DilateY") DilateY(*a, r, *out);
+  else if (op == "t_ErodeY") ErodeY(*a, r, *out);
+  else if (op == "Reconstruction")
+    geo::Reconstruction(f[2] == "dilate" ? geo::reconstruction_type::dilate : geo::reconstruction_type::erode, *a, *b,
+                        se, *out);
+  else if (op == "low_leveling") geo::low_leveling(*a, *b, se, *out);
+  else if (op == "high_leveling") geo::high_leveling(*a, *b, se, *out);
+  else if (op == "leveling") geo::leveling(*a, *b, se, *out);
+  else if (op == "h_minima") geo::h_minima(*a, h, se, *out);
+  else if (op == "h_maxima") geo::h_maxima(*a, h, se, *out);
+  else if (op == "h_convex") geo::h_convex(*a, h, se, *out);
+  else if (op == "h_concave") geo::h_concave(*a, h, se, *out);
+  else if (op == "dynamic_pseudo_opening") geo::dynamic_pseudo_opening(*a, h, se, *out);
+  else if (op == "dynamic_pseudo_closing") geo::dynamic_pseudo_closing(*a, h, se, *out);
+  else return 2;
+  const std::string got = ImageToString(*out);
+  const std::string exp = ImageToString(*ImageFromString(f[7]));
+  if (got != exp) {
+    std::printf("FAIL %s %s dev=%d\n  got: %s\n  exp: %s\n", op.c_str(), f[1].c_str(), (int)dev, got.c_str(),
+                exp.c_str());
+    return 1;
+  }
+  return 0;
+}
+
+static int expect_throw(const char *what, const std::function<void()> &fn)
+{
+  try {
+    fn();
+  } catch (const std::runtime_error &e) {
+    if (std::string(e.what()).rfind("Throw ", 0) == 0) return 0;
+    std::printf("FAIL %s: unexpected message %s\n", what, e.what());
+    return 1;
+  }
+  std::printf("FAIL %s: no exception\n", what);
+  return 1;
+}
+
+int main(int argc, char **argv)
+{
+  if (argc < 2) return 2;
+  std::ifstream in(argv[1]);
+  std::string line;
+  int n = 0, fails = 0, skipped = 0;
+  while (std::getline(in, line)) {
+    if (line.empty()) continue;
+    auto f = split(line, '|');
+    if (f.size() != 8) return 3;
+    for (bool dev : {false, true}) {
+      try {
+        const int rc = run_case(f, dev);
+        if (rc == 2) { skipped++; continue; }
+        fails += rc;
+        n++;
+      } catch (const std::exception &e) {
+        std::printf("FAIL %s: exception %s\n", f[0].c_str(), e.what());
+        fails++;
+        n++;
+      }
+    }
+  }
+  // error behaviour of the reference's checks
+  auto a = Create({4, 4}, CompoundType::CompoundType_Scalar, PType::PType_GrayUINT8);
+  auto b = Create({4, 5}, CompoundType::CompoundType_Scalar, PType::PType_GrayUINT8);
+  auto c = Create({4, 4}, CompoundType::CompoundType_Scalar, PType::PType_GrayINT32);
+  fails += expect_throw("size mismatch", [&] { Dilate(*a, se::Common_NL_SE::SESquare2D, 1, *b); });
+  fails += expect_throw("type mismatch", [&] { Dilate(*a, se::Common_NL_SE::SESquare2D, 1, *c); });
+  fails += expect_throw("aliasing", [&] { Dilate(*a, se::Common_NL_SE::SESquare2D, 1, *a); });
+  fails += expect_throw("negative iter", [&] { Erode(*a, se::Common_NL_SE::SESquare2D, -1, *CloneGeometry(*a)); });
+  fails += expect_throw("rank mismatch", [&] { Erode(*a, se::Common_NL_SE::SESquare3D, 1, *CloneGeometry(*a)); });
+  auto ns = Create({4, 6}, CompoundType::CompoundType_Scalar, PType::PType_GrayUINT8);
+  fails += expect_throw("non-square DilateY", [&] { DilateY(*ns, 1, *CloneGeometry(*ns)); });
+  // SE model (test/structuring_element/neighbor_list_se_t.cpp:20-69)
+  if (!se::SESquare2D.remove_center().is_symmetric() || se::SESquare2D.remove_center().get_coordinates().size() != 8) fails++;
+  if (!se::SESquare2D.is_equal_unordered(se::SESquare2D.transpose())) fails++;
+  auto [up, lo] = se::SESquare3D.split_upper_lower();
+  if (up.is_symmetric() || lo.is_symmetric()) fails++;
+  if (se::SESquare2D.maximum_extension(0, true) != 1 || se::SESquare2D.maximum_extension(1, false) != 1) fails++;
+  if (se::SECross3D.GetSize() != 7 || se::SESquare3D.GetSize() != 27) fails++;
+  std::printf("api_kats: %d cases run (%d skipped), %d failures\n", n, skipped, fails);
+  return fails ? 1 : 0;
+}
