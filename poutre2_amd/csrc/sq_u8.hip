@@ -19,8 +19,12 @@
 // K (run length) is the largest power of two <= min(2r, 32): the suffix covers rows
 // [j, K) and the prefix rows [K, j+2r] of output j, which needs K <= 2r + 1.  For r = 1
 // a direct 3-tap form with K = 4 is used.
-// Tiles are mapped XCD-aware: workgroup b runs on XCD b % 8, so consecutive tiles of a
-// band go to the same XCD and share their halo rows in that XCD's L2.
+// The transposed buffer T reuses the LDS of the region A (results of the vertical pass
+// are held in registers across a barrier), so a workgroup needs max(|A|, |T|) bytes.
+// The kernel is persistent (as many workgroups as fit on the chip, each looping over
+// tiles) and prefetches the next tile's region into registers while computing the
+// current one.  Tiles are mapped XCD-aware: each XCD walks a contiguous tile range, so
+// neighbouring tiles share their halo rows in that XCD's L2.
 #include "pm_common.hpp"
 #include "sq_u8.hpp"
 
@@ -76,14 +80,19 @@ __device__ __forceinline__ void vh_run(const uint8_t *base, int stride, int r, u
       e[j] = pk<DIL>(e[j], e[j + 1]);
       o[j] = pk<DIL>(o[j], o[j + 1]);
     }
+    // prefix over rows [K, 2r): at most K rows (2r <= 2K for every radius mapped to K).
+    // The loop is unrolled over K rows and predicated, so that its LDS reads issue back to
+    // back instead of one per dependent iteration; rows past 2r-1 are read (they lie inside
+    // the region since K <= 2r and K <= R16 + r) and ignored.
     uint32_t pe = NEU, po = NEU;
-    const uint8_t *p = base + K * stride;
-    for (int t = K; t < 2 * r; t++, p += stride) {
-      const uint32_t w = lds_word(p);
-      pe = pk<DIL>(pe, even_pairs(w));
-      po = pk<DIL>(po, odd_pairs(w));
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+      const uint32_t w = lds_word(base + (K + t) * stride);
+      const bool use = K + t < 2 * r;
+      pe = use ? pk<DIL>(pe, even_pairs(w)) : pe;
+      po = use ? pk<DIL>(po, odd_pairs(w)) : po;
     }
-    p = base + 2 * r * stride;
+    const uint8_t *p = base + 2 * r * stride;
 #pragma unroll
     for (int j = 0; j < K; j++) {
       const uint32_t w = lds_word(p + j * stride);
@@ -93,40 +102,34 @@ __device__ __forceinline__ void vh_run(const uint8_t *base, int stride, int r, u
       o[j] = pk<DIL>(o[j], po);
     }
   } else if constexpr (K == 4) {
-    // direct taps for 2r < K (only r = 1 with K = 4 reaches here)
-    uint32_t xe[K + 3], xo[K + 3];
+    // r = 1 (2r < K): direct three-tap form over six rows
+    uint32_t xe[6], xo[6];
 #pragma unroll
-    for (int t = 0; t < K + 3; t++) {
-      if (t < K + 2 * r) {
-        const uint32_t w = lds_word(base + t * stride);
-        xe[t] = even_pairs(w);
-        xo[t] = odd_pairs(w);
-      } else {
-        xe[t] = NEU;
-        xo[t] = NEU;
-      }
+    for (int t = 0; t < 6; t++) {
+      const uint32_t w = lds_word(base + t * stride);
+      xe[t] = even_pairs(w);
+      xo[t] = odd_pairs(w);
     }
 #pragma unroll
     for (int j = 0; j < K; j++) {
-      uint32_t ae = xe[j], ao = xo[j];
-#pragma unroll
-      for (int d = 1; d <= 3; d++)
-        if (d <= 2 * r) {
-          ae = pk<DIL>(ae, xe[j + d]);
-          ao = pk<DIL>(ao, xo[j + d]);
-        }
-      e[j] = ae;
-      o[j] = ao;
+      e[j] = pk<DIL>(pk<DIL>(xe[j], xe[j + 1]), xe[j + 2]);
+      o[j] = pk<DIL>(pk<DIL>(xo[j], xo[j + 1]), xo[j + 2]);
     }
   }
 }
 
-__device__ __forceinline__ int xcd_tile(int b, int ntiles)
+// Persistent tile schedule.  Workgroups are dispatched round-robin over the 8 XCDs
+// (workgroup b on XCD b % 8); XCD x owns the contiguous tile range [x*n/8, (x+1)*n/8) so
+// that neighbouring tiles, which share halo rows, are read through the same L2.
+struct TileSched {
+  int first, step, end;
+};
+__device__ __forceinline__ TileSched tile_sched(int b, int G, int ntiles)
 {
-  // workgroups are dispatched round-robin over the 8 XCDs; give each XCD a contiguous
-  // range of tiles so that neighbouring tiles share halo rows in one L2
-  if ((ntiles & 7) != 0) return b;
-  return (b & 7) * (ntiles >> 3) + (b >> 3);
+  if ((G & 7) != 0 || ntiles < 8) return {b, G, ntiles};
+  const int x = b & 7, j = b >> 3, gx = G >> 3;
+  const int lo = (int)((int64_t)x * ntiles / 8), hi = (int)((int64_t)(x + 1) * ntiles / 8);
+  return {lo + j, gx, hi};
 }
 
 template <int K, bool DIL>
@@ -134,54 +137,94 @@ __global__ __launch_bounds__(FTHREADS) void sq_u8_kernel(const uint8_t *__restri
                                                          int H, int W, int r, int R16, int tiles_x, int tiles_per_img,
                                                          int ntiles)
 {
+  // region chunks (16 bytes) per thread: enough for the largest radius of each K
+  constexpr int MAXC = (K == 32) ? 9 : (K == 16) ? 7 : 6;
   extern __shared__ __align__(16) uint8_t smem[];
   const int AW = FT + 2 * R16;  // region width in bytes (multiple of 16)
   const int AH = FT + 2 * r;    // region height
   const int TS = FT + TPAD;     // transposed row pitch
   uint8_t *A = smem;
-  uint8_t *T = smem + AH * AW;
+  uint8_t *T = smem;  // the transposed buffer reuses A's storage once A has been consumed
   const int tid = threadIdx.x;
+  const int cpr = AW >> 4;
+  const int nchunk = AH * cpr;
+  const uint32_t nv = DIL ? 0u : 0xFFFFFFFFu;
 
-  const int tile = xcd_tile(blockIdx.x, ntiles);
-  const int img = tile / tiles_per_img;
-  const int t_in = tile - img * tiles_per_img;
-  const int ty = t_in / tiles_x, tx = t_in - ty * tiles_x;
-  const int y0 = ty * FT, x0 = tx * FT;
-  const int64_t ibase = (int64_t)img * H * W;
-
-  // ---- 1. region -> LDS (W is a multiple of 16, so a 16-byte chunk is all in or all out)
-  {
-    const uint32_t nv = DIL ? 0u : 0xFFFFFFFFu;
-    const uint4 neutral = make_uint4(nv, nv, nv, nv);
-    const int cpr = AW >> 4;
-    const int n = AH * cpr;
-    for (int c = tid; c < n; c += FTHREADS) {
-      const int i = c / cpr, cc = c - i * cpr;
-      const int gy = y0 - r + i, gx = x0 - R16 + (cc << 4);
-      uint4 v = neutral;
-      if ((unsigned)gy < (unsigned)H && (unsigned)gx < (unsigned)W)
-        v = *reinterpret_cast<const uint4 *>(in + ibase + (int64_t)gy * W + gx);
-      *reinterpret_cast<uint4 *>(A + i * AW + (cc << 4)) = v;
+  // The next tile's region is fetched into registers while the current tile is computed,
+  // so the HBM reads of a workgroup overlap its own LDS work.
+  uint4 pre[MAXC];
+  auto fetch = [&](int tile) {
+    const int img = tile / tiles_per_img;
+    const int t_in = tile - img * tiles_per_img;
+    const int ty = t_in / tiles_x, tx = t_in - ty * tiles_x;
+    const uint8_t *src = in + (int64_t)img * H * W;
+#pragma unroll
+    for (int i = 0; i < MAXC; i++) {
+      const int c = tid + i * FTHREADS;
+      pre[i] = make_uint4(nv, nv, nv, nv);
+      if (c < nchunk) {
+        const int ri = c / cpr, cc = c - ri * cpr;
+        const int gy = ty * FT - r + ri, gx = tx * FT - R16 + (cc << 4);
+        // W is a multiple of 16, so a 16-byte chunk is all inside or all outside
+        if ((unsigned)gy < (unsigned)H && (unsigned)gx < (unsigned)W)
+          pre[i] = *reinterpret_cast<const uint4 *>(src + (int64_t)gy * W + gx);
+      }
     }
-  }
-  __syncthreads();
+  };
 
-  // ---- 2. vertical pass, written transposed into T (T row = region column)
+  const TileSched ts = tile_sched(blockIdx.x, gridDim.x, ntiles);
+  if (ts.first < ts.end) fetch(ts.first);
+  for (int tile = ts.first; tile < ts.end; tile += ts.step) {
+    const int img = tile / tiles_per_img;
+    const int t_in = tile - img * tiles_per_img;
+    const int ty = t_in / tiles_x, tx = t_in - ty * tiles_x;
+    const int y0 = ty * FT, x0 = tx * FT;
+    const int64_t ibase = (int64_t)img * H * W;
+
+    // ---- 1. prefetched region -> LDS, then start fetching the next tile
+#pragma unroll
+    for (int i = 0; i < MAXC; i++) {
+      const int c = tid + i * FTHREADS;
+      if (c < nchunk) {
+        const int ri = c / cpr, cc = c - ri * cpr;
+        *reinterpret_cast<uint4 *>(A + ri * AW + (cc << 4)) = pre[i];
+      }
+    }
+    __syncthreads();
+    if (tile + ts.step < ts.end) fetch(tile + ts.step);
+
+  // ---- 2. vertical pass.  Every thread keeps the (transposed) results of its at most
+  // MAXT tasks in registers; after a barrier they are written into T, which reuses the
+  // storage of A.  Aliasing the two buffers halves the LDS footprint and doubles the
+  // number of resident workgroups.
   {
+    constexpr int MAXT = (K == 4) ? 5 : (K == 8) ? 3 : (K == 16) ? 2 : 1;
     const int q0 = (R16 - r) >> 2, q1 = (R16 + FT + r + 3) >> 2;
     const int nq = q1 - q0;
     const int ntask = nq * (FT / K);
-    for (int task = tid; task < ntask; task += FTHREADS) {
-      const int k = task / nq, q = q0 + (task - k * nq);
-      uint32_t e[K], o[K];
-      vh_run<K, DIL>(A + (k * K) * AW + 4 * q, AW, r, e, o);
+    uint32_t res[MAXT][K];
 #pragma unroll
-      for (int g = 0; g < K / 4; g++) {
-        uint32_t w[4];
-        transpose4(e + 4 * g, o + 4 * g, w);
+    for (int sl = 0; sl < MAXT; sl++) {
+      const int task = tid + sl * FTHREADS;
+      if (task < ntask) {
+        const int k = task / nq, q = q0 + (task - k * nq);
+        uint32_t e[K], o[K];
+        vh_run<K, DIL>(A + (k * K) * AW + 4 * q, AW, r, e, o);
 #pragma unroll
-        for (int c = 0; c < 4; c++)
-          *reinterpret_cast<uint32_t *>(T + (4 * q + c) * TS + k * K + 4 * g) = w[c];
+        for (int g = 0; g < K / 4; g++) transpose4(e + 4 * g, o + 4 * g, &res[sl][4 * g]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int sl = 0; sl < MAXT; sl++) {
+      const int task = tid + sl * FTHREADS;
+      if (task < ntask) {
+        const int k = task / nq, q = q0 + (task - k * nq);
+#pragma unroll
+        for (int g = 0; g < K / 4; g++)
+#pragma unroll
+          for (int c = 0; c < 4; c++)
+            *reinterpret_cast<uint32_t *>(T + (4 * q + c) * TS + k * K + 4 * g) = res[sl][4 * g + c];
       }
     }
   }
@@ -224,6 +267,8 @@ __global__ __launch_bounds__(FTHREADS) void sq_u8_kernel(const uint8_t *__restri
       }
     }
   }
+    __syncthreads();  // A/T are rewritten by the next tile
+  }
 }
 
 template <bool DIL>
@@ -234,11 +279,22 @@ static int sq_u8_once(const uint8_t *in, uint8_t *out, int64_t B, int64_t H, int
   const int64_t per = (int64_t)tiles_x * tiles_y;
   const int64_t ntiles = per * B;
   if (ntiles > INT32_MAX) PM_FAIL(PM_E_ARG, "image too large for the fused square kernel");
-  const size_t lds = (size_t)(FT + 2 * r) * (FT + 2 * R16) + (size_t)(FT + 2 * R16) * (FT + TPAD);
-  const dim3 grid((unsigned)ntiles), block(FTHREADS);
+  const size_t lds_a = (size_t)(FT + 2 * r) * (FT + 2 * R16), lds_t = (size_t)(FT + 2 * R16) * (FT + TPAD);
+  const size_t lds = lds_a > lds_t ? lds_a : lds_t;
+  int dev = 0, cus = 0;
+  PM_HIP(hipGetDevice(&dev));
+  PM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const dim3 block(FTHREADS);
 #define PM_SQ_LAUNCH(KK)                                                                                     \
-  hipLaunchKernelGGL((sq_u8_kernel<KK, DIL>), grid, block, lds, s, in, out, (int)H, (int)W, r, R16, tiles_x, \
-                     (int)per, (int)ntiles)
+  do {                                                                                                       \
+    int occ = 0;                                                                                             \
+    PM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, sq_u8_kernel<KK, DIL>, FTHREADS, lds));        \
+    int64_t g = (int64_t)(occ > 0 ? occ : 1) * cus;                                                          \
+    if (g >= 8) g &= ~(int64_t)7;                                                                            \
+    if (g > ntiles) g = ntiles;                                                                              \
+    hipLaunchKernelGGL((sq_u8_kernel<KK, DIL>), dim3((unsigned)g), block, lds, s, in, out, (int)H, (int)W, r, \
+                       R16, tiles_x, (int)per, (int)ntiles);                                                 \
+  } while (0)
   if (r >= 16) PM_SQ_LAUNCH(32);
   else if (r >= 8) PM_SQ_LAUNCH(16);
   else if (r >= 4) PM_SQ_LAUNCH(8);
