@@ -34,7 +34,11 @@ typedef unsigned short u16x2_t __attribute__((ext_vector_type(2)));
 
 constexpr int FT = 128;       // output tile side
 constexpr int FTHREADS = 256;
-constexpr int TPAD = 4;       // transposed buffer row padding (bytes)
+// Transposed buffer T: one 128-byte row per region column, 32-bit word w of row R stored
+// at word (w ^ ((R >> 2) & 31)).  Pass 1 writes rows 4q..4q+3 from lanes of consecutive q
+// and pass 2 reads rows k2*K+t at word q2; with the swizzle both land on distinct banks
+// (a plain pitch maps them onto 8 banks).
+__device__ __forceinline__ int t_off(int R, int w) { return R * FT + ((w ^ ((R >> 2) & 31)) << 2); }
 
 template <bool DIL> __device__ __forceinline__ uint32_t pk(uint32_t a, uint32_t b)
 {
@@ -62,16 +66,15 @@ __device__ __forceinline__ void transpose4(const uint32_t *e, const uint32_t *o,
 __device__ __forceinline__ uint32_t lds_word(const uint8_t *p) { return *reinterpret_cast<const uint32_t *>(p); }
 
 // Windowed max/min of 2r+1 rows for K consecutive output rows of one 4-byte column quad.
-// base points at the quad in the first input row (output row j covers input rows j..j+2r);
-// stride is the row pitch in bytes.
-template <int K, bool DIL>
-__device__ __forceinline__ void vh_run(const uint8_t *base, int stride, int r, uint32_t (&e)[K], uint32_t (&o)[K])
+// row(t) returns the quad's 32-bit word in input row t (output row j covers rows j..j+2r).
+template <int K, bool DIL, typename LD>
+__device__ __forceinline__ void vh_run(LD &&row, int r, uint32_t (&e)[K], uint32_t (&o)[K])
 {
   const uint32_t NEU = DIL ? 0u : 0x00FF00FFu;
   if (K > 4 || 2 * r >= K) {
 #pragma unroll
     for (int j = 0; j < K; j++) {
-      const uint32_t w = lds_word(base + j * stride);
+      const uint32_t w = row(j);
       e[j] = even_pairs(w);
       o[j] = odd_pairs(w);
     }
@@ -87,15 +90,14 @@ __device__ __forceinline__ void vh_run(const uint8_t *base, int stride, int r, u
     uint32_t pe = NEU, po = NEU;
 #pragma unroll
     for (int t = 0; t < K; t++) {
-      const uint32_t w = lds_word(base + (K + t) * stride);
+      const uint32_t w = row(K + t);
       const bool use = K + t < 2 * r;
       pe = use ? pk<DIL>(pe, even_pairs(w)) : pe;
       po = use ? pk<DIL>(po, odd_pairs(w)) : po;
     }
-    const uint8_t *p = base + 2 * r * stride;
 #pragma unroll
     for (int j = 0; j < K; j++) {
-      const uint32_t w = lds_word(p + j * stride);
+      const uint32_t w = row(2 * r + j);
       pe = pk<DIL>(pe, even_pairs(w));
       po = pk<DIL>(po, odd_pairs(w));
       e[j] = pk<DIL>(e[j], pe);
@@ -106,7 +108,7 @@ __device__ __forceinline__ void vh_run(const uint8_t *base, int stride, int r, u
     uint32_t xe[6], xo[6];
 #pragma unroll
     for (int t = 0; t < 6; t++) {
-      const uint32_t w = lds_word(base + t * stride);
+      const uint32_t w = row(t);
       xe[t] = even_pairs(w);
       xo[t] = odd_pairs(w);
     }
@@ -142,7 +144,6 @@ __global__ __launch_bounds__(FTHREADS) void sq_u8_kernel(const uint8_t *__restri
   extern __shared__ __align__(16) uint8_t smem[];
   const int AW = FT + 2 * R16;  // region width in bytes (multiple of 16)
   const int AH = FT + 2 * r;    // region height
-  const int TS = FT + TPAD;     // transposed row pitch
   uint8_t *A = smem;
   uint8_t *T = smem;  // the transposed buffer reuses A's storage once A has been consumed
   const int tid = threadIdx.x;
@@ -209,7 +210,8 @@ __global__ __launch_bounds__(FTHREADS) void sq_u8_kernel(const uint8_t *__restri
       if (task < ntask) {
         const int k = task / nq, q = q0 + (task - k * nq);
         uint32_t e[K], o[K];
-        vh_run<K, DIL>(A + (k * K) * AW + 4 * q, AW, r, e, o);
+        const uint8_t *base = A + (k * K) * AW + 4 * q;
+        vh_run<K, DIL>([&](int t) { return lds_word(base + t * AW); }, r, e, o);
 #pragma unroll
         for (int g = 0; g < K / 4; g++) transpose4(e + 4 * g, o + 4 * g, &res[sl][4 * g]);
       }
@@ -224,7 +226,7 @@ __global__ __launch_bounds__(FTHREADS) void sq_u8_kernel(const uint8_t *__restri
         for (int g = 0; g < K / 4; g++)
 #pragma unroll
           for (int c = 0; c < 4; c++)
-            *reinterpret_cast<uint32_t *>(T + (4 * q + c) * TS + k * K + 4 * g) = res[sl][4 * g + c];
+            *reinterpret_cast<uint32_t *>(T + t_off(4 * q + c, (k * K >> 2) + g)) = res[sl][4 * g + c];
       }
     }
   }
@@ -237,7 +239,8 @@ __global__ __launch_bounds__(FTHREADS) void sq_u8_kernel(const uint8_t *__restri
     for (int task = tid; task < ntask; task += FTHREADS) {
       const int q2 = task / NR, k2 = task - q2 * NR;
       uint32_t e[K], o[K];
-      vh_run<K, DIL>(T + (k2 * K + R16 - r) * TS + 4 * q2, TS, r, e, o);
+      const int rbase = k2 * K + R16 - r;
+      vh_run<K, DIL>([&](int t) { return lds_word(T + t_off(rbase + t, q2)); }, r, e, o);
       const int gx = x0 + k2 * K;
       uint32_t rows[4][K / 4];
 #pragma unroll
@@ -279,7 +282,7 @@ static int sq_u8_once(const uint8_t *in, uint8_t *out, int64_t B, int64_t H, int
   const int64_t per = (int64_t)tiles_x * tiles_y;
   const int64_t ntiles = per * B;
   if (ntiles > INT32_MAX) PM_FAIL(PM_E_ARG, "image too large for the fused square kernel");
-  const size_t lds_a = (size_t)(FT + 2 * r) * (FT + 2 * R16), lds_t = (size_t)(FT + 2 * R16) * (FT + TPAD);
+  const size_t lds_a = (size_t)(FT + 2 * r) * (FT + 2 * R16), lds_t = (size_t)(FT + 2 * R16) * FT;
   const size_t lds = lds_a > lds_t ? lds_a : lds_t;
   int dev = 0, cus = 0;
   PM_HIP(hipGetDevice(&dev));
