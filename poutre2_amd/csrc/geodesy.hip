@@ -15,8 +15,11 @@
 //   low_leveling(r, g)  = r ^ R^delta_{r v g}(g)
 //   high_leveling(r, g) = r v R^eps_{r ^ g}(g)
 //   leveling(f, g)      = low_leveling(high_leveling(f, g), g)   (leveling.cpp:231-237)
+#include <type_traits>
+
 #include "arith.hpp"
 #include "erodil.hpp"
+#include "rec_u8.hpp"
 
 namespace pm {
 
@@ -141,6 +144,9 @@ int reconstruct_dev(const T *marker, const T *mask, T *out, Dims d, int se, hipS
 {
   const int64_t n = d.D * d.H * d.W;
   if (n == 0) return PM_OK;
+  if constexpr (std::is_same<T, uint8_t>::value) {
+    if (rec_u8_applicable(se, d)) return rec_u8<DIL>(marker, mask, out, d, se, s);
+  }
   // select_marker: marker ^ mask (dilate) / marker v mask (erode), mreconstruct_t.hpp:51,69
   PM_TRY((ew2<T, DIL ? EW_INF : EW_SUP>(marker, mask, out, n, s)));
   Scratch flag;

@@ -1,0 +1,294 @@
+// rec_u8.hip -- geodesic reconstruction of uint8 2D images (3x3 neighbourhoods) on gfx950.
+//
+// Reconstruction by dilation of marker g under mask f is the least fixed point of
+// x = f ^ delta_B(x) above g ^ f (geo::Reconstruction, mreconstruct_t.hpp:39-195); by erosion
+// it is the dual.  The fixed point is unique, so any schedule of monotone updates that only
+// uses current (or older) neighbour values reaches the same bits as the reference's
+// raster scans + FIFO.
+//
+// One wavefront owns a 62x62 tile.  Its 64 lanes hold 64 columns (the tile plus one halo
+// column on each side) and 64 registers per lane hold the 64 rows (tile plus halo rows), so
+// the whole tile lives in VGPRs.  Halo pixels get mask = value, which pins them (the
+// update min(mask, max(...)) cannot move them), and pixels outside the image get the
+// neutral value as both value and mask; no lane or row needs a special case.
+//  * vertical sweep: each lane walks its column top->bottom then bottom->top; per row
+//    step one v_max3 over the column window, two DPP wave shifts (wave_shr:1 / wave_shl:1)
+//    for the left/right columns, one min with the mask -- 64 pixels per ~6 instructions;
+//  * horizontal sweep: the tile is transposed through LDS (bytes) and the same code runs
+//    with lanes = rows;
+//  * the tile iterates vertical/horizontal sweeps until a full iteration changes nothing.
+// Global rounds: a round launches one workgroup per tile; a tile runs in round R only if its
+// tag act[t] == R; when a tile's border moves it tags its neighbours R+1 and counts itself
+// in moved[R].  Rounds are enqueued in batches without host synchronisation; round R exits
+// at once if moved[R-1] == 0.  Round 0 reads marker ^ mask (marker v mask for erosion)
+// instead of the current image, fusing the reference's select_marker step.
+#include "pm_common.hpp"
+#include "rec_u8.hpp"
+
+namespace pm {
+
+constexpr int RW = 62;        // tile interior side
+constexpr int LP = 68;        // LDS transpose pitch (bytes)
+constexpr int REC_MAX_ROUNDS = 1 << 16;
+
+enum { NBK_SQUARE = 0, NBK_CROSS = 1, NBK_SEGX = 2, NBK_SEGY = 3 };
+
+template <bool DIL> __device__ __forceinline__ uint32_t op3(uint32_t a, uint32_t b, uint32_t c)
+{
+  return DIL ? max(max(a, b), c) : min(min(a, b), c);  // folded into v_max3_u32 / v_min3_u32
+}
+// value of the lane to the left (x-1) / right (x+1); lanes 0 and 63 are halo and pinned
+// (bound_ctrl: lanes without a source read 0; they are the pinned halo lanes, so the value
+// is never used, and no register has to be initialised for the DPP "old" operand)
+__device__ __forceinline__ uint32_t from_left(uint32_t v) { return __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, true); }
+__device__ __forceinline__ uint32_t from_right(uint32_t v) { return __builtin_amdgcn_mov_dpp(v, 0x130, 0xf, 0xf, true); }
+
+__device__ __forceinline__ uint32_t mbyte(const uint32_t (&m)[16], int y) { return (m[y >> 2] >> ((y & 3) * 8)) & 0xffu; }
+
+// One sweep along the register (row) index in direction DOWN (increasing y) or up, over
+// rows 1..62; returns the lane's sum of values after the sweep (monotone updates: the sum
+// moves iff a pixel moved).  NB is the neighbourhood as seen in this layout.
+template <bool DIL, int NB, bool DOWN>
+__device__ __forceinline__ uint32_t sweep(uint32_t (&v)[64], const uint32_t (&m)[16])
+{
+  uint32_t sum = v[0] + v[63];
+#pragma unroll
+  for (int i = 1; i <= 62; i++) {
+    const int y = DOWN ? i : 63 - i;
+    const int yp = DOWN ? y - 1 : y + 1;  // already updated in this sweep
+    const int yn = DOWN ? y + 1 : y - 1;
+    uint32_t c;
+    if (NB == NBK_SQUARE) {
+      const uint32_t w = op3<DIL>(v[yp], v[y], v[yn]);
+      c = op3<DIL>(w, from_left(w), from_right(w));
+    } else if (NB == NBK_CROSS) {
+      c = op3<DIL>(op3<DIL>(v[yp], v[y], v[yn]), from_left(v[y]), from_right(v[y]));
+    } else if (NB == NBK_SEGY) {  // neighbours along the register index only
+      c = op3<DIL>(v[yp], v[y], v[yn]);
+    } else {  // NBK_SEGX: neighbours across lanes only
+      c = op3<DIL>(v[y], from_left(v[y]), from_right(v[y]));
+    }
+    const uint32_t mk = mbyte(m, y);
+    v[y] = DIL ? min(c, mk) : max(c, mk);
+    sum += v[y];
+  }
+  return sum;
+}
+
+template <bool DIL, int NB>
+__global__ __launch_bounds__(64) void rec_u8_kernel(uint8_t *__restrict__ cur, const uint8_t *__restrict__ marker,
+                                                    const uint8_t *__restrict__ mask, int H, int W, int tiles_x,
+                                                    int round, int *__restrict__ act, int *__restrict__ moved)
+{
+  if (round > 0 && moved[round - 1] == 0) return;  // converged in an earlier round
+  const int t = blockIdx.x;
+  // every tile runs in round 0 (it initialises its part of cur); later rounds run the tiles
+  // tagged by a neighbour whose border moved in the previous round
+  if (round > 0 && act[t] != round) return;
+  __shared__ uint8_t L[64 * LP];
+  const int lane = threadIdx.x;
+  const int ty = t / tiles_x, tx = t - ty * tiles_x;
+  const int gx = tx * RW - 1 + lane;
+  const int gy0 = ty * RW - 1;
+  const uint32_t NEU = DIL ? 0u : 255u;
+  const bool lane_halo = (lane == 0 || lane == 63);
+  const bool col_in = (unsigned)gx < (unsigned)W;
+
+  // ---- load the column of this lane (layout A: lane = column, register = row).  Addresses
+  // are clamped into the image and the loads issued without branches, so all of them are in
+  // flight together; out-of-image pixels are replaced by the neutral value afterwards.
+  uint32_t v[64], mA[16];
+  const int cgx = gx < 0 ? 0 : (gx >= W ? W - 1 : gx);
+  const uint8_t *vsrc = (round == 0) ? marker : cur;
+  uint32_t mk_raw[64];
+#pragma unroll
+  for (int y = 0; y < 64; y++) {
+    int gy = gy0 + y;
+    gy = gy < 0 ? 0 : (gy >= H ? H - 1 : gy);
+    const int64_t o = (int64_t)gy * W + cgx;
+    v[y] = vsrc[o];
+    mk_raw[y] = mask[o];
+  }
+#pragma unroll
+  for (int k = 0; k < 16; k++) mA[k] = 0;
+#pragma unroll
+  for (int y = 0; y < 64; y++) {
+    const int gy = gy0 + y;
+    const bool in = col_in && (unsigned)gy < (unsigned)H;
+    uint32_t mk = mk_raw[y], val = v[y];
+    if (round == 0) val = DIL ? min(val, mk) : max(val, mk);  // select_marker
+    if (lane_halo || y == 0 || y == 63) mk = val;             // halo pixels are pinned
+    v[y] = in ? val : NEU;
+    mA[y >> 2] |= (in ? mk : NEU) << ((y & 3) * 8);
+  }
+
+  // ---- mask in layout B (lane = row) through LDS
+  uint32_t mB[16];
+#pragma unroll
+  for (int y = 0; y < 64; y++) L[y * LP + lane] = (uint8_t)mbyte(mA, y);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; k++) mB[k] = *reinterpret_cast<const uint32_t *>(&L[lane * LP + 4 * k]);
+  __syncthreads();
+
+  const uint32_t top0 = v[1], bot0 = v[62];
+  uint32_t colsum0 = 0;
+#pragma unroll
+  for (int y = 1; y <= 62; y++) colsum0 += v[y];
+
+  constexpr int NBA = NB;  // neighbourhood seen by the vertical sweep
+  constexpr int NBB = (NB == NBK_SEGX) ? NBK_SEGY : (NB == NBK_SEGY) ? NBK_SEGX : NB;
+  uint32_t s_prev = 0;
+#pragma unroll
+  for (int y = 0; y < 64; y++) s_prev += v[y];
+  bool any = false;
+  for (int it = 0; it < 4096; it++) {
+    bool ch = false;
+    uint32_t s = sweep<DIL, NBA, true>(v, mA);
+    ch |= s != s_prev;
+    s_prev = s;
+    s = sweep<DIL, NBA, false>(v, mA);
+    ch |= s != s_prev;
+    // transpose to layout B
+#pragma unroll
+    for (int y = 0; y < 64; y++) L[y * LP + lane] = (uint8_t)v[y];
+    __syncthreads();
+    uint32_t sb = 0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const uint32_t w = *reinterpret_cast<const uint32_t *>(&L[lane * LP + 4 * k]);
+#pragma unroll
+      for (int b = 0; b < 4; b++) {
+        v[4 * k + b] = (w >> (8 * b)) & 0xffu;
+        sb += v[4 * k + b];
+      }
+    }
+    __syncthreads();
+    s = sweep<DIL, NBB, true>(v, mB);
+    ch |= s != sb;
+    sb = s;
+    s = sweep<DIL, NBB, false>(v, mB);
+    ch |= s != sb;
+    // back to layout A
+#pragma unroll
+    for (int x = 0; x < 64; x++) L[x * LP + lane] = (uint8_t)v[x];
+    __syncthreads();
+    s_prev = 0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const uint32_t w = *reinterpret_cast<const uint32_t *>(&L[lane * LP + 4 * k]);
+#pragma unroll
+      for (int b = 0; b < 4; b++) {
+        v[4 * k + b] = (w >> (8 * b)) & 0xffu;
+        s_prev += v[4 * k + b];
+      }
+    }
+    __syncthreads();
+    if (!__any(ch)) break;
+    any = true;
+  }
+
+  // ---- write back the interior (always in round 0: it initialises cur)
+  if (any || round == 0) {
+    if (!lane_halo && col_in) {
+#pragma unroll
+      for (int y = 1; y <= 62; y++) {
+        const int gy = gy0 + y;
+        if (gy < H) cur[(int64_t)gy * W + gx] = (uint8_t)v[y];
+      }
+    }
+  }
+  if (!any) return;
+  // ---- which borders moved: rows 1/62 in any lane, columns 1/62 (lanes 1 and 62)
+  uint32_t colsum = 0;
+#pragma unroll
+  for (int y = 1; y <= 62; y++) colsum += v[y];
+  const bool interior_lane = !lane_halo;
+  const bool top = __any(interior_lane && v[1] != top0);
+  const bool bot = __any(interior_lane && v[62] != bot0);
+  const bool left = __any(lane == 1 && colsum != colsum0);
+  const bool right = __any(lane == 62 && colsum != colsum0);
+  if (lane == 0 && (top || bot || left || right)) {
+    const int tiles_y = gridDim.x / tiles_x;
+    const int nr = round + 1;
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dx = -1; dx <= 1; dx++) {
+        if (dx == 0 && dy == 0) continue;
+        const bool need = (dy < 0 && top) || (dy > 0 && bot) || (dx < 0 && left) || (dx > 0 && right);
+        const int ny = ty + dy, nx = tx + dx;
+        if (need && (unsigned)ny < (unsigned)tiles_y && (unsigned)nx < (unsigned)tiles_x) act[ny * tiles_x + nx] = nr;
+      }
+    atomicAdd(&moved[round], 1);
+  }
+}
+
+__global__ void rec_init_kernel(int *act, int n, int *moved, int nm)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) act[i] = 0;
+  if (i < nm) moved[i] = 0;
+}
+
+bool rec_u8_applicable(int se, Dims d)
+{
+  return d.D == 1 && d.H > 0 && d.W > 0 && d.H < (1 << 30) && d.W < (1 << 30) &&
+         (se == PM_SE_SQUARE_2D || se == PM_SE_CROSS_2D || se == PM_SE_SEGMENT_X_1D || se == PM_SE_SEGMENT_X_2D ||
+          se == PM_SE_SEGMENT_Y_2D);
+}
+
+template <bool DIL>
+int rec_u8(const uint8_t *marker, const uint8_t *mask, uint8_t *out, Dims d, int se, hipStream_t s)
+{
+  const int tiles_x = (int)((d.W + RW - 1) / RW), tiles_y = (int)((d.H + RW - 1) / RW);
+  const int64_t nt64 = (int64_t)tiles_x * tiles_y;
+  if (nt64 > INT32_MAX / 2) PM_FAIL(PM_E_ARG, "image too large for the reconstruction kernel");
+  const int nt = (int)nt64;
+  Scratch sc;
+  PM_TRY(sc.alloc(sizeof(int) * ((size_t)nt + REC_MAX_ROUNDS), s));
+  int *act = (int *)sc.ptr;
+  int *moved = act + nt;
+  const int ninit = nt > REC_MAX_ROUNDS ? nt : REC_MAX_ROUNDS;
+  hipLaunchKernelGGL(rec_init_kernel, dim3((ninit + 255) / 256), dim3(256), 0, s, act, nt, moved, REC_MAX_ROUNDS);
+  PM_HIP(hipGetLastError());
+  // pinned host buffer for the convergence counters, allocated once per host thread
+  static thread_local int *hmoved = nullptr;
+  if (!hmoved) PM_HIP(hipHostMalloc((void **)&hmoved, sizeof(int) * 64, hipHostMallocDefault));
+  int nbk = NBK_SQUARE;
+  if (se == PM_SE_CROSS_2D) nbk = NBK_CROSS;
+  else if (se == PM_SE_SEGMENT_X_1D || se == PM_SE_SEGMENT_X_2D) nbk = NBK_SEGX;
+  else if (se == PM_SE_SEGMENT_Y_2D) nbk = NBK_SEGY;
+  int round = 0, batch = 8;
+  while (round < REC_MAX_ROUNDS) {
+    const int nb = (round + batch <= REC_MAX_ROUNDS) ? batch : REC_MAX_ROUNDS - round;
+    for (int i = 0; i < nb; i++) {
+      const int R = round + i;
+      switch (nbk) {
+      case NBK_SQUARE:
+        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SQUARE>), dim3(nt), dim3(64), 0, s, out, marker, mask, (int)d.H, (int)d.W, tiles_x, R, act, moved);
+        break;
+      case NBK_CROSS:
+        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_CROSS>), dim3(nt), dim3(64), 0, s, out, marker, mask, (int)d.H, (int)d.W, tiles_x, R, act, moved);
+        break;
+      case NBK_SEGX:
+        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SEGX>), dim3(nt), dim3(64), 0, s, out, marker, mask, (int)d.H, (int)d.W, tiles_x, R, act, moved);
+        break;
+      default:
+        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SEGY>), dim3(nt), dim3(64), 0, s, out, marker, mask, (int)d.H, (int)d.W, tiles_x, R, act, moved);
+        break;
+      }
+    }
+    PM_HIP(hipGetLastError());
+    PM_HIP(hipMemcpyAsync(hmoved, moved + round, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
+    PM_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < nb; i++)
+      if (hmoved[i] == 0) return PM_OK;
+    round += nb;
+    batch = batch * 2 > 64 ? 64 : batch * 2;
+  }
+  PM_FAIL(PM_E_NOCONV, "reconstruction did not converge");
+}
+
+template int rec_u8<true>(const uint8_t *, const uint8_t *, uint8_t *, Dims, int, hipStream_t);
+template int rec_u8<false>(const uint8_t *, const uint8_t *, uint8_t *, Dims, int, hipStream_t);
+
+}  // namespace pm

@@ -1,0 +1,12 @@
+// rec_u8.hpp -- register-tiled uint8 geodesic reconstruction (rec_u8.hip).
+#pragma once
+#include "pm_common.hpp"
+
+namespace pm {
+
+// True for 2D images (rank 1 or 2) with a 3x3-class SE the kernel handles.
+bool rec_u8_applicable(int se, Dims d);
+// Reconstruction of marker under mask into out (out distinct from both), by dilation (DIL) or erosion.
+template <bool DIL> int rec_u8(const uint8_t *marker, const uint8_t *mask, uint8_t *out, Dims d, int se, hipStream_t s);
+
+}  // namespace pm
