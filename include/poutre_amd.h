@@ -150,6 +150,19 @@ int pm_arith_ssub_image(const pm_image *a, const pm_image *b, pm_image *out, voi
 int pm_arith_ssub_const(const pm_image *in, double c, pm_image *out, void *stream);
 int pm_arith_sadd_const(const pm_image *in, double c, pm_image *out, void *stream);
 
+/* ---- differentiable float32 dilation / erosion (no reference counterpart; extends the flat
+ * Dilate/Erode of include/poutre/low_level_morpho/ero_dil.hpp:32-47 to a structuring function).
+ * x, y, gy, gx: B contiguous float32 images of H x W (device pointers).  w: K x K float32
+ * structuring function (row-major, tap (dy,dx) at [(dy+R)*K + dx+R]) or NULL for the flat
+ * K x K square.  K in {3,5,7,9}.  dilate != 0: y[q] = max_t x[q+t] + w[t]; else
+ * y[q] = min_t x[q+t] - w[t]; taps outside the image are skipped.  The backward routes gy[q]
+ * to the first maximiser (minimiser) in raster order of t and, if w != NULL, writes the
+ * gradient of w (K*K floats) to gw.  Deterministic (no atomics). */
+int pm_maxplus_fwd(const float *x, int64_t B, int64_t H, int64_t W, const float *w, int K, int dilate, float *y,
+                   void *stream);
+int pm_maxplus_bwd(const float *x, const float *gy, int64_t B, int64_t H, int64_t W, const float *w, int K,
+                   int dilate, float *gx, float *gw, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -69,6 +69,8 @@ SIGNATURES = {
     "pm_arith_ssub_image": [_P, _P, _P, _VP],
     "pm_arith_ssub_const": [_P, _D, _P, _VP],
     "pm_arith_sadd_const": [_P, _D, _P, _VP],
+    "pm_maxplus_fwd": [_VP, _I64, _I64, _I64, _VP, _I, _I, _VP, _VP],
+    "pm_maxplus_bwd": [_VP, _VP, _I64, _I64, _I64, _VP, _I, _I, _VP, _VP, _VP],
 }
 
 

@@ -1,0 +1,94 @@
+"""Differentiable float32 morphology on PyTorch-ROCm, backed by the hand-written kernels of
+libpoutre_amd.so (pm_maxplus_fwd / pm_maxplus_bwd).
+
+    y = dilation2d(x, k=3)            flat dilation by a k x k square
+    y = erosion2d(x, k=7)             flat erosion
+    y = dilation2d(x, w=w)            max-plus dilation, y[q] = max_t x[q+t] + w[t]
+    y = erosion2d(x, w=w)             y[q] = min_t x[q+t] - w[t]
+
+x: float32 tensor (..., H, W) on a ROCm device (contiguous); w: (K, K) float32 with K odd in
+{3, 5, 7, 9}.  Taps outside the image are skipped.  The gradient goes to the first
+maximiser (minimiser) in raster order of the window (torch.max's first occurrence over an
+F.unfold window); w receives the sum of the routed output gradients (negated for erosion).
+"""
+import ctypes
+
+import torch
+
+from . import _capi
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def _stream(t):
+    return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
+
+
+def _dims(x):
+    if x.dim() < 2:
+        raise ValueError("expected a tensor of shape (..., H, W)")
+    H, W = x.shape[-2], x.shape[-1]
+    B = x.numel() // (H * W) if H * W else 0
+    return B, H, W
+
+
+class _MorphFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, w, k, dilate):
+        if x.dtype != torch.float32 or not x.is_cuda:
+            raise TypeError("x must be a float32 device tensor")
+        x = x.contiguous()
+        if w is not None:
+            w = w.contiguous().to(torch.float32)
+            k = w.shape[-1]
+            if w.shape != (k, k):
+                raise ValueError("w must be (K, K)")
+        B, H, W = _dims(x)
+        y = torch.empty_like(x)
+        _capi.check(_capi.lib().pm_maxplus_fwd(_ptr(x), B, H, W, _ptr(w), int(k), int(dilate), _ptr(y), _stream(x)))
+        ctx.save_for_backward(x, w)
+        ctx.k, ctx.dilate = int(k), int(dilate)
+        return y
+
+    @staticmethod
+    def backward(ctx, gy):
+        x, w = ctx.saved_tensors
+        gy = gy.contiguous()
+        B, H, W = _dims(x)
+        gx = torch.empty_like(x)
+        gw = torch.empty_like(w) if w is not None else None
+        _capi.check(_capi.lib().pm_maxplus_bwd(_ptr(x), _ptr(gy), B, H, W, _ptr(w), ctx.k, ctx.dilate, _ptr(gx),
+                                               _ptr(gw), _stream(x)))
+        return gx, (gw if (w is not None and ctx.needs_input_grad[1]) else None), None, None
+
+
+def dilation2d(x, k=3, w=None):
+    """Flat (k x k square) or max-plus (structuring function w) dilation, differentiable."""
+    return _MorphFn.apply(x, w, k, True)
+
+
+def erosion2d(x, k=3, w=None):
+    """Flat (k x k square) or max-plus (structuring function w) erosion, differentiable."""
+    return _MorphFn.apply(x, w, k, False)
+
+
+def reference_cpu(x, k=3, w=None, dilate=True):
+    """torch CPU reference of the same operator (F.unfold + max/min, first occurrence), used by
+    the tests as the autograd reference: out-of-image taps are -inf (+inf) so they are never
+    selected."""
+    import torch.nn.functional as F
+    H, W = x.shape[-2], x.shape[-1]
+    if w is not None:
+        k = w.shape[-1]
+    r = (k - 1) // 2
+    pad = float("-inf") if dilate else float("inf")
+    xb = x.reshape(-1, 1, H, W)
+    xp = F.pad(xb, (r, r, r, r), value=pad)
+    u = F.unfold(xp, k)  # (B, k*k, H*W), taps in raster order
+    if w is not None:
+        wv = w.reshape(1, k * k, 1)
+        u = u + wv if dilate else u - wv
+    y = u.max(dim=1).values if dilate else u.min(dim=1).values
+    return y.reshape(x.shape)

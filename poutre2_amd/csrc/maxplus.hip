@@ -1,0 +1,270 @@
+// maxplus.hip -- differentiable float32 dilation / erosion (flat and max-plus) on gfx950.
+//
+// No reference counterpart: this extends the flat Dilate/Erode of ero_dil.hpp:32-47 to a
+// batch of float32 images with a structuring FUNCTION w (K x K, K odd) and a backward pass.
+//   dilation: y[q] = max_t ( x[q + t] + w[t] )        erosion: y[q] = min_t ( x[q + t] - w[t] )
+//   flat:     w == 0 on the K x K square (the SESquare2D^((K-1)/2) operator of the reference)
+// t = (dy, dx) in [-R, R]^2, R = (K-1)/2; taps falling outside the image are skipped (the
+// reference's neutral padding).  The selected tap of q is the FIRST maximiser (minimiser) in
+// raster order of t (dy outer, dx inner), i.e. torch.max's first-occurrence rule over an
+// F.unfold window.  Backward (gather form, deterministic):
+//   gx[p] = sum over taps t in raster order of [sel(p - t) == t] * gy[p - t]
+//   gw[t] = +-sum over q of [sel(q) == t] * gy[q]      (+ for dilation, - for erosion)
+// gw is reduced per workgroup in a fixed order into a partials buffer, then summed over
+// workgroups in a fixed order by a second kernel.
+// Memory per pixel: forward reads x, writes y (8 B); backward reads x and gy, writes gx
+// (12 B); the selected taps are recomputed in the backward instead of being stored.
+#include <cmath>
+
+#include "maxplus.hpp"
+#include "pm_common.hpp"
+
+namespace pm {
+
+constexpr int MTW = 64, MTH = 16, MTHREADS = 256;
+
+template <bool DIL> __device__ __forceinline__ float pad_value() { return DIL ? -INFINITY : INFINITY; }
+
+// load a (rows x cols) float region whose origin is (y0, x0) in image b into LDS, padding with pad
+__device__ __forceinline__ void load_region(float *dst, int rows, int cols, const float *__restrict__ src, int H, int W,
+                                            int y0, int x0, float pad)
+{
+  const int n = rows * cols;
+  for (int e = threadIdx.x; e < n; e += MTHREADS) {
+    const int i = e / cols, j = e - i * cols;
+    const int gy = y0 + i, gx = x0 + j;
+    dst[e] = ((unsigned)gy < (unsigned)H && (unsigned)gx < (unsigned)W) ? src[(int64_t)gy * W + gx] : pad;
+  }
+}
+
+// selected tap (raster index dy*K + dx) and value of output q whose window starts at row
+// pointer p (region pitch pitch); out-of-image taps hold +-inf and are never selected
+template <int K, bool DIL, bool FLAT>
+__device__ __forceinline__ int select_tap(const float *p, int pitch, const float *w, float &best)
+{
+  int sel = 0;
+  best = pad_value<DIL>();
+#pragma unroll
+  for (int dy = 0; dy < K; dy++)
+#pragma unroll
+    for (int dx = 0; dx < K; dx++) {
+      const float xv = p[dy * pitch + dx];
+      const float v = FLAT ? xv : (DIL ? xv + w[dy * K + dx] : xv - w[dy * K + dx]);
+      const bool better = DIL ? (v > best) : (v < best);
+      if (better) {
+        best = v;
+        sel = dy * K + dx;
+      }
+    }
+  return sel;
+}
+
+template <int K, bool DIL, bool FLAT>
+__global__ __launch_bounds__(MTHREADS) void mp_fwd_kernel(const float *__restrict__ x, const float *__restrict__ wg,
+                                                          float *__restrict__ y, int H, int W, int tiles_x, int tiles_per_img)
+{
+  constexpr int R = (K - 1) / 2, RC = MTW + K - 1, RR = MTH + K - 1;
+  __shared__ float X[RR * RC];
+  __shared__ float w[K * K];
+  const int img = blockIdx.x / tiles_per_img, t = blockIdx.x - img * tiles_per_img;
+  const int ty = t / tiles_x, tx = t - ty * tiles_x;
+  const int y0 = ty * MTH, x0 = tx * MTW;
+  const float *xi = x + (int64_t)img * H * W;
+  if (!FLAT && threadIdx.x < K * K) w[threadIdx.x] = wg[threadIdx.x];
+  load_region(X, RR, RC, xi, H, W, y0 - R, x0 - R, pad_value<DIL>());
+  __syncthreads();
+  float *yi = y + (int64_t)img * H * W;
+  for (int e = threadIdx.x; e < MTH * MTW; e += MTHREADS) {
+    const int i = e / MTW, j = e - i * MTW;
+    const int gy = y0 + i, gx = x0 + j;
+    if (gy >= H || gx >= W) continue;
+    float best;
+    select_tap<K, DIL, FLAT>(&X[i * RC + j], RC, w, best);
+    yi[(int64_t)gy * W + gx] = best;
+  }
+}
+
+template <int K, bool DIL, bool FLAT>
+__global__ __launch_bounds__(MTHREADS) void mp_bwd_kernel(const float *__restrict__ x, const float *__restrict__ gy,
+                                                          const float *__restrict__ wg, float *__restrict__ gx,
+                                                          float *__restrict__ gw_part, int H, int W, int tiles_x,
+                                                          int tiles_per_img)
+{
+  constexpr int R = (K - 1) / 2;
+  constexpr int QC = MTW + K - 1, QR = MTH + K - 1;            // outputs q whose window reaches the tile
+  constexpr int XC = MTW + 2 * (K - 1), XR = MTH + 2 * (K - 1);  // inputs those q read
+  __shared__ float X[XR * XC];
+  __shared__ float G[QR * QC];
+  __shared__ unsigned char S[QR * QC];
+  __shared__ float w[K * K];
+  __shared__ float red[MTHREADS];
+  const int img = blockIdx.x / tiles_per_img, t = blockIdx.x - img * tiles_per_img;
+  const int ty = t / tiles_x, tx = t - ty * tiles_x;
+  const int y0 = ty * MTH, x0 = tx * MTW;
+  const int64_t ib = (int64_t)img * H * W;
+  if (!FLAT && threadIdx.x < K * K) w[threadIdx.x] = wg[threadIdx.x];
+  load_region(X, XR, XC, x + ib, H, W, y0 - 2 * R, x0 - 2 * R, pad_value<DIL>());
+  // gy of out-of-image q is 0: they receive nothing and contribute nothing
+  load_region(G, QR, QC, gy + ib, H, W, y0 - R, x0 - R, 0.0f);
+  __syncthreads();
+  // selected tap of every q in the tile extended by R (q = (y0 - R + i, x0 - R + j))
+  for (int e = threadIdx.x; e < QR * QC; e += MTHREADS) {
+    const int i = e / QC, j = e - i * QC;
+    float best;
+    S[e] = (unsigned char)select_tap<K, DIL, FLAT>(&X[i * XC + j], XC, w, best);
+  }
+  __syncthreads();
+  // gather: p = (y0 + i, x0 + j) receives gy[q] from q = p - t when sel(q) == t
+  for (int e = threadIdx.x; e < MTH * MTW; e += MTHREADS) {
+    const int i = e / MTW, j = e - i * MTW;
+    const int py = y0 + i, px = x0 + j;
+    float acc = 0.0f;
+#pragma unroll
+    for (int dy = 0; dy < K; dy++)
+#pragma unroll
+      for (int dx = 0; dx < K; dx++) {
+        // q = p - (dy - R, dx - R); in q-region coordinates (i + R - (dy - R), j + R - (dx - R))
+        const int qi = i + 2 * R - dy, qj = j + 2 * R - dx;
+        const int qe = qi * QC + qj;
+        if (S[qe] == dy * K + dx) acc += G[qe];
+      }
+    if (py < H && px < W) gx[ib + (int64_t)py * W + px] = acc;
+  }
+  if (FLAT) return;
+  // structuring-function gradient: per-thread partial per tap over this tile's q, then a
+  // fixed-order tree over the workgroup
+  float part[K * K];
+#pragma unroll
+  for (int k = 0; k < K * K; k++) part[k] = 0.0f;
+  for (int e = threadIdx.x; e < MTH * MTW; e += MTHREADS) {
+    const int i = e / MTW, j = e - i * MTW;
+    const int qe = (i + R) * QC + (j + R);
+    const int s = S[qe];
+    const float g = G[qe];
+#pragma unroll
+    for (int k = 0; k < K * K; k++) part[k] += (s == k) ? g : 0.0f;
+  }
+  float *dst = gw_part + (int64_t)blockIdx.x * K * K;
+  for (int k = 0; k < K * K; k++) {
+    red[threadIdx.x] = part[k];
+    __syncthreads();
+    for (int sft = MTHREADS / 2; sft > 0; sft >>= 1) {
+      if (threadIdx.x < sft) red[threadIdx.x] += red[threadIdx.x + sft];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) dst[k] = DIL ? red[0] : -red[0];
+    __syncthreads();
+  }
+}
+
+// gw[k] = sum over blocks b (in order) of part[b][k]: one workgroup per tap, fixed-order tree
+__global__ __launch_bounds__(256) void gw_reduce_kernel(const float *__restrict__ part, int nblocks, int KK,
+                                                        float *__restrict__ gw)
+{
+  __shared__ float red[256];
+  const int k = blockIdx.x;
+  float acc = 0.0f;
+  for (int b = threadIdx.x; b < nblocks; b += 256) acc += part[(int64_t)b * KK + k];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) gw[k] = red[0];
+}
+
+static int mp_check(int64_t B, int64_t H, int64_t W, int K, const void *a, const void *b)
+{
+  clear_error();
+  if (B < 0 || H < 0 || W < 0) PM_FAIL(PM_E_SIZE, "maxplus: negative extent");
+  if (K != 3 && K != 5 && K != 7 && K != 9) PM_FAIL(PM_E_ARG, "maxplus: K must be 3, 5, 7 or 9 (got %d)", K);
+  if (B * H * W > 0 && (!a || !b)) PM_FAIL(PM_E_ARG, "maxplus: null pointer");
+  if (H > INT32_MAX || W > INT32_MAX) PM_FAIL(PM_E_SIZE, "maxplus: image too large");
+  return PM_OK;
+}
+
+static int64_t mp_blocks(int64_t B, int64_t H, int64_t W, int &tiles_x, int &per)
+{
+  tiles_x = (int)((W + MTW - 1) / MTW);
+  per = tiles_x * (int)((H + MTH - 1) / MTH);
+  return B * per;
+}
+
+}  // namespace pm
+
+using namespace pm;
+
+#define PM_MP_DISPATCH(MACRO)                                                                                \
+  switch (K) {                                                                                               \
+  case 3: MACRO(3); break;                                                                                   \
+  case 5: MACRO(5); break;                                                                                   \
+  case 7: MACRO(7); break;                                                                                   \
+  default: MACRO(9); break;                                                                                  \
+  }
+
+extern "C" int pm_maxplus_fwd(const float *x, int64_t B, int64_t H, int64_t W, const float *w, int K, int dilate,
+                              float *y, void *stream)
+{
+  PM_TRY(mp_check(B, H, W, K, x, y));
+  if (x == y && B * H * W > 0) PM_FAIL(PM_E_ALIAS, "maxplus: output must be != input");
+  int tiles_x, per;
+  const int64_t nb = mp_blocks(B, H, W, tiles_x, per);
+  if (nb == 0) return PM_OK;
+  if (nb > INT32_MAX) PM_FAIL(PM_E_SIZE, "maxplus: batch too large");
+  hipStream_t s = as_stream(stream);
+  const bool flat = (w == nullptr);
+#define L(KK)                                                                                                \
+  do {                                                                                                       \
+    if (dilate) {                                                                                            \
+      if (flat) hipLaunchKernelGGL((mp_fwd_kernel<KK, true, true>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, x, w, y, (int)H, (int)W, tiles_x, per); \
+      else hipLaunchKernelGGL((mp_fwd_kernel<KK, true, false>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, x, w, y, (int)H, (int)W, tiles_x, per); \
+    } else {                                                                                                 \
+      if (flat) hipLaunchKernelGGL((mp_fwd_kernel<KK, false, true>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, x, w, y, (int)H, (int)W, tiles_x, per); \
+      else hipLaunchKernelGGL((mp_fwd_kernel<KK, false, false>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, x, w, y, (int)H, (int)W, tiles_x, per); \
+    }                                                                                                        \
+  } while (0)
+  PM_MP_DISPATCH(L)
+#undef L
+  PM_HIP(hipGetLastError());
+  return PM_OK;
+}
+
+extern "C" int pm_maxplus_bwd(const float *x, const float *gy, int64_t B, int64_t H, int64_t W, const float *w, int K,
+                              int dilate, float *gx, float *gw, void *stream)
+{
+  PM_TRY(mp_check(B, H, W, K, x, gx));
+  if (!gy && B * H * W > 0) PM_FAIL(PM_E_ARG, "maxplus: null gradient");
+  if ((gx == x || gx == gy) && B * H * W > 0) PM_FAIL(PM_E_ALIAS, "maxplus: gx must be a distinct buffer");
+  const bool flat = (w == nullptr);
+  if (!flat && !gw) PM_FAIL(PM_E_ARG, "maxplus: gw buffer required for a structuring function");
+  int tiles_x, per;
+  const int64_t nb = mp_blocks(B, H, W, tiles_x, per);
+  hipStream_t s = as_stream(stream);
+  if (nb == 0) {
+    if (!flat) PM_HIP(hipMemsetAsync(gw, 0, sizeof(float) * K * K, s));
+    return PM_OK;
+  }
+  if (nb > INT32_MAX) PM_FAIL(PM_E_SIZE, "maxplus: batch too large");
+  Scratch part;
+  if (!flat) PM_TRY(part.alloc(sizeof(float) * (size_t)nb * K * K, s));
+  float *pp = (float *)part.ptr;
+#define L(KK)                                                                                                \
+  do {                                                                                                       \
+    if (dilate) {                                                                                            \
+      if (flat) hipLaunchKernelGGL((mp_bwd_kernel<KK, true, true>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, x, gy, w, gx, pp, (int)H, (int)W, tiles_x, per); \
+      else hipLaunchKernelGGL((mp_bwd_kernel<KK, true, false>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, x, gy, w, gx, pp, (int)H, (int)W, tiles_x, per); \
+    } else {                                                                                                 \
+      if (flat) hipLaunchKernelGGL((mp_bwd_kernel<KK, false, true>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, x, gy, w, gx, pp, (int)H, (int)W, tiles_x, per); \
+      else hipLaunchKernelGGL((mp_bwd_kernel<KK, false, false>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, x, gy, w, gx, pp, (int)H, (int)W, tiles_x, per); \
+    }                                                                                                        \
+  } while (0)
+  PM_MP_DISPATCH(L)
+#undef L
+  PM_HIP(hipGetLastError());
+  if (!flat) {
+    hipLaunchKernelGGL(gw_reduce_kernel, dim3(K * K), dim3(256), 0, s, pp, (int)nb, K * K, gw);
+    PM_HIP(hipGetLastError());
+  }
+  return PM_OK;
+}
