@@ -163,6 +163,17 @@ int pm_maxplus_fwd(const float *x, int64_t B, int64_t H, int64_t W, const float 
 int pm_maxplus_bwd(const float *x, const float *gy, int64_t B, int64_t H, int64_t W, const float *w, int K,
                    int dilate, float *gx, float *gw, void *stream);
 
+/* ---- batched launches (no reference counterpart: the reference processes one image per call).
+ * pm_batch_dilate_open: for each i < n, dil_outs[i] = Dilate(ins[i], SESquare2D, radii[i]) and
+ * open_outs[i] = Dilate(Erode(ins[i], SESquare2D, radii[i]), SESquare2D, radii[i]), all images
+ * uint8 2D of any sizes, radii in [1, 32], in one cooperative persistent launch.
+ * pm_batch_reconstruct: n reconstructions (geo::Reconstruction semantics, uint8 2D, 3x3 SEs)
+ * sharing one sequence of global rounds. */
+int pm_batch_dilate_open(int n, const pm_image *ins, const int *radii, pm_image *dil_outs, pm_image *open_outs,
+                         void *stream);
+int pm_batch_reconstruct(int n, int rec_type, const pm_image *markers, const pm_image *masks, int se, pm_image *outs,
+                         void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,7 +25,8 @@ __all__ = ["static_nl_se", "static_compound_se", "RecType", "erode", "dilate", "
            "dilateY", "erode_rect", "dilate_rect", "opening", "closing", "gradient", "reconstruction",
            "h_minima", "h_maxima", "h_concave", "h_convex", "dynamic_pseudo_closing", "dynamic_pseudo_opening",
            "high_leveling", "low_leveling", "leveling", "arith_sup", "arith_inf", "arith_ssub_image",
-           "arith_ssub_const", "arith_sadd_const", "PoutreError", "version"]
+           "arith_ssub_const", "arith_sadd_const", "batch_dilate_open", "batch_reconstruction", "PoutreError",
+           "version"]
 
 
 class static_nl_se(enum.IntEnum):
@@ -298,3 +299,33 @@ def arith_ssub_const(a, c, out=None):
 
 def arith_sadd_const(a, c, out=None):
     return _ewc("pm_arith_sadd_const", a, c, out)
+
+
+# ---------------------------------------------------------------- batched launches
+def batch_dilate_open(images, radii, dil_outs=None, open_outs=None):
+    """For every uint8 2D image i: (Dilate(img, square2d, r_i), Dilate(Erode(img, square2d, r_i), square2d, r_i)),
+    all images of any sizes in one persistent launch (pm_batch_dilate_open).  Returns the two lists."""
+    n = len(images)
+    if len(radii) != n:
+        raise PoutreError(-6, "batch: one radius per image")
+    dil_outs = [_out(im, None) for im in images] if dil_outs is None else dil_outs
+    open_outs = [_out(im, None) for im in images] if open_outs is None else open_outs
+    if n == 0:
+        return dil_outs, open_outs
+    arr = _capi.PmImage * n
+    di, dd, do = arr(*[_desc(t) for t in images]), arr(*[_desc(t) for t in dil_outs]), arr(*[_desc(t) for t in open_outs])
+    rr = (ctypes.c_int * n)(*[int(r) for r in radii])
+    _call("pm_batch_dilate_open", n, di, rr, dd, do, _stream(images[0]))
+    return dil_outs, open_outs
+
+
+def batch_reconstruction(rec_type, markers, masks, se, outs=None):
+    """n uint8 2D reconstructions (geo::Reconstruction semantics) sharing one sequence of global rounds."""
+    n = len(markers)
+    outs = [_out(m, None) for m in masks] if outs is None else outs
+    if n == 0:
+        return outs
+    arr = _capi.PmImage * n
+    dm, dk, do = arr(*[_desc(t) for t in markers]), arr(*[_desc(t) for t in masks]), arr(*[_desc(t) for t in outs])
+    _call("pm_batch_reconstruct", n, int(rec_type), dm, dk, int(se), do, _stream(masks[0]))
+    return outs

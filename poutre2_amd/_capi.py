@@ -71,6 +71,8 @@ SIGNATURES = {
     "pm_arith_sadd_const": [_P, _D, _P, _VP],
     "pm_maxplus_fwd": [_VP, _I64, _I64, _I64, _VP, _I, _I, _VP, _VP],
     "pm_maxplus_bwd": [_VP, _VP, _I64, _I64, _I64, _VP, _I, _I, _VP, _VP, _VP],
+    "pm_batch_dilate_open": [_I, _P, ctypes.POINTER(_I), _P, _P, _VP],
+    "pm_batch_reconstruct": [_I, _I, _P, _P, _I, _P, _VP],
 }
 
 

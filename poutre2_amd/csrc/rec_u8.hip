@@ -22,6 +22,9 @@
 // in moved[R].  Rounds are enqueued in batches without host synchronisation; round R exits
 // at once if moved[R-1] == 0.  Round 0 reads marker ^ mask (marker v mask for erosion)
 // instead of the current image, fusing the reference's select_marker step.
+#include <vector>
+
+#include "erodil.hpp"
 #include "pm_common.hpp"
 #include "rec_u8.hpp"
 
@@ -32,6 +35,24 @@ constexpr int LP = 68;        // LDS transpose pitch (bytes)
 constexpr int REC_MAX_ROUNDS = 1 << 16;
 
 enum { NBK_SQUARE = 0, NBK_CROSS = 1, NBK_SEGX = 2, NBK_SEGY = 3 };
+
+// one image of a (batched) reconstruction; its tiles are tile0 .. tile0 + tiles_x*tiles_y - 1
+struct RecImage {
+  uint8_t *cur;
+  const uint8_t *marker, *mask;
+  int H, W, tiles_x, tiles_y, tile0;
+};
+
+__device__ __forceinline__ int rec_find(const RecImage *im, int n, int t)
+{
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (im[mid].tile0 <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
 
 template <bool DIL> __device__ __forceinline__ uint32_t op3(uint32_t a, uint32_t b, uint32_t c)
 {
@@ -76,8 +97,7 @@ __device__ __forceinline__ uint32_t sweep(uint32_t (&v)[64], const uint32_t (&m)
 }
 
 template <bool DIL, int NB>
-__global__ __launch_bounds__(64) void rec_u8_kernel(uint8_t *__restrict__ cur, const uint8_t *__restrict__ marker,
-                                                    const uint8_t *__restrict__ mask, int H, int W, int tiles_x,
+__global__ __launch_bounds__(64) void rec_u8_kernel(RecImage one, const RecImage *__restrict__ many, int nimg,
                                                     int round, int *__restrict__ act, int *__restrict__ moved)
 {
   if (round > 0 && moved[round - 1] == 0) return;  // converged in an earlier round
@@ -85,9 +105,15 @@ __global__ __launch_bounds__(64) void rec_u8_kernel(uint8_t *__restrict__ cur, c
   // every tile runs in round 0 (it initialises its part of cur); later rounds run the tiles
   // tagged by a neighbour whose border moved in the previous round
   if (round > 0 && act[t] != round) return;
+  const RecImage im = many ? many[rec_find(many, nimg, t)] : one;
+  uint8_t *__restrict__ cur = im.cur;
+  const uint8_t *__restrict__ marker = im.marker;
+  const uint8_t *__restrict__ mask = im.mask;
+  const int H = im.H, W = im.W, tiles_x = im.tiles_x;
   __shared__ uint8_t L[64 * LP];
   const int lane = threadIdx.x;
-  const int ty = t / tiles_x, tx = t - ty * tiles_x;
+  const int tl = t - im.tile0;
+  const int ty = tl / tiles_x, tx = tl - ty * tiles_x;
   const int gx = tx * RW - 1 + lane;
   const int gy0 = ty * RW - 1;
   const uint32_t NEU = DIL ? 0u : 255u;
@@ -209,14 +235,15 @@ __global__ __launch_bounds__(64) void rec_u8_kernel(uint8_t *__restrict__ cur, c
   const bool left = __any(lane == 1 && colsum != colsum0);
   const bool right = __any(lane == 62 && colsum != colsum0);
   if (lane == 0 && (top || bot || left || right)) {
-    const int tiles_y = gridDim.x / tiles_x;
+    const int tiles_y = im.tiles_y;
     const int nr = round + 1;
     for (int dy = -1; dy <= 1; dy++)
       for (int dx = -1; dx <= 1; dx++) {
         if (dx == 0 && dy == 0) continue;
         const bool need = (dy < 0 && top) || (dy > 0 && bot) || (dx < 0 && left) || (dx > 0 && right);
         const int ny = ty + dy, nx = tx + dx;
-        if (need && (unsigned)ny < (unsigned)tiles_y && (unsigned)nx < (unsigned)tiles_x) act[ny * tiles_x + nx] = nr;
+        if (need && (unsigned)ny < (unsigned)tiles_y && (unsigned)nx < (unsigned)tiles_x)
+          act[im.tile0 + ny * tiles_x + nx] = nr;
       }
     atomicAdd(&moved[round], 1);
   }
@@ -236,13 +263,10 @@ bool rec_u8_applicable(int se, Dims d)
           se == PM_SE_SEGMENT_Y_2D);
 }
 
+// Rounds driver shared by the single-image and batched entry points.
 template <bool DIL>
-int rec_u8(const uint8_t *marker, const uint8_t *mask, uint8_t *out, Dims d, int se, hipStream_t s)
+static int rec_rounds(const RecImage &one, const RecImage *many, int nimg, int nt, int se, hipStream_t s)
 {
-  const int tiles_x = (int)((d.W + RW - 1) / RW), tiles_y = (int)((d.H + RW - 1) / RW);
-  const int64_t nt64 = (int64_t)tiles_x * tiles_y;
-  if (nt64 > INT32_MAX / 2) PM_FAIL(PM_E_ARG, "image too large for the reconstruction kernel");
-  const int nt = (int)nt64;
   Scratch sc;
   PM_TRY(sc.alloc(sizeof(int) * ((size_t)nt + REC_MAX_ROUNDS), s));
   int *act = (int *)sc.ptr;
@@ -264,16 +288,16 @@ int rec_u8(const uint8_t *marker, const uint8_t *mask, uint8_t *out, Dims d, int
       const int R = round + i;
       switch (nbk) {
       case NBK_SQUARE:
-        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SQUARE>), dim3(nt), dim3(64), 0, s, out, marker, mask, (int)d.H, (int)d.W, tiles_x, R, act, moved);
+        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SQUARE>), dim3(nt), dim3(64), 0, s, one, many, nimg, R, act, moved);
         break;
       case NBK_CROSS:
-        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_CROSS>), dim3(nt), dim3(64), 0, s, out, marker, mask, (int)d.H, (int)d.W, tiles_x, R, act, moved);
+        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_CROSS>), dim3(nt), dim3(64), 0, s, one, many, nimg, R, act, moved);
         break;
       case NBK_SEGX:
-        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SEGX>), dim3(nt), dim3(64), 0, s, out, marker, mask, (int)d.H, (int)d.W, tiles_x, R, act, moved);
+        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SEGX>), dim3(nt), dim3(64), 0, s, one, many, nimg, R, act, moved);
         break;
       default:
-        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SEGY>), dim3(nt), dim3(64), 0, s, out, marker, mask, (int)d.H, (int)d.W, tiles_x, R, act, moved);
+        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SEGY>), dim3(nt), dim3(64), 0, s, one, many, nimg, R, act, moved);
         break;
       }
     }
@@ -288,7 +312,60 @@ int rec_u8(const uint8_t *marker, const uint8_t *mask, uint8_t *out, Dims d, int
   PM_FAIL(PM_E_NOCONV, "reconstruction did not converge");
 }
 
+static int rec_describe(const uint8_t *marker, const uint8_t *mask, uint8_t *out, Dims d, int64_t tile0, RecImage &r)
+{
+  const int64_t tx = (d.W + RW - 1) / RW, ty = (d.H + RW - 1) / RW;
+  if (tile0 + tx * ty > INT32_MAX / 2) PM_FAIL(PM_E_ARG, "image too large for the reconstruction kernel");
+  r.cur = out;
+  r.marker = marker;
+  r.mask = mask;
+  r.H = (int)d.H;
+  r.W = (int)d.W;
+  r.tiles_x = (int)tx;
+  r.tiles_y = (int)ty;
+  r.tile0 = (int)tile0;
+  return PM_OK;
+}
+
+template <bool DIL>
+int rec_u8(const uint8_t *marker, const uint8_t *mask, uint8_t *out, Dims d, int se, hipStream_t s)
+{
+  RecImage one;
+  PM_TRY(rec_describe(marker, mask, out, d, 0, one));
+  return rec_rounds<DIL>(one, nullptr, 1, one.tiles_x * one.tiles_y, se, s);
+}
+
 template int rec_u8<true>(const uint8_t *, const uint8_t *, uint8_t *, Dims, int, hipStream_t);
 template int rec_u8<false>(const uint8_t *, const uint8_t *, uint8_t *, Dims, int, hipStream_t);
 
 }  // namespace pm
+
+using namespace pm;
+
+// Batched reconstruction: the tiles of all images go through the same rounds.
+extern "C" int pm_batch_reconstruct(int n, int rec_type, const pm_image *markers, const pm_image *masks, int se,
+                                    pm_image *outs, void *stream)
+{
+  clear_error();
+  if (n < 0 || (n > 0 && (!markers || !masks || !outs))) PM_FAIL(PM_E_ARG, "batch reconstruct: bad arguments");
+  if (rec_type != PM_REC_DILATE && rec_type != PM_REC_ERODE) PM_FAIL(PM_E_ARG, "Unsupported reconstruction type");
+  if (n == 0) return PM_OK;
+  hipStream_t s = as_stream(stream);
+  std::vector<RecImage> host(n);
+  int64_t tiles = 0;
+  for (int i = 0; i < n; i++) {
+    PM_TRY(check_binary(&markers[i], &masks[i], &outs[i], "Reconstruction"));
+    if (markers[i].ptype != PM_PTYPE_U8) PM_FAIL(PM_E_UNSUPPORTED, "batch reconstruct: uint8 images only");
+    PM_TRY(check_se_rank(se, markers[i].rank));
+    const Dims d = dims3(markers[i]);
+    if (!rec_u8_applicable(se, d)) PM_FAIL(PM_E_UNSUPPORTED, "batch reconstruct: 2D images with a 3x3 SE only");
+    PM_TRY(rec_describe((const uint8_t *)markers[i].data, (const uint8_t *)masks[i].data, (uint8_t *)outs[i].data, d,
+                        tiles, host[i]));
+    tiles += (int64_t)host[i].tiles_x * host[i].tiles_y;
+  }
+  Scratch dsc;
+  PM_TRY(dsc.alloc(sizeof(RecImage) * (size_t)n, s));
+  PM_HIP(hipMemcpyAsync(dsc.ptr, host.data(), sizeof(RecImage) * (size_t)n, hipMemcpyHostToDevice, s));
+  if (rec_type == PM_REC_DILATE) return rec_rounds<true>(host[0], (const RecImage *)dsc.ptr, n, (int)tiles, se, s);
+  return rec_rounds<false>(host[0], (const RecImage *)dsc.ptr, n, (int)tiles, se, s);
+}

@@ -1,0 +1,259 @@
+// sq_u8_tile.hpp -- device code of one 128x128 output tile of the fused uint8 square
+// erosion / dilation (used by the single-image kernel of sq_u8.hip and by the batched
+// kernel of batch.hip).  See sq_u8.hip for the algorithm.
+#pragma once
+#include "pm_common.hpp"
+
+namespace pm {
+
+typedef unsigned short u16x2_t __attribute__((ext_vector_type(2)));
+
+constexpr int FT = 128;       // output tile side
+constexpr int FTHREADS = 256;
+// Transposed buffer T: one 128-byte row per region column, 32-bit word w of row R stored
+// at word (w ^ ((R >> 2) & 31)).  Pass 1 writes rows 4q..4q+3 from lanes of consecutive q
+// and pass 2 reads rows k2*K+t at word q2; with the swizzle both land on distinct banks
+// (a plain pitch maps them onto 8 banks).
+__device__ __forceinline__ int t_off(int R, int w) { return R * FT + ((w ^ ((R >> 2) & 31)) << 2); }
+
+template <bool DIL> __device__ __forceinline__ uint32_t pk(uint32_t a, uint32_t b)
+{
+  const u16x2_t x = __builtin_bit_cast(u16x2_t, a), y = __builtin_bit_cast(u16x2_t, b);
+  return __builtin_bit_cast(uint32_t, DIL ? __builtin_elementwise_max(x, y) : __builtin_elementwise_min(x, y));
+}
+// word (c0,c1,c2,c3) -> (c0,c2) and (c1,c3) as u16x2
+__device__ __forceinline__ uint32_t even_pairs(uint32_t w) { return __builtin_amdgcn_perm(0u, w, 0x0c020c00u); }
+__device__ __forceinline__ uint32_t odd_pairs(uint32_t w) { return __builtin_amdgcn_perm(0u, w, 0x0c030c01u); }
+
+// 4x4 byte transpose: e[j] = (c0_j, c2_j), o[j] = (c1_j, c3_j) for j = 0..3 ->
+// w[c] = (c_0, c_1, c_2, c_3) bytes
+__device__ __forceinline__ void transpose4(const uint32_t *e, const uint32_t *o, uint32_t *w)
+{
+  const uint32_t ea = __builtin_amdgcn_perm(e[1], e[0], 0x06020400u);
+  const uint32_t eb = __builtin_amdgcn_perm(e[3], e[2], 0x06020400u);
+  const uint32_t oa = __builtin_amdgcn_perm(o[1], o[0], 0x06020400u);
+  const uint32_t ob = __builtin_amdgcn_perm(o[3], o[2], 0x06020400u);
+  w[0] = __builtin_amdgcn_perm(eb, ea, 0x05040100u);
+  w[2] = __builtin_amdgcn_perm(eb, ea, 0x07060302u);
+  w[1] = __builtin_amdgcn_perm(ob, oa, 0x05040100u);
+  w[3] = __builtin_amdgcn_perm(ob, oa, 0x07060302u);
+}
+
+__device__ __forceinline__ uint32_t lds_word(const uint8_t *p) { return *reinterpret_cast<const uint32_t *>(p); }
+
+// Windowed max/min of 2r+1 rows for K consecutive output rows of one 4-byte column quad.
+// row(t) returns the quad's 32-bit word in input row t (output row j covers rows j..j+2r).
+template <int K, bool DIL, typename LD>
+__device__ __forceinline__ void vh_run(LD &&row, int r, uint32_t (&e)[K], uint32_t (&o)[K])
+{
+  const uint32_t NEU = DIL ? 0u : 0x00FF00FFu;
+  if (K > 4 || 2 * r >= K) {
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+      const uint32_t w = row(j);
+      e[j] = even_pairs(w);
+      o[j] = odd_pairs(w);
+    }
+#pragma unroll
+    for (int j = K - 2; j >= 0; j--) {
+      e[j] = pk<DIL>(e[j], e[j + 1]);
+      o[j] = pk<DIL>(o[j], o[j + 1]);
+    }
+    // prefix over rows [K, 2r): at most K rows (2r <= 2K for every radius mapped to K).
+    // The loop is unrolled over K rows and predicated, so that its LDS reads issue back to
+    // back instead of one per dependent iteration; rows past 2r-1 are read (they lie inside
+    // the region since K <= 2r and K <= R16 + r) and ignored.
+    uint32_t pe = NEU, po = NEU;
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+      const uint32_t w = row(K + t);
+      const bool use = K + t < 2 * r;
+      pe = use ? pk<DIL>(pe, even_pairs(w)) : pe;
+      po = use ? pk<DIL>(po, odd_pairs(w)) : po;
+    }
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+      const uint32_t w = row(2 * r + j);
+      pe = pk<DIL>(pe, even_pairs(w));
+      po = pk<DIL>(po, odd_pairs(w));
+      e[j] = pk<DIL>(e[j], pe);
+      o[j] = pk<DIL>(o[j], po);
+    }
+  } else if constexpr (K == 4) {
+    // r = 1 (2r < K): direct three-tap form over six rows
+    uint32_t xe[6], xo[6];
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+      const uint32_t w = row(t);
+      xe[t] = even_pairs(w);
+      xo[t] = odd_pairs(w);
+    }
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+      e[j] = pk<DIL>(pk<DIL>(xe[j], xe[j + 1]), xe[j + 2]);
+      o[j] = pk<DIL>(pk<DIL>(xo[j], xo[j + 1]), xo[j + 2]);
+    }
+  }
+}
+
+// Passes 2 and 3 of one tile whose input region (rows y0-r .. y0+FT+r-1, columns
+// x0-R16 .. x0+FT+R16-1 of the image) is in smem (row pitch AW = FT + 2*R16).  dst is the
+// output image (row pitch W).  ALIGNED: W % 16 == 0 and dst 16-byte aligned.
+template <int K, bool DIL, bool ALIGNED>
+__device__ __forceinline__ void sq_tile_passes(uint8_t *smem, uint8_t *__restrict__ dst_img, int H, int W, int r,
+                                               int R16, int y0, int x0)
+{
+  const int AW = FT + 2 * R16;
+  uint8_t *A = smem;
+  uint8_t *T = smem;
+  const int tid = threadIdx.x;
+  // ---- 2. vertical pass.  Every thread keeps the (transposed) results of its at most
+  // MAXT tasks in registers; after a barrier they are written into T, which reuses the
+  // storage of A.  Aliasing the two buffers halves the LDS footprint and doubles the
+  // number of resident workgroups.
+  {
+    constexpr int MAXT = (K == 4) ? 5 : (K == 8) ? 3 : (K == 16) ? 2 : 1;
+    const int q0 = (R16 - r) >> 2, q1 = (R16 + FT + r + 3) >> 2;
+    const int nq = q1 - q0;
+    const int ntask = nq * (FT / K);
+    uint32_t res[MAXT][K];
+#pragma unroll
+    for (int sl = 0; sl < MAXT; sl++) {
+      const int task = tid + sl * FTHREADS;
+      if (task < ntask) {
+        const int k = task / nq, q = q0 + (task - k * nq);
+        uint32_t e[K], o[K];
+        const uint8_t *base = A + (k * K) * AW + 4 * q;
+        vh_run<K, DIL>([&](int t) { return lds_word(base + t * AW); }, r, e, o);
+#pragma unroll
+        for (int g = 0; g < K / 4; g++) transpose4(e + 4 * g, o + 4 * g, &res[sl][4 * g]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int sl = 0; sl < MAXT; sl++) {
+      const int task = tid + sl * FTHREADS;
+      if (task < ntask) {
+        const int k = task / nq, q = q0 + (task - k * nq);
+#pragma unroll
+        for (int g = 0; g < K / 4; g++)
+#pragma unroll
+          for (int c = 0; c < 4; c++)
+            *reinterpret_cast<uint32_t *>(T + t_off(4 * q + c, (k * K >> 2) + g)) = res[sl][4 * g + c];
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- 3. horizontal pass over T, transposed back in registers, stored to HBM
+  {
+    constexpr int NR = FT / K;
+    const int ntask = (FT / 4) * NR;
+    for (int task = tid; task < ntask; task += FTHREADS) {
+      const int q2 = task / NR, k2 = task - q2 * NR;
+      uint32_t e[K], o[K];
+      const int rbase = k2 * K + R16 - r;
+      vh_run<K, DIL>([&](int t) { return lds_word(T + t_off(rbase + t, q2)); }, r, e, o);
+      const int gx = x0 + k2 * K;
+      uint32_t rows[4][K / 4];
+#pragma unroll
+      for (int g = 0; g < K / 4; g++) {
+        uint32_t w[4];
+        transpose4(e + 4 * g, o + 4 * g, w);
+#pragma unroll
+        for (int c = 0; c < 4; c++) rows[c][g] = w[c];
+      }
+      // W is a multiple of 16: a 16-byte (or 4-byte) chunk starting inside the row is whole
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const int gy = y0 + 4 * q2 + c;
+        if (gy >= H) continue;
+        uint8_t *dst = dst_img + (int64_t)gy * W + gx;
+        if (ALIGNED) {
+          if (K >= 16) {
+#pragma unroll
+            for (int hh = 0; hh < K / 16; hh++)
+              if (gx + 16 * hh < W)
+                *reinterpret_cast<uint4 *>(dst + 16 * hh) =
+                    make_uint4(rows[c][4 * hh], rows[c][4 * hh + 1], rows[c][4 * hh + 2], rows[c][4 * hh + 3]);
+          } else {
+#pragma unroll
+            for (int g = 0; g < K / 4; g++)
+              if (gx + 4 * g < W) *reinterpret_cast<uint32_t *>(dst + 4 * g) = rows[c][g];
+          }
+        } else {
+          // any width / alignment: byte stores, clipped at the right edge
+#pragma unroll
+          for (int g = 0; g < K / 4; g++)
+#pragma unroll
+            for (int bb = 0; bb < 4; bb++)
+              if (gx + 4 * g + bb < W) dst[4 * g + bb] = (uint8_t)(rows[c][g] >> (8 * bb));
+        }
+      }
+    }
+  }
+}
+
+// Copy the input region of a tile into smem (any width / alignment).  16-byte chunks that
+// are aligned and fully inside a row use one 128-bit load; others are read bytewise; pixels
+// outside the image get the neutral value.
+template <bool DIL>
+__device__ __forceinline__ void sq_load_region(uint8_t *smem, const uint8_t *__restrict__ src, int H, int W, int r,
+                                               int R16, int y0, int x0)
+{
+  const int AW = FT + 2 * R16, AH = FT + 2 * r, cpr = AW >> 4;
+  const uint8_t nb = DIL ? 0 : 255;
+  const uint32_t nv = DIL ? 0u : 0xFFFFFFFFu;
+  constexpr int BATCH = 4;
+  const int n = AH * cpr;
+  for (int c0 = threadIdx.x; c0 < n; c0 += BATCH * FTHREADS) {
+    uint4 v[BATCH];
+#pragma unroll
+    for (int u = 0; u < BATCH; u++) {
+      const int c = c0 + u * FTHREADS;
+      v[u] = make_uint4(nv, nv, nv, nv);
+      if (c < n) {
+        const int ri = c / cpr, cc = c - ri * cpr;
+        const int gy = y0 - r + ri, gx = x0 - R16 + (cc << 4);
+        if ((unsigned)gy < (unsigned)H) {
+          const uint8_t *p = src + (int64_t)gy * W + gx;
+          if (gx >= 0 && gx + 16 <= W && ((uintptr_t)p & 15) == 0) {
+            v[u] = *reinterpret_cast<const uint4 *>(p);
+          } else {
+            uint32_t wd[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+              uint32_t acc = 0;
+#pragma unroll
+              for (int bb = 0; bb < 4; bb++) {
+                const int x = gx + 4 * k + bb;
+                const uint32_t byte = ((unsigned)x < (unsigned)W) ? p[4 * k + bb] : nb;
+                acc |= byte << (8 * bb);
+              }
+              wd[k] = acc;
+            }
+            v[u] = make_uint4(wd[0], wd[1], wd[2], wd[3]);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < BATCH; u++) {
+      const int c = c0 + u * FTHREADS;
+      if (c < n) {
+        const int ri = c / cpr, cc = c - ri * cpr;
+        *reinterpret_cast<uint4 *>(smem + ri * AW + (cc << 4)) = v[u];
+      }
+    }
+  }
+}
+
+
+constexpr int SQ_TILE_RMAX = 32;
+inline size_t sq_tile_lds_bytes(int r, int R16)
+{
+  const size_t a = (size_t)(FT + 2 * r) * (FT + 2 * R16), t = (size_t)(FT + 2 * R16) * FT;
+  return a > t ? a : t;
+}
+
+}  // namespace pm

@@ -72,7 +72,6 @@ def test_finite_differences_tie_free(k, nonflat, dilate):
     wg = w.clone().requires_grad_(True) if w is not None else None
     (f(xg, k=k, w=wg) * g).sum().backward()
     # windows whose best and second-best candidates are more than 2*eps apart
-    u = ag.reference_cpu(x.cpu(), k=k, w=None if w is None else w.cpu(), dilate=dilate)  # noqa: F841
     r = (k - 1) // 2
     xp = torch.nn.functional.pad(x.cpu().reshape(1, 1, 32, 32), (r, r, r, r),
                                  value=float("-inf") if dilate else float("inf"))
@@ -80,22 +79,25 @@ def test_finite_differences_tie_free(k, nonflat, dilate):
     if w is not None:
         cand = cand + w.cpu().reshape(-1, 1) if dilate else cand - w.cpu().reshape(-1, 1)
     srt = cand.sort(dim=0, descending=dilate).values
-    gap = (srt[0] - srt[1]).abs()
-    if (gap <= 2 * eps).any():
-        pytest.skip("input has near-ties; choose another seed")
+    gap = (srt[0] - srt[1]).abs().reshape(32, 32)
+    tie = (gap <= 2 * eps).float().reshape(1, 1, 32, 32)
+    # an input pixel is checked only if no window that can select it has a near tie
+    near = torch.nn.functional.max_pool2d(tie, k, stride=1, padding=r).reshape(-1) > 0
+    ok = np.flatnonzero(~near.numpy())
+    assert ok.size > 100
 
     def F(xx):
         return float((f(xx, k=k, w=w).double() * g.double()).sum())
 
     rng = np.random.default_rng(0)
-    for idx in rng.choice(32 * 32, size=12, replace=False):
+    for idx in rng.choice(ok, size=12, replace=False):
         d = torch.zeros_like(x).view(-1)
         d[int(idx)] = eps
         d = d.view_as(x)
         fd = (F(x + d) - F(x - d)) / (2 * eps)
         an = float(xg.grad.view(-1)[int(idx)])
         assert abs(fd - an) <= 1e-3 * max(1.0, abs(an)), (idx, fd, an)
-    if w is not None:
+    if w is not None and not near.any():
         for t in range(k * k):
             dw = torch.zeros_like(w).view(-1)
             dw[t] = eps
