@@ -76,23 +76,16 @@ def bench_dilate_8k(pm, torch, steps, warmup, dev):
         per_side[str(s)] = {"ms": round(t, 4), "gpx_s": round(npx / (t * 1e-3) / 1e9, 2),
                             "pct_hbm": round(100 * 2 * npx / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 2)}
 
-    # dominant kernel: one line pass (one launch) over the 8K image, side 15 (r = 7)
-    kern = {}
-    for axis, fn in (("x", pm.dilateX), ("y", pm.dilateY)):
-        def lp(fn=fn):
-            a, b = pairs[state["i"] % 3]
-            state["i"] += 1
-            fn(a, 7, out=b)
-        lp()
-        t = timed(lp, 10, stream) / 10
-        kern[axis] = t
-    dom_axis = max(kern, key=kern.get)
-    t_dom = kern[dom_axis]
+    # dominant kernel: the single fused launch of the slowest side (sq_u8_kernel<K, true>)
+    dom = max(per_side, key=lambda k: per_side[k]["ms"])
+    t_dom = per_side[dom]["ms"]
+    r_dom = (int(dom) - 1) // 2
+    kk = 32 if r_dom >= 16 else 16 if r_dom >= 8 else 8 if r_dom >= 4 else 4
     achieved = 2 * npx / (t_dom * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": f"line_kernel<uint8_t,dilate,axis={dom_axis}> r=7 on 8192x8192",
-                "kernel_ms": round(t_dom, 4), "bytes_per_launch": 2 * npx}
+                "kernel": f"sq_u8_kernel<{kk}, true> (side {dom}, r={r_dom}) on 8192x8192 uint8",
+                "kernel_ms": t_dom, "bytes_per_launch": 2 * npx, "side": int(dom)}
     # op-level fraction of HBM peak over the whole step (2 B/px algorithmic)
     pct = 100 * 2 * npx * len(SIDES) * steps / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS
     del pairs, base
@@ -146,6 +139,179 @@ def bench_recon_4k(pm, torch, dev, reps=3):
     return res
 
 
+def bench_batched(pm, torch, dev, reps=3):
+    """512 uint8 images, H, W ~ U{64..2048}, square side ~ U{3,5,..,31}: dilation + opening per
+    image in one launch vs the per-image path; 256 h-maxima reconstructions (sizes
+    U{128..1024}, blurred-noise masks, h = 16) in one round sequence vs one call per image."""
+    import numpy as np
+    from poutre2_amd import synth
+    g = synth.rng(5)
+    shapes = [(int(g.integers(64, 2049)), int(g.integers(64, 2049))) for _ in range(512)]
+    radii = [int(g.integers(1, 16)) for _ in range(512)]
+    imgs = [torch.from_numpy(g.integers(0, 256, size=s, dtype=np.uint8)).to(dev) for s in shapes]
+    dil = [torch.empty_like(a) for a in imgs]
+    opn = [torch.empty_like(a) for a in imgs]
+    npx = sum(a.numel() for a in imgs)
+    stream = torch.cuda.current_stream(dev)
+    fb = lambda: pm.batch_dilate_open(imgs, radii, dil, opn)  # noqa: E731
+    fb()
+    tb = timed(fb, reps, stream) / reps
+
+    def per_image():
+        for a, r, d, o in zip(imgs, radii, dil, opn):
+            pm.dilate(a, pm.static_nl_se.square2d, r, out=d)
+            pm.opening(a, pm.static_nl_se.square2d, r, out=o)
+    per_image()
+    tp = timed(per_image, 1, stream)
+    res = {"dilate_open_512": {"mpx": round(npx / 1e6, 1), "ms": round(tb, 3),
+                               "gpx_s": round(npx / (tb * 1e-3) / 1e9, 2),
+                               "pct_hbm": round(100 * 3 * npx / (tb * 1e-3) / 1e9 / HBM_PEAK_GBS, 2),
+                               "per_image_ms": round(tp, 3), "speedup_vs_per_image": round(tp / tb, 2)}}
+    del imgs, dil, opn
+    g = synth.rng(6)
+    masks, markers = [], []
+    for i in range(256):
+        H, W = int(g.integers(128, 1025)), int(g.integers(128, 1025))
+        m = synth.blurred_noise_mask_fast(H, W, 1000 + i)
+        masks.append(torch.from_numpy(m).to(dev))
+        markers.append(torch.from_numpy(synth.hmax_marker(m, 16)).to(dev))
+    outs = [torch.empty_like(m) for m in masks]
+    rpx = sum(m.numel() for m in masks)
+    fr = lambda: pm.batch_reconstruction(pm.RecType.dilate, markers, masks, pm.static_nl_se.square2d, outs)  # noqa: E731
+    fr()
+    tr = timed(fr, reps, stream) / reps
+
+    def per_rec():
+        for mk, ms, o in zip(markers, masks, outs):
+            pm.reconstruction(pm.RecType.dilate, mk, ms, pm.static_nl_se.square2d, out=o)
+    per_rec()
+    tpr = timed(per_rec, 1, stream)
+    res["reconstruct_256"] = {"mpx": round(rpx / 1e6, 1), "ms": round(tr, 3), "gpx_s": round(rpx / (tr * 1e-3) / 1e9, 2),
+                              "pct_hbm": round(100 * 3 * rpx / (tr * 1e-3) / 1e9 / HBM_PEAK_GBS, 2),
+                              "per_image_ms": round(tpr, 3), "speedup_vs_per_image": round(tpr / tr, 2)}
+    return res
+
+
+def bench_autograd(pm, torch, dev, reps=3):
+    """float32 [64, 32, 256, 256] (N(0,1) + 1e-4 jitter, seed 7): flat 3x3 / 7x7 dilation and
+    erosion and 5x5 max-plus dilation, forward and forward+backward.  8 B/px forward,
+    12 B/px backward."""
+    from poutre2_amd import autograd as ag
+    from poutre2_amd import synth
+    x = torch.from_numpy(synth.normal_f32((64, 32, 256, 256), 7)).to(dev)
+    gy = torch.ones_like(x)
+    w = torch.from_numpy((0.1 * synth.rng(8).standard_normal((5, 5))).astype("float32")).to(dev)
+    n = x.numel()
+    stream = torch.cuda.current_stream(dev)
+    res = {}
+    for name, f, k, ww in (("dilate3", ag.dilation2d, 3, None), ("erode3", ag.erosion2d, 3, None),
+                           ("dilate7", ag.dilation2d, 7, None), ("erode7", ag.erosion2d, 7, None),
+                           ("maxplus5", ag.dilation2d, 5, w)):
+        xf = x.detach()
+        fw = lambda: f(xf, k=k, w=ww)  # noqa: E731
+        fw()
+        tf = timed(fw, reps, stream) / reps
+        xr = x.detach().requires_grad_(True)
+        wr = ww.detach().requires_grad_(True) if ww is not None else None
+
+        def fb():
+            xr.grad = None
+            if wr is not None:
+                wr.grad = None
+            y = f(xr, k=k, w=wr)
+            y.backward(gy)
+        fb()
+        fb()
+        tfb = timed(fb, reps, stream) / reps
+        res[name] = {"fwd_ms": round(tf, 3), "fwd_gpx_s": round(n / (tf * 1e-3) / 1e9, 2),
+                     "fwd_pct_hbm": round(100 * 8 * n / (tf * 1e-3) / 1e9 / HBM_PEAK_GBS, 2),
+                     "fwdbwd_ms": round(tfb, 3), "fwdbwd_gpx_s": round(n / (tfb * 1e-3) / 1e9, 2),
+                     "fwdbwd_pct_hbm": round(100 * 20 * n / (tfb * 1e-3) / 1e9 / HBM_PEAK_GBS, 2)}
+    # the plain per-image path (non-differentiable Dilate per 256x256 image) for the 3x3 forward
+    imgs = x.view(-1, 256, 256)
+    outp = torch.empty_like(imgs)
+
+    def plain():
+        for i in range(imgs.shape[0]):
+            pm.dilate(imgs[i], pm.static_nl_se.square2d, 1, out=outp[i])
+    plain()
+    tp = timed(plain, 1, stream)
+    res["dilate3"]["plain_per_image_fwd_ms"] = round(tp, 3)
+    res["dilate3"]["fwd_speedup_vs_per_image"] = round(tp / res["dilate3"]["fwd_ms"], 2)
+    return res
+
+
+def pmc_child(side):
+    """Workload profiled by measure_traffic: a plain elementwise uint8 kernel of known traffic
+    (bitwise_not over 64 MiB: 64 MiB read + 64 MiB written) and the 8K dilation of one side."""
+    import torch
+    import poutre2_amd as pm
+    from poutre2_amd import synth
+    dev = torch.device("cuda:0")
+    base = torch.from_numpy(synth.uniform_u8((8192, 8192), 2)).to(dev)
+    pairs = [(base.roll(i * 977, dims=1).contiguous(), torch.empty_like(base)) for i in range(3)]
+    for i in range(6):
+        a, b = pairs[i % 3]
+        torch.bitwise_not(a, out=b)
+    for i in range(6):
+        a, b = pairs[i % 3]
+        pm.dilate(a, pm.static_nl_se.square2d, (side - 1) // 2, out=b)
+    torch.cuda.synchronize()
+    print("pmc child done", flush=True)
+
+
+def _pmc_pass(counter, side, outdir):
+    import csv
+    import glob
+    import subprocess
+    env = dict(os.environ, TMPDIR="/tmp")
+    cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", outdir, "-o", "run", "--",
+           sys.executable, os.path.abspath(__file__), "--pmc-child", str(side)]
+    subprocess.run(cmd, env=env, cwd="/tmp", stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=120,
+                   check=False)
+    files = glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise RuntimeError(f"rocprofv3 --pmc {counter}: no counter file")
+    vals = {"copy": [], "dil": []}
+    with open(files[0]) as f:
+        for row in csv.DictReader(f):
+            if row["Counter_Name"] != counter:
+                continue
+            name = row["Kernel_Name"]
+            if "bitwise_not" in name:
+                vals["copy"].append(float(row["Counter_Value"]))
+            elif "sq_u8_kernel" in name:
+                vals["dil"].append(float(row["Counter_Value"]))
+    if len(vals["copy"]) < 2 or len(vals["dil"]) < 2:
+        raise RuntimeError(f"rocprofv3 --pmc {counter}: kernels not found")
+    # first dispatch of each kernel is a cold start; average the rest
+    return (sum(vals["copy"][1:]) / (len(vals["copy"]) - 1), sum(vals["dil"][1:]) / (len(vals["dil"]) - 1))
+
+
+def measure_traffic(side):
+    """HBM bytes per launch of the dominant kernel from two rocprofv3 --pmc passes (FETCH_SIZE,
+    then WRITE_SIZE, KiB per dispatch), each scaled by the factor that turns the same counter on
+    a plain elementwise copy of 64 MiB into the bytes that copy moves."""
+    import shutil
+    import tempfile
+    if shutil.which("rocprofv3") is None:
+        return None, "rocprofv3 not found"
+    npx = 8192 * 8192
+    out = {}
+    tmp = tempfile.mkdtemp(prefix="pm_pmc_", dir="/tmp")
+    try:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            copy_kib, dil_kib = _pmc_pass(counter, side, os.path.join(tmp, counter))
+            factor = npx / (copy_kib * 1024.0)
+            out[counter] = {"copy_kib": round(copy_kib, 1), "kernel_kib": round(dil_kib, 1),
+                            "factor": round(factor, 4), "bytes": dil_kib * 1024.0 * factor}
+    except Exception as e:  # noqa: BLE001 - traffic is optional; report why it is missing
+        return None, f"{type(e).__name__}: {e}"
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return out["FETCH_SIZE"]["bytes"] + out["WRITE_SIZE"]["bytes"], out
+
+
 def cpu_baseline_8k():
     """Single-thread oracle (restatement of DilateX + DilateY, ero_dil_line_se_t.hpp:49-280)
     on a bounded sample of the headline workload: one 8192x8192 uint8 dilation per side."""
@@ -173,8 +339,13 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--sections", default="dilate,ops,recon")
+    ap.add_argument("--sections", default="dilate,ops,recon,batch,autograd")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc passes")
+    ap.add_argument("--pmc-child", type=int, default=0, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.pmc_child:
+        pmc_child(args.pmc_child)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU path is a design "
               "(row bands + halo exchange) and is not built in this version.", file=sys.stderr)
@@ -186,12 +357,23 @@ def main():
     log("bench: 8192x8192 u8 dilation, sides", SIDES)
     gpx, ms_step, per_side, roofline, pct = bench_dilate_8k(pm, torch, args.steps, args.warmup, dev)
     extra = {"dilate_per_side": per_side, "pct_hbm": pct}
+    if not args.no_traffic:
+        log("bench: rocprofv3 FETCH_SIZE / WRITE_SIZE passes on the dominant kernel")
+        traffic, detail = measure_traffic(roofline["side"])
+        roofline["traffic"] = None if traffic is None else round(traffic)
+        roofline["traffic_detail"] = detail
     if "ops" in sections:
         log("bench: erode/open/close/gradient 8K")
         extra["ops_8k"] = bench_ops_8k(pm, torch, dev)
     if "recon" in sections:
         log("bench: 4K reconstruction / leveling")
         extra["geodesy_4k"] = bench_recon_4k(pm, torch, dev)
+    if "batch" in sections:
+        log("bench: batched 512 dilate+open, 256 reconstructions")
+        extra["batched"] = bench_batched(pm, torch, dev)
+    if "autograd" in sections:
+        log("bench: autograd [64,32,256,256] fp32")
+        extra["autograd"] = bench_autograd(pm, torch, dev)
     cpu = None
     if not args.no_cpu:
         log("bench: cpu baseline")

@@ -6,7 +6,8 @@
 //
 // One workgroup (256 threads) owns a 128x128 output tile.
 //  1. The input region (128+2r rows, 128+2*R16 columns, R16 = r rounded up to 16 so that
-//     global loads are 16-byte aligned) is copied to LDS with 128-bit loads; pixels
+//     global loads of aligned rows are 16-byte aligned) is copied to LDS with 128-bit
+//     loads (dword-aligned loads realigned in registers when rows are not); pixels
 //     outside the image get the neutral value (0 for dilation, 255 for erosion).
 //  2. Vertical pass: a thread owns a 4-column quad and a run of K output rows.  It reads
 //     K+2r words down the column, unpacks every word into two u16x2 (columns 0/2 and 1/3)
@@ -45,7 +46,7 @@ __device__ __forceinline__ TileSched tile_sched(int b, int G, int ntiles)
   return {lo + j, gx, hi};
 }
 
-template <int K, bool DIL>
+template <int K, bool DIL, bool ALIGNED>
 __global__ __launch_bounds__(FTHREADS) void sq_u8_kernel(const uint8_t *__restrict__ in, uint8_t *__restrict__ out,
                                                          int H, int W, int r, int R16, int tiles_x, int tiles_per_img,
                                                          int ntiles)
@@ -77,9 +78,13 @@ __global__ __launch_bounds__(FTHREADS) void sq_u8_kernel(const uint8_t *__restri
       if (c < nchunk) {
         const int ri = c / cpr, cc = c - ri * cpr;
         const int gy = ty * FT - r + ri, gx = tx * FT - R16 + (cc << 4);
-        // W is a multiple of 16, so a 16-byte chunk is all inside or all outside
-        if ((unsigned)gy < (unsigned)H && (unsigned)gx < (unsigned)W)
-          pre[i] = *reinterpret_cast<const uint4 *>(src + (int64_t)gy * W + gx);
+        if (ALIGNED) {
+          // W is a multiple of 16, so a 16-byte chunk is all inside or all outside
+          if ((unsigned)gy < (unsigned)H && (unsigned)gx < (unsigned)W)
+            pre[i] = *reinterpret_cast<const uint4 *>(src + (int64_t)gy * W + gx);
+        } else {
+          pre[i] = load_chunk_any(src, H, W, gy, gx, DIL ? 0 : 255);
+        }
       }
     }
   };
@@ -105,7 +110,7 @@ __global__ __launch_bounds__(FTHREADS) void sq_u8_kernel(const uint8_t *__restri
     __syncthreads();
     if (tile + ts.step < ts.end) fetch(tile + ts.step);
 
-    sq_tile_passes<K, DIL, true>(smem, out + ibase, H, W, r, R16, y0, x0);
+    sq_tile_passes<K, DIL, ALIGNED>(smem, out + ibase, H, W, r, R16, y0, x0);
     __syncthreads();  // A/T are rewritten by the next tile
   }
 }
@@ -124,29 +129,35 @@ static int sq_u8_once(const uint8_t *in, uint8_t *out, int64_t B, int64_t H, int
   PM_HIP(hipGetDevice(&dev));
   PM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   const dim3 block(FTHREADS);
-#define PM_SQ_LAUNCH(KK)                                                                                     \
-  do {                                                                                                       \
-    int occ = 0;                                                                                             \
-    PM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, sq_u8_kernel<KK, DIL>, FTHREADS, lds));        \
-    int64_t g = (int64_t)(occ > 0 ? occ : 1) * cus;                                                          \
-    if (g >= 8) g &= ~(int64_t)7;                                                                            \
-    if (g > ntiles) g = ntiles;                                                                              \
-    hipLaunchKernelGGL((sq_u8_kernel<KK, DIL>), dim3((unsigned)g), block, lds, s, in, out, (int)H, (int)W, r, \
-                       R16, tiles_x, (int)per, (int)ntiles);                                                 \
+  const bool aligned = (W % 16) == 0 && ((uintptr_t)in % 16) == 0 && ((uintptr_t)out % 16) == 0;
+#define PM_SQ_LAUNCH(KK, AL)                                                                                      \
+  do {                                                                                                            \
+    int occ = 0;                                                                                                  \
+    PM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, sq_u8_kernel<KK, DIL, AL>, FTHREADS, lds));         \
+    int64_t g = (int64_t)(occ > 0 ? occ : 1) * cus;                                                               \
+    if (g >= 8) g &= ~(int64_t)7;                                                                                 \
+    if (g > ntiles) g = ntiles;                                                                                   \
+    hipLaunchKernelGGL((sq_u8_kernel<KK, DIL, AL>), dim3((unsigned)g), block, lds, s, in, out, (int)H, (int)W, r, \
+                       R16, tiles_x, (int)per, (int)ntiles);                                                      \
   } while (0)
-  if (r >= 16) PM_SQ_LAUNCH(32);
-  else if (r >= 8) PM_SQ_LAUNCH(16);
-  else if (r >= 4) PM_SQ_LAUNCH(8);
-  else PM_SQ_LAUNCH(4);
+#define PM_SQ_LAUNCH_K(AL)            \
+  do {                                \
+    if (r >= 16) PM_SQ_LAUNCH(32, AL); \
+    else if (r >= 8) PM_SQ_LAUNCH(16, AL); \
+    else if (r >= 4) PM_SQ_LAUNCH(8, AL); \
+    else PM_SQ_LAUNCH(4, AL);          \
+  } while (0)
+  if (aligned) PM_SQ_LAUNCH_K(true);
+  else PM_SQ_LAUNCH_K(false);
+#undef PM_SQ_LAUNCH_K
 #undef PM_SQ_LAUNCH
   PM_HIP(hipGetLastError());
   return PM_OK;
 }
 
-bool sq_u8_applicable(const void *in, const void *out, int64_t H, int64_t W)
+bool sq_u8_applicable(const void *, const void *, int64_t H, int64_t W)
 {
-  return (W % 16) == 0 && W > 0 && H > 0 && H <= INT32_MAX / 2 && W <= INT32_MAX / 2 &&
-         ((uintptr_t)in % 16) == 0 && ((uintptr_t)out % 16) == 0;
+  return W > 0 && H > 0 && H <= INT32_MAX / 2 && W <= INT32_MAX / 2;
 }
 
 template <bool DIL>

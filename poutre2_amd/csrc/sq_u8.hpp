@@ -6,7 +6,8 @@ namespace pm {
 
 constexpr int64_t SQ_U8_RMAX = 32;  // largest radius handled by one launch
 
-// True if the fused kernel can run on these buffers (16-byte aligned rows).
+// True if the fused kernel can run on these buffers (any width and alignment; images whose
+// rows are 16-byte aligned take the 128-bit load/store variant).
 bool sq_u8_applicable(const void *in, const void *out, int64_t H, int64_t W);
 // (2r+1)^2 square on B images of H x W; r > SQ_U8_RMAX runs several launches.
 template <bool DIL> int sq_u8(const uint8_t *in, uint8_t *out, int64_t B, int64_t H, int64_t W, int64_t r, hipStream_t s);

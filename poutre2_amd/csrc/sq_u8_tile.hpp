@@ -41,6 +41,82 @@ __device__ __forceinline__ void transpose4(const uint32_t *e, const uint32_t *o,
 
 __device__ __forceinline__ uint32_t lds_word(const uint8_t *p) { return *reinterpret_cast<const uint32_t *>(p); }
 
+// Four dwords at a 4-byte aligned address: one global_load_dwordx4 (multi-dword accesses
+// need dword alignment only).
+typedef unsigned int u32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
+
+// 16 bytes at any address p, all of them inside the image.  The dwords are read from
+// p rounded down to 4 (the fifth dword only when p is not 4-aligned: it then holds at
+// least one of the 16 bytes, so every dword read contains an image byte and lies inside
+// the image's allocation) and realigned with v_alignbyte_b32.
+__device__ __forceinline__ uint4 load16_any(const uint8_t *p)
+{
+  const uint32_t sh = (uint32_t)((uintptr_t)p & 3);
+  const uint32_t *a = reinterpret_cast<const uint32_t *>(p - sh);
+  const u32x4_a4 v = *reinterpret_cast<const u32x4_a4 *>(a);
+  const uint32_t v4 = sh ? a[4] : 0u;
+  return make_uint4(__builtin_amdgcn_alignbyte(v.y, v.x, sh), __builtin_amdgcn_alignbyte(v.z, v.y, sh),
+                    __builtin_amdgcn_alignbyte(v.w, v.z, sh), __builtin_amdgcn_alignbyte(v4, v.w, sh));
+}
+
+// One 16-byte region chunk starting at image column gx of row gy (any width / alignment):
+// bytes outside the image get the neutral value nb.  Chunks wholly inside a row take
+// load16_any, chunks straddling the left or right edge are read bytewise.
+__device__ __forceinline__ uint4 load_chunk_any(const uint8_t *__restrict__ src, int H, int W, int gy, int gx,
+                                                uint8_t nb)
+{
+  const uint32_t nv = nb * 0x01010101u;
+  if ((unsigned)gy >= (unsigned)H || gx + 16 <= 0 || gx >= W) return make_uint4(nv, nv, nv, nv);
+  const uint8_t *p = src + (int64_t)gy * W + gx;
+  if (gx >= 0 && gx + 16 <= W) return load16_any(p);
+  uint32_t wd[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    uint32_t acc = 0;
+#pragma unroll
+    for (int bb = 0; bb < 4; bb++) {
+      const int x = gx + 4 * k + bb;
+      const uint32_t byte = ((unsigned)x < (unsigned)W) ? p[4 * k + bb] : nb;
+      acc |= byte << (8 * bb);
+    }
+    wd[k] = acc;
+  }
+  return make_uint4(wd[0], wd[1], wd[2], wd[3]);
+}
+
+// Store 4*NW bytes w (little-endian across words) at any address dst.  When dst is not
+// 4-aligned, the bytes before the first dword boundary and after the last one are written
+// with byte / short stores and the NW-1 dwords in between with dword stores of realigned
+// words; sub-dword stores only touch their own bytes, so neighbouring rows and tiles that
+// share a dword do not race.
+template <int NW> __device__ __forceinline__ void store_any(uint8_t *dst, const uint32_t (&w)[NW])
+{
+  const uint32_t s = (uint32_t)((uintptr_t)dst & 3);
+  if (s == 0) {
+#pragma unroll
+    for (int m = 0; m < NW; m++) reinterpret_cast<uint32_t *>(dst)[m] = w[m];
+    return;
+  }
+  const uint32_t hs = 4 - s;  // head bytes
+  uint32_t *mid = reinterpret_cast<uint32_t *>(dst + hs);
+#pragma unroll
+  for (int m = 0; m < NW - 1; m++) mid[m] = __builtin_amdgcn_alignbyte(w[m + 1], w[m], hs);
+  uint8_t *tail = dst + hs + 4 * (NW - 1);
+  const uint32_t tw = w[NW - 1] >> (8 * hs);  // the last s bytes
+  if (s == 1) {
+    dst[0] = (uint8_t)w[0];
+    *reinterpret_cast<uint16_t *>(dst + 1) = (uint16_t)(w[0] >> 8);
+    tail[0] = (uint8_t)tw;
+  } else if (s == 2) {
+    *reinterpret_cast<uint16_t *>(dst) = (uint16_t)w[0];
+    *reinterpret_cast<uint16_t *>(tail) = (uint16_t)tw;
+  } else {
+    dst[0] = (uint8_t)w[0];
+    *reinterpret_cast<uint16_t *>(tail) = (uint16_t)tw;
+    tail[2] = (uint8_t)(tw >> 16);
+  }
+}
+
 // Windowed max/min of 2r+1 rows for K consecutive output rows of one 4-byte column quad.
 // row(t) returns the quad's 32-bit word in input row t (output row j covers rows j..j+2r).
 template <int K, bool DIL, typename LD>
@@ -181,8 +257,10 @@ __device__ __forceinline__ void sq_tile_passes(uint8_t *smem, uint8_t *__restric
             for (int g = 0; g < K / 4; g++)
               if (gx + 4 * g < W) *reinterpret_cast<uint32_t *>(dst + 4 * g) = rows[c][g];
           }
+        } else if (gx + K <= W) {
+          store_any<K / 4>(dst, rows[c]);
         } else {
-          // any width / alignment: byte stores, clipped at the right edge
+          // the run crosses the right edge: byte stores, clipped
 #pragma unroll
           for (int g = 0; g < K / 4; g++)
 #pragma unroll
@@ -194,16 +272,12 @@ __device__ __forceinline__ void sq_tile_passes(uint8_t *smem, uint8_t *__restric
   }
 }
 
-// Copy the input region of a tile into smem (any width / alignment).  16-byte chunks that
-// are aligned and fully inside a row use one 128-bit load; others are read bytewise; pixels
-// outside the image get the neutral value.
+// Copy the input region of a tile into smem (any width / alignment, load_chunk_any).
 template <bool DIL>
 __device__ __forceinline__ void sq_load_region(uint8_t *smem, const uint8_t *__restrict__ src, int H, int W, int r,
                                                int R16, int y0, int x0)
 {
   const int AW = FT + 2 * R16, AH = FT + 2 * r, cpr = AW >> 4;
-  const uint8_t nb = DIL ? 0 : 255;
-  const uint32_t nv = DIL ? 0u : 0xFFFFFFFFu;
   constexpr int BATCH = 4;
   const int n = AH * cpr;
   for (int c0 = threadIdx.x; c0 < n; c0 += BATCH * FTHREADS) {
@@ -211,31 +285,9 @@ __device__ __forceinline__ void sq_load_region(uint8_t *smem, const uint8_t *__r
 #pragma unroll
     for (int u = 0; u < BATCH; u++) {
       const int c = c0 + u * FTHREADS;
-      v[u] = make_uint4(nv, nv, nv, nv);
-      if (c < n) {
-        const int ri = c / cpr, cc = c - ri * cpr;
-        const int gy = y0 - r + ri, gx = x0 - R16 + (cc << 4);
-        if ((unsigned)gy < (unsigned)H) {
-          const uint8_t *p = src + (int64_t)gy * W + gx;
-          if (gx >= 0 && gx + 16 <= W && ((uintptr_t)p & 15) == 0) {
-            v[u] = *reinterpret_cast<const uint4 *>(p);
-          } else {
-            uint32_t wd[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-              uint32_t acc = 0;
-#pragma unroll
-              for (int bb = 0; bb < 4; bb++) {
-                const int x = gx + 4 * k + bb;
-                const uint32_t byte = ((unsigned)x < (unsigned)W) ? p[4 * k + bb] : nb;
-                acc |= byte << (8 * bb);
-              }
-              wd[k] = acc;
-            }
-            v[u] = make_uint4(wd[0], wd[1], wd[2], wd[3]);
-          }
-        }
-      }
+      const int ri = c / cpr, cc = c - ri * cpr;
+      // c >= n: a row below the region (gy >= H is not guaranteed), so clamp it out explicitly
+      v[u] = load_chunk_any(src, c < n ? H : 0, W, y0 - r + ri, x0 - R16 + (cc << 4), DIL ? 0 : 255);
     }
 #pragma unroll
     for (int u = 0; u < BATCH; u++) {
@@ -247,7 +299,6 @@ __device__ __forceinline__ void sq_load_region(uint8_t *smem, const uint8_t *__r
     }
   }
 }
-
 
 constexpr int SQ_TILE_RMAX = 32;
 inline size_t sq_tile_lds_bytes(int r, int R16)

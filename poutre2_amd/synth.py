@@ -47,6 +47,21 @@ def blurred_noise_mask(H, W, seed, sigma=8.0, radius=32):
     noise = rng(seed).standard_normal((H, W))
     k = gaussian_kernel(sigma, radius)
     b = _blur_axis(_blur_axis(noise, k, 0), k, 1)
+    return _rescale_u8(b)
+
+
+def blurred_noise_mask_fast(H, W, seed, sigma=8.0, radius=32):
+    """Same recipe as blurred_noise_mask with scipy's separable correlation (mode "mirror"),
+    for the large batched workloads; float64 summation order differs, so the two may differ
+    by one grey level at exact .5 rounding ties."""
+    from scipy.ndimage import correlate1d
+    noise = rng(seed).standard_normal((H, W))
+    k = gaussian_kernel(sigma, radius)
+    b = correlate1d(correlate1d(noise, k, axis=0, mode="mirror"), k, axis=1, mode="mirror")
+    return _rescale_u8(b)
+
+
+def _rescale_u8(b):
     lo, hi = b.min(), b.max()
     scaled = (b - lo) * (255.0 / (hi - lo)) if hi > lo else np.zeros_like(b)
     return np.rint(scaled).clip(0, 255).astype(np.uint8)

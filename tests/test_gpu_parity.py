@@ -275,7 +275,8 @@ def test_errors_follow_reference_checks():
 
 
 # ----------------------------------------------------------------------------- fused uint8 square kernel
-@pytest.mark.parametrize("shape", [(256, 272), (130, 48), (517, 1008), (16, 16), (300, 16)])
+@pytest.mark.parametrize("shape", [(256, 272), (130, 48), (517, 1008), (16, 16), (300, 16),
+                                   (131, 37), (77, 1001), (3, 5), (129, 130), (260, 259)])
 @pytest.mark.parametrize("r", [1, 2, 3, 4, 7, 8, 15, 16, 31, 32, 33, 70])
 def test_square_u8_fused(shape, r):
     img = rand(shape, np.uint8, 60 + r)
@@ -294,10 +295,21 @@ def test_square3d_u8_fused(it):
                                       oracle.static_iter(img, "SESquare3D", it, dil))
 
 
-def test_square_u8_unaligned_view_falls_back():
-    """A view whose data pointer is not 16-byte aligned takes the generic path, same bits."""
-    base = torch.from_numpy(rand((65, 64), np.uint8, 71)).to(DEV)
-    view = base.view(-1)[1:1 + 64 * 64].view(64, 64)
+@pytest.mark.parametrize("w", [64, 61, 150])
+@pytest.mark.parametrize("off_in,off_out", [(1, 0), (2, 3), (3, 1), (0, 2), (5, 7)])
+@pytest.mark.parametrize("r", [1, 3, 9, 20])
+def test_square_u8_unaligned_views(w, off_in, off_out, r):
+    """Input and output views at any byte offset (rows not 4- or 16-byte aligned), same bits;
+    the bytes of the output buffer around the view are left untouched."""
+    H = 67
+    base = torch.from_numpy(rand((H + 1, w), np.uint8, 71)).to(DEV)
+    view = base.view(-1)[off_in:off_in + H * w].view(H, w)
     img = view.cpu().numpy()
-    np.testing.assert_array_equal(h(pm.dilate(view, pm.static_nl_se.square2d, 3)),
-                                  oracle.static_iter(img, "SESquare2D", 3, True))
+    obuf = torch.full(((H + 1) * w,), 77, dtype=torch.uint8, device=DEV)
+    oview = obuf[off_out:off_out + H * w].view(H, w)
+    for dil in (True, False):
+        f = pm.dilate if dil else pm.erode
+        f(view, pm.static_nl_se.square2d, r, out=oview)
+        np.testing.assert_array_equal(h(oview), oracle.static_iter(img, "SESquare2D", r, dil))
+        rest = obuf.cpu().numpy()
+        assert (rest[:off_out] == 77).all() and (rest[off_out + H * w:] == 77).all()
