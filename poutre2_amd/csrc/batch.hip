@@ -39,13 +39,23 @@ template <bool DIL>
 __device__ __forceinline__ void batch_tile(uint8_t *smem, const uint8_t *src, uint8_t *dst, int H, int W, int r, int y0,
                                            int x0)
 {
-  const int R16 = (r + 15) & ~15;
-  sq_load_region<DIL>(smem, src, H, W, r, R16, y0, x0);
-  __syncthreads();
-  if (r >= 16) sq_tile_passes<32, DIL, false>(smem, dst, H, W, r, R16, y0, x0);
-  else if (r >= 8) sq_tile_passes<16, DIL, false>(smem, dst, H, W, r, R16, y0, x0);
-  else if (r >= 4) sq_tile_passes<8, DIL, false>(smem, dst, H, W, r, R16, y0, x0);
-  else sq_tile_passes<4, DIL, false>(smem, dst, H, W, r, R16, y0, x0);
+  if (r >= 16) {
+    sq_load_region<32, DIL>(smem, src, H, W, r, y0, x0);
+    __syncthreads();
+    sq_tile_passes<32, DIL, false>(smem, dst, H, W, r, y0, x0);
+  } else if (r >= 8) {
+    sq_load_region<16, DIL>(smem, src, H, W, r, y0, x0);
+    __syncthreads();
+    sq_tile_passes<16, DIL, false>(smem, dst, H, W, r, y0, x0);
+  } else if (r >= 4) {
+    sq_load_region<8, DIL>(smem, src, H, W, r, y0, x0);
+    __syncthreads();
+    sq_tile_passes<8, DIL, false>(smem, dst, H, W, r, y0, x0);
+  } else {
+    sq_load_region<4, DIL>(smem, src, H, W, r, y0, x0);
+    __syncthreads();
+    sq_tile_passes<4, DIL, false>(smem, dst, H, W, r, y0, x0);
+  }
   __syncthreads();
 }
 
@@ -139,8 +149,7 @@ extern "C" int pm_batch_dilate_open(int n, const pm_image *ins, const int *radii
   PM_HIP(hipMemcpyAsync(dimg, host.data(), (size_t)n * sizeof(BatchImage), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, bar);
   PM_HIP(hipGetLastError());
-  const int R16 = (int)((maxr + 15) & ~15);
-  const size_t lds = sq_tile_lds_bytes((int)maxr, R16);
+  const size_t lds = sq_tile_lds_bytes((int)maxr);
   int dev = 0, cus = 0, occ = 0;
   PM_HIP(hipGetDevice(&dev));
   PM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));

@@ -4,13 +4,15 @@
 // uint8 (ero_dil_static.cpp:229-305, ero_dil_line_se_t.hpp:49-280): one launch reads
 // every input pixel once from HBM and writes every output once.
 //
-// One workgroup (256 threads) owns a 128x128 output tile.
-//  1. The input region (128+2r rows, 128+2*R16 columns, R16 = r rounded up to 16 so that
-//     global loads of aligned rows are 16-byte aligned) is copied to LDS with 128-bit
+// One workgroup (256 threads) owns an output tile of 128 rows and TW = 256 (r <= 16) or
+// 128 columns.
+//  1. The input region (128+2r rows, TW+2*R16 columns, R16 = 16 for r <= 15 and 32 above,
+//     so that global loads of aligned rows are 16-byte aligned) is copied to LDS with 128-bit
 //     loads (dword-aligned loads realigned in registers when rows are not); pixels
 //     outside the image get the neutral value (0 for dilation, 255 for erosion).
 //  2. Vertical pass: a thread owns a 4-column quad and a run of K output rows.  It reads
-//     K+2r words down the column, unpacks every word into two u16x2 (columns 0/2 and 1/3)
+//     K+2r words down the column, splits every word into two u16x2 whose high bytes hold
+//     columns 0/2 and 1/3 (one shift)
 //     and computes the windowed max with a single-split van Herk: suffix maxima over the
 //     first K rows, running prefix over the rest, one v_pk_max_u16 per two pixels.
 //     Results are written TRANSPOSED into a second LDS buffer (region column -> row).
@@ -26,6 +28,8 @@
 // tiles) and prefetches the next tile's region into registers while computing the
 // current one.  Tiles are mapped XCD-aware: each XCD walks a contiguous tile range, so
 // neighbouring tiles share their halo rows in that XCD's L2.
+#include <utility>
+
 #include "pm_common.hpp"
 #include "sq_u8.hpp"
 #include "sq_u8_tile.hpp"
@@ -46,20 +50,28 @@ __device__ __forceinline__ TileSched tile_sched(int b, int G, int ntiles)
   return {lo + j, gx, hi};
 }
 
-template <int K, bool DIL, bool ALIGNED>
+// Tile width (output columns): 256 up to r = 16, 128 above (RC = 0: run-time radius of
+// run length K).  Wider tiles make longer contiguous row segments for HBM; above r = 16
+// the registers and LDS of the wider region cost more than they save (measured on
+// 8192x8192: r = 1 37.9 -> 34.9 us, r = 15 42.5 -> 41.4 us, r = 31 55.5 -> 67.4 us).
+constexpr int sq_tw(int K, int RC) { return (RC > 0 ? RC <= 16 : K < 32) ? 256 : 128; }
+
+template <int K, bool DIL, bool ALIGNED, int RC>
 __global__ __launch_bounds__(FTHREADS) void sq_u8_kernel(const uint8_t *__restrict__ in, uint8_t *__restrict__ out,
-                                                         int H, int W, int r, int R16, int tiles_x, int tiles_per_img,
+                                                         int H, int W, int r_rt, int tiles_x, int tiles_per_img,
                                                          int ntiles)
 {
+  constexpr int TW = sq_tw(K, RC);
   // region chunks (16 bytes) per thread: enough for the largest radius of each K
-  constexpr int MAXC = (K == 32) ? 9 : (K == 16) ? 7 : 6;
+  constexpr int MAXC = sq_maxc(K, TW);
+  constexpr int R16 = sq_r16<K>();
+  constexpr int AW = TW + 2 * R16;  // region width in bytes (multiple of 16)
+  constexpr int cpr = AW >> 4;
   extern __shared__ __align__(16) uint8_t smem[];
-  const int AW = FT + 2 * R16;  // region width in bytes (multiple of 16)
-  const int AH = FT + 2 * r;    // region height
+  const int r = RC > 0 ? RC : r_rt;
+  const int AH = FT + 2 * r;  // region height
   uint8_t *A = smem;
-  uint8_t *T = smem;  // the transposed buffer reuses A's storage once A has been consumed
   const int tid = threadIdx.x;
-  const int cpr = AW >> 4;
   const int nchunk = AH * cpr;
   const uint32_t nv = DIL ? 0u : 0xFFFFFFFFu;
 
@@ -77,7 +89,7 @@ __global__ __launch_bounds__(FTHREADS) void sq_u8_kernel(const uint8_t *__restri
       pre[i] = make_uint4(nv, nv, nv, nv);
       if (c < nchunk) {
         const int ri = c / cpr, cc = c - ri * cpr;
-        const int gy = ty * FT - r + ri, gx = tx * FT - R16 + (cc << 4);
+        const int gy = ty * FT - r + ri, gx = tx * TW - R16 + (cc << 4);
         if (ALIGNED) {
           // W is a multiple of 16, so a 16-byte chunk is all inside or all outside
           if ((unsigned)gy < (unsigned)H && (unsigned)gx < (unsigned)W)
@@ -95,7 +107,7 @@ __global__ __launch_bounds__(FTHREADS) void sq_u8_kernel(const uint8_t *__restri
     const int img = tile / tiles_per_img;
     const int t_in = tile - img * tiles_per_img;
     const int ty = t_in / tiles_x, tx = t_in - ty * tiles_x;
-    const int y0 = ty * FT, x0 = tx * FT;
+    const int y0 = ty * FT, x0 = tx * TW;
     const int64_t ibase = (int64_t)img * H * W;
 
     // ---- 1. prefetched region -> LDS, then start fetching the next tile
@@ -110,47 +122,49 @@ __global__ __launch_bounds__(FTHREADS) void sq_u8_kernel(const uint8_t *__restri
     __syncthreads();
     if (tile + ts.step < ts.end) fetch(tile + ts.step);
 
-    sq_tile_passes<K, DIL, ALIGNED>(smem, out + ibase, H, W, r, R16, y0, x0);
+    sq_tile_passes<K, DIL, ALIGNED, RC, TW>(smem, out + ibase, H, W, r, y0, x0);
     __syncthreads();  // A/T are rewritten by the next tile
+  }
+}
+
+typedef void (*sq_kernel_t)(const uint8_t *, uint8_t *, int, int, int, int, int, int);
+
+// Kernel for radius r: aligned rows get the variant specialised on r (1..32), others the
+// run-length variant with a run-time radius.
+template <bool DIL, int... RS>
+static sq_kernel_t sq_pick(int r, bool aligned, std::integer_sequence<int, RS...>)
+{
+  static constexpr sq_kernel_t spec[] = {sq_u8_kernel<sq_k_of(RS + 1), DIL, true, RS + 1>...};
+  if (aligned) return spec[r - 1];
+  switch (sq_k_of(r)) {
+  case 32: return sq_u8_kernel<32, DIL, false, 0>;
+  case 16: return sq_u8_kernel<16, DIL, false, 0>;
+  case 8: return sq_u8_kernel<8, DIL, false, 0>;
+  default: return sq_u8_kernel<4, DIL, false, 0>;
   }
 }
 
 template <bool DIL>
 static int sq_u8_once(const uint8_t *in, uint8_t *out, int64_t B, int64_t H, int64_t W, int r, hipStream_t s)
 {
-  const int R16 = (r + 15) & ~15;
-  const int tiles_x = (int)((W + FT - 1) / FT), tiles_y = (int)((H + FT - 1) / FT);
+  const bool aligned = (W % 16) == 0 && ((uintptr_t)in % 16) == 0 && ((uintptr_t)out % 16) == 0;
+  const int TW = sq_tw(sq_k_of(r), aligned ? r : 0);
+  const int tiles_x = (int)((W + TW - 1) / TW), tiles_y = (int)((H + FT - 1) / FT);
   const int64_t per = (int64_t)tiles_x * tiles_y;
   const int64_t ntiles = per * B;
   if (ntiles > INT32_MAX) PM_FAIL(PM_E_ARG, "image too large for the fused square kernel");
-  const size_t lds_a = (size_t)(FT + 2 * r) * (FT + 2 * R16), lds_t = (size_t)(FT + 2 * R16) * FT;
-  const size_t lds = lds_a > lds_t ? lds_a : lds_t;
+  const size_t lds = sq_tile_lds_bytes(r, TW);
   int dev = 0, cus = 0;
   PM_HIP(hipGetDevice(&dev));
   PM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  const dim3 block(FTHREADS);
-  const bool aligned = (W % 16) == 0 && ((uintptr_t)in % 16) == 0 && ((uintptr_t)out % 16) == 0;
-#define PM_SQ_LAUNCH(KK, AL)                                                                                      \
-  do {                                                                                                            \
-    int occ = 0;                                                                                                  \
-    PM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, sq_u8_kernel<KK, DIL, AL>, FTHREADS, lds));         \
-    int64_t g = (int64_t)(occ > 0 ? occ : 1) * cus;                                                               \
-    if (g >= 8) g &= ~(int64_t)7;                                                                                 \
-    if (g > ntiles) g = ntiles;                                                                                   \
-    hipLaunchKernelGGL((sq_u8_kernel<KK, DIL, AL>), dim3((unsigned)g), block, lds, s, in, out, (int)H, (int)W, r, \
-                       R16, tiles_x, (int)per, (int)ntiles);                                                      \
-  } while (0)
-#define PM_SQ_LAUNCH_K(AL)            \
-  do {                                \
-    if (r >= 16) PM_SQ_LAUNCH(32, AL); \
-    else if (r >= 8) PM_SQ_LAUNCH(16, AL); \
-    else if (r >= 4) PM_SQ_LAUNCH(8, AL); \
-    else PM_SQ_LAUNCH(4, AL);          \
-  } while (0)
-  if (aligned) PM_SQ_LAUNCH_K(true);
-  else PM_SQ_LAUNCH_K(false);
-#undef PM_SQ_LAUNCH_K
-#undef PM_SQ_LAUNCH
+  const sq_kernel_t k = sq_pick<DIL>(r, aligned, std::make_integer_sequence<int, (int)SQ_U8_RMAX>{});
+  int occ = 0;
+  PM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, FTHREADS, lds));
+  int64_t g = (int64_t)(occ > 0 ? occ : 1) * cus;
+  if (g >= 8) g &= ~(int64_t)7;
+  if (g > ntiles) g = ntiles;
+  hipLaunchKernelGGL(k, dim3((unsigned)g), dim3(FTHREADS), lds, s, in, out, (int)H, (int)W, r, tiles_x, (int)per,
+                     (int)ntiles);
   PM_HIP(hipGetLastError());
   return PM_OK;
 }

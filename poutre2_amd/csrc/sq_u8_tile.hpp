@@ -1,4 +1,4 @@
-// sq_u8_tile.hpp -- device code of one 128x128 output tile of the fused uint8 square
+// sq_u8_tile.hpp -- device code of one 128-row output tile of the fused uint8 square
 // erosion / dilation (used by the single-image kernel of sq_u8.hip and by the batched
 // kernel of batch.hip).  See sq_u8.hip for the algorithm.
 #pragma once
@@ -8,7 +8,7 @@ namespace pm {
 
 typedef unsigned short u16x2_t __attribute__((ext_vector_type(2)));
 
-constexpr int FT = 128;       // output tile side
+constexpr int FT = 128;       // output tile height (and default width)
 constexpr int FTHREADS = 256;
 // Transposed buffer T: one 128-byte row per region column, 32-bit word w of row R stored
 // at word (w ^ ((R >> 2) & 31)).  Pass 1 writes rows 4q..4q+3 from lanes of consecutive q
@@ -21,18 +21,22 @@ template <bool DIL> __device__ __forceinline__ uint32_t pk(uint32_t a, uint32_t 
   const u16x2_t x = __builtin_bit_cast(u16x2_t, a), y = __builtin_bit_cast(u16x2_t, b);
   return __builtin_bit_cast(uint32_t, DIL ? __builtin_elementwise_max(x, y) : __builtin_elementwise_min(x, y));
 }
-// word (c0,c1,c2,c3) -> (c0,c2) and (c1,c3) as u16x2
-__device__ __forceinline__ uint32_t even_pairs(uint32_t w) { return __builtin_amdgcn_perm(0u, w, 0x0c020c00u); }
-__device__ __forceinline__ uint32_t odd_pairs(uint32_t w) { return __builtin_amdgcn_perm(0u, w, 0x0c030c01u); }
+// The four bytes (c0, c1, c2, c3) of a word are processed as two u16x2 values whose HIGH
+// bytes carry the pixels: "odd" = the word itself (c1, c3 in the high bytes) and "even" =
+// the word shifted left by 8 (c0, c2 in the high bytes).  The max / min of two u16 values
+// has the max / min of their high bytes as its high byte whatever the low bytes hold, so
+// one v_pk_max_u16 / v_pk_min_u16 handles two pixels and unpacking costs one shift.
+__device__ __forceinline__ uint32_t even_hi(uint32_t w) { return w << 8; }
+__device__ __forceinline__ uint32_t odd_hi(uint32_t w) { return w; }
 
-// 4x4 byte transpose: e[j] = (c0_j, c2_j), o[j] = (c1_j, c3_j) for j = 0..3 ->
-// w[c] = (c_0, c_1, c_2, c_3) bytes
+// 4x4 byte transpose of high-byte pairs: e[j] = (c0_j, c2_j), o[j] = (c1_j, c3_j) in bytes
+// 1 and 3, j = 0..3 -> w[c] = (c_0, c_1, c_2, c_3) bytes
 __device__ __forceinline__ void transpose4(const uint32_t *e, const uint32_t *o, uint32_t *w)
 {
-  const uint32_t ea = __builtin_amdgcn_perm(e[1], e[0], 0x06020400u);
-  const uint32_t eb = __builtin_amdgcn_perm(e[3], e[2], 0x06020400u);
-  const uint32_t oa = __builtin_amdgcn_perm(o[1], o[0], 0x06020400u);
-  const uint32_t ob = __builtin_amdgcn_perm(o[3], o[2], 0x06020400u);
+  const uint32_t ea = __builtin_amdgcn_perm(e[1], e[0], 0x07030501u);
+  const uint32_t eb = __builtin_amdgcn_perm(e[3], e[2], 0x07030501u);
+  const uint32_t oa = __builtin_amdgcn_perm(o[1], o[0], 0x07030501u);
+  const uint32_t ob = __builtin_amdgcn_perm(o[3], o[2], 0x07030501u);
   w[0] = __builtin_amdgcn_perm(eb, ea, 0x05040100u);
   w[2] = __builtin_amdgcn_perm(eb, ea, 0x07060302u);
   w[1] = __builtin_amdgcn_perm(ob, oa, 0x05040100u);
@@ -117,18 +121,43 @@ template <int NW> __device__ __forceinline__ void store_any(uint8_t *dst, const 
   }
 }
 
+// Radius r and run length K of the fused square kernel: K is the largest power of two
+// <= min(2r, 32) (4 for r = 1), and the region is padded by R16 = 16 columns on each side
+// for K <= 16 (r <= 15) and by 32 for K = 32 (16 <= r <= 32).
+constexpr int sq_k_of(int r) { return r >= 16 ? 32 : r >= 8 ? 16 : r >= 4 ? 8 : 4; }
+template <int K> constexpr int sq_r16() { return K == 32 ? 32 : 16; }
+// largest radius served by run length K
+constexpr int sq_rmax_of(int K) { return K == 4 ? 3 : K == 8 ? 7 : K == 16 ? 15 : 32; }
+// vertical-pass tasks (column quad x run of K rows) per thread for tiles TW wide: the
+// quads cover columns [R16 - r, R16 + TW + r) of the region
+constexpr int sq_maxt(int K, int TW)
+{
+  const int r = sq_rmax_of(K), R16 = K == 32 ? 32 : 16;
+  const int nq = ((R16 + TW + r + 3) >> 2) - ((R16 - r) >> 2);  // largest at the largest radius
+  return (nq * (FT / K) + FTHREADS - 1) / FTHREADS;
+}
+// 16-byte region chunks per thread (region of FT + 2r rows, TW + 2*R16 columns)
+constexpr int sq_maxc(int K, int TW)
+{
+  const int r = sq_rmax_of(K), R16 = K == 32 ? 32 : 16;
+  return ((FT + 2 * r) * ((TW + 2 * R16) >> 4) + FTHREADS - 1) / FTHREADS;
+}
+
 // Windowed max/min of 2r+1 rows for K consecutive output rows of one 4-byte column quad.
 // row(t) returns the quad's 32-bit word in input row t (output row j covers rows j..j+2r).
-template <int K, bool DIL, typename LD>
-__device__ __forceinline__ void vh_run(LD &&row, int r, uint32_t (&e)[K], uint32_t (&o)[K])
+// RC > 0: the radius is the compile-time constant RC (r is ignored), so every row offset
+// is an immediate and the prefix loop has its exact length.
+template <int K, bool DIL, int RC, typename LD>
+__device__ __forceinline__ void vh_run(LD &&row, int r_rt, uint32_t (&e)[K], uint32_t (&o)[K])
 {
-  const uint32_t NEU = DIL ? 0u : 0x00FF00FFu;
+  const uint32_t NEU = DIL ? 0u : 0xFFFFFFFFu;
+  const int r = RC > 0 ? RC : r_rt;
   if (K > 4 || 2 * r >= K) {
 #pragma unroll
     for (int j = 0; j < K; j++) {
       const uint32_t w = row(j);
-      e[j] = even_pairs(w);
-      o[j] = odd_pairs(w);
+      e[j] = even_hi(w);
+      o[j] = odd_hi(w);
     }
 #pragma unroll
     for (int j = K - 2; j >= 0; j--) {
@@ -140,18 +169,27 @@ __device__ __forceinline__ void vh_run(LD &&row, int r, uint32_t (&e)[K], uint32
     // back instead of one per dependent iteration; rows past 2r-1 are read (they lie inside
     // the region since K <= 2r and K <= R16 + r) and ignored.
     uint32_t pe = NEU, po = NEU;
+    if constexpr (RC > 0) {
 #pragma unroll
-    for (int t = 0; t < K; t++) {
-      const uint32_t w = row(K + t);
-      const bool use = K + t < 2 * r;
-      pe = use ? pk<DIL>(pe, even_pairs(w)) : pe;
-      po = use ? pk<DIL>(po, odd_pairs(w)) : po;
+      for (int t = 0; t < 2 * RC - K; t++) {
+        const uint32_t w = row(K + t);
+        pe = pk<DIL>(pe, even_hi(w));
+        po = pk<DIL>(po, odd_hi(w));
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < K; t++) {
+        const uint32_t w = row(K + t);
+        const bool use = K + t < 2 * r;
+        pe = use ? pk<DIL>(pe, even_hi(w)) : pe;
+        po = use ? pk<DIL>(po, odd_hi(w)) : po;
+      }
     }
 #pragma unroll
     for (int j = 0; j < K; j++) {
       const uint32_t w = row(2 * r + j);
-      pe = pk<DIL>(pe, even_pairs(w));
-      po = pk<DIL>(po, odd_pairs(w));
+      pe = pk<DIL>(pe, even_hi(w));
+      po = pk<DIL>(po, odd_hi(w));
       e[j] = pk<DIL>(e[j], pe);
       o[j] = pk<DIL>(o[j], po);
     }
@@ -161,8 +199,8 @@ __device__ __forceinline__ void vh_run(LD &&row, int r, uint32_t (&e)[K], uint32
 #pragma unroll
     for (int t = 0; t < 6; t++) {
       const uint32_t w = row(t);
-      xe[t] = even_pairs(w);
-      xo[t] = odd_pairs(w);
+      xe[t] = even_hi(w);
+      xo[t] = odd_hi(w);
     }
 #pragma unroll
     for (int j = 0; j < K; j++) {
@@ -172,14 +210,17 @@ __device__ __forceinline__ void vh_run(LD &&row, int r, uint32_t (&e)[K], uint32
   }
 }
 
-// Passes 2 and 3 of one tile whose input region (rows y0-r .. y0+FT+r-1, columns
-// x0-R16 .. x0+FT+R16-1 of the image) is in smem (row pitch AW = FT + 2*R16).  dst is the
-// output image (row pitch W).  ALIGNED: W % 16 == 0 and dst 16-byte aligned.
-template <int K, bool DIL, bool ALIGNED>
-__device__ __forceinline__ void sq_tile_passes(uint8_t *smem, uint8_t *__restrict__ dst_img, int H, int W, int r,
-                                               int R16, int y0, int x0)
+// Passes 2 and 3 of one tile of FT rows and TW columns whose input region (rows
+// y0-r .. y0+FT+r-1, columns x0-R16 .. x0+TW+R16-1 of the image) is in smem (row pitch
+// AW = TW + 2*R16).  dst is the output image (row pitch W).  ALIGNED: W % 16 == 0 and dst
+// 16-byte aligned.  RC > 0: the radius is the compile-time constant RC (r_rt is ignored).
+template <int K, bool DIL, bool ALIGNED, int RC = 0, int TW = FT>
+__device__ __forceinline__ void sq_tile_passes(uint8_t *smem, uint8_t *__restrict__ dst_img, int H, int W, int r_rt,
+                                               int y0, int x0)
 {
-  const int AW = FT + 2 * R16;
+  constexpr int R16 = sq_r16<K>();
+  constexpr int AW = TW + 2 * R16;
+  const int r = RC > 0 ? RC : r_rt;
   uint8_t *A = smem;
   uint8_t *T = smem;
   const int tid = threadIdx.x;
@@ -188,8 +229,8 @@ __device__ __forceinline__ void sq_tile_passes(uint8_t *smem, uint8_t *__restric
   // storage of A.  Aliasing the two buffers halves the LDS footprint and doubles the
   // number of resident workgroups.
   {
-    constexpr int MAXT = (K == 4) ? 5 : (K == 8) ? 3 : (K == 16) ? 2 : 1;
-    const int q0 = (R16 - r) >> 2, q1 = (R16 + FT + r + 3) >> 2;
+    constexpr int MAXT = sq_maxt(K, TW);
+    const int q0 = (R16 - r) >> 2, q1 = (R16 + TW + r + 3) >> 2;
     const int nq = q1 - q0;
     const int ntask = nq * (FT / K);
     uint32_t res[MAXT][K];
@@ -200,7 +241,7 @@ __device__ __forceinline__ void sq_tile_passes(uint8_t *smem, uint8_t *__restric
         const int k = task / nq, q = q0 + (task - k * nq);
         uint32_t e[K], o[K];
         const uint8_t *base = A + (k * K) * AW + 4 * q;
-        vh_run<K, DIL>([&](int t) { return lds_word(base + t * AW); }, r, e, o);
+        vh_run<K, DIL, RC>([&](int t) { return lds_word(base + t * AW); }, r, e, o);
 #pragma unroll
         for (int g = 0; g < K / 4; g++) transpose4(e + 4 * g, o + 4 * g, &res[sl][4 * g]);
       }
@@ -223,13 +264,26 @@ __device__ __forceinline__ void sq_tile_passes(uint8_t *smem, uint8_t *__restric
 
   // ---- 3. horizontal pass over T, transposed back in registers, stored to HBM
   {
-    constexpr int NR = FT / K;
-    const int ntask = (FT / 4) * NR;
+    constexpr int NR = TW / K;
+    constexpr int ntask = (FT / 4) * NR;
+    const int c0 = R16 - r;  // T row of output column 0
+    // tile wholly inside the image: no bounds checks on the stores
+    const bool full = (y0 + FT <= H) && (x0 + TW <= W);
     for (int task = tid; task < ntask; task += FTHREADS) {
       const int q2 = task / NR, k2 = task - q2 * NR;
       uint32_t e[K], o[K];
-      const int rbase = k2 * K + R16 - r;
-      vh_run<K, DIL>([&](int t) { return lds_word(T + t_off(rbase + t, q2)); }, r, e, o);
+      // T row R0 + t, word q2: the swizzle key (R >> 2) & 31 is (m + u) & 31 with
+      // u = ((R0 & 3) + t) >> 2 known at compile time when r is, so the rows of one key
+      // group share a base address and differ by immediate offsets.
+      const int R0 = k2 * K + c0;
+      const int m = k2 * (K / 4) + (c0 >> 2), s0 = c0 & 3;
+      const uint8_t *Tb = T + R0 * FT;
+      vh_run<K, DIL, RC>(
+          [&](int t) {
+            const int u = (s0 + t) >> 2;
+            return lds_word((Tb + ((q2 ^ ((m + u) & 31)) << 2)) + t * FT);
+          },
+          r, e, o);
       const int gx = x0 + k2 * K;
       uint32_t rows[4][K / 4];
 #pragma unroll
@@ -239,12 +293,29 @@ __device__ __forceinline__ void sq_tile_passes(uint8_t *smem, uint8_t *__restric
 #pragma unroll
         for (int c = 0; c < 4; c++) rows[c][g] = w[c];
       }
-      // W is a multiple of 16: a 16-byte (or 4-byte) chunk starting inside the row is whole
+      uint8_t *d0 = dst_img + (int64_t)(y0 + 4 * q2) * W + gx;
+      if (ALIGNED && full) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          uint8_t *dst = d0 + (int64_t)c * W;
+          if (K >= 16) {
+#pragma unroll
+            for (int hh = 0; hh < K / 16; hh++)
+              *reinterpret_cast<uint4 *>(dst + 16 * hh) =
+                  make_uint4(rows[c][4 * hh], rows[c][4 * hh + 1], rows[c][4 * hh + 2], rows[c][4 * hh + 3]);
+          } else {
+#pragma unroll
+            for (int g = 0; g < K / 4; g++) reinterpret_cast<uint32_t *>(dst)[g] = rows[c][g];
+          }
+        }
+        continue;
+      }
+      // W is a multiple of 16 when ALIGNED: a 16-byte (or 4-byte) chunk starting inside the row is whole
 #pragma unroll
       for (int c = 0; c < 4; c++) {
         const int gy = y0 + 4 * q2 + c;
         if (gy >= H) continue;
-        uint8_t *dst = dst_img + (int64_t)gy * W + gx;
+        uint8_t *dst = d0 + (int64_t)c * W;
         if (ALIGNED) {
           if (K >= 16) {
 #pragma unroll
@@ -273,11 +344,13 @@ __device__ __forceinline__ void sq_tile_passes(uint8_t *smem, uint8_t *__restric
 }
 
 // Copy the input region of a tile into smem (any width / alignment, load_chunk_any).
-template <bool DIL>
+template <int K, bool DIL, int TW = FT>
 __device__ __forceinline__ void sq_load_region(uint8_t *smem, const uint8_t *__restrict__ src, int H, int W, int r,
-                                               int R16, int y0, int x0)
+                                               int y0, int x0)
 {
-  const int AW = FT + 2 * R16, AH = FT + 2 * r, cpr = AW >> 4;
+  constexpr int R16 = sq_r16<K>();
+  constexpr int AW = TW + 2 * R16, cpr = AW >> 4;
+  const int AH = FT + 2 * r;
   constexpr int BATCH = 4;
   const int n = AH * cpr;
   for (int c0 = threadIdx.x; c0 < n; c0 += BATCH * FTHREADS) {
@@ -301,9 +374,12 @@ __device__ __forceinline__ void sq_load_region(uint8_t *smem, const uint8_t *__r
 }
 
 constexpr int SQ_TILE_RMAX = 32;
-inline size_t sq_tile_lds_bytes(int r, int R16)
+// LDS bytes of one workgroup for radius r and tile width TW (region A and transposed
+// buffer T alias)
+inline size_t sq_tile_lds_bytes(int r, int TW = FT)
 {
-  const size_t a = (size_t)(FT + 2 * r) * (FT + 2 * R16), t = (size_t)(FT + 2 * R16) * FT;
+  const int R16 = sq_k_of(r) == 32 ? 32 : 16;
+  const size_t a = (size_t)(FT + 2 * r) * (TW + 2 * R16), t = (size_t)(TW + 2 * R16) * FT;
   return a > t ? a : t;
 }
 
