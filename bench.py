@@ -139,6 +139,45 @@ def bench_recon_4k(pm, torch, dev, reps=3):
     return res
 
 
+def bench_3d(pm, torch, dev, reps=3):
+    """512^3 uint8 volume (iid uniform, seed 3): dilation and erosion by SECross3D and SESquare3D
+    iterated 1, 2, 3 times (2 B/voxel algorithmic), on two alternating volume pairs."""
+    from poutre2_amd import synth
+    n = 512 ** 3
+    base = torch.from_numpy(synth.uniform_u8((512, 512, 512), 3)).to(dev)
+    # two input/output pairs (512 MiB), alternated so that the 256 MiB MALL cannot hold them
+    pairs = [(base, torch.empty_like(base)), (base.roll(97, dims=2).contiguous(), torch.empty_like(base))]
+    stream = torch.cuda.current_stream(dev)
+    out = {}
+    state = {"i": 0}
+    for se_name, se in (("cross3d", pm.static_nl_se.cross3d), ("square3d", pm.static_nl_se.square3d)):
+        for it in (1, 2, 3):
+            for op_name, fn in (("dilate", pm.dilate), ("erode", pm.erode)):
+                def one(fn=fn, se=se, it=it):
+                    a, b = pairs[state["i"] % 2]
+                    state["i"] += 1
+                    fn(a, se, it, out=b)
+                one()
+                t = timed(one, 2 * reps, stream) / (2 * reps)
+                out[f"{op_name}_{se_name}_iter{it}"] = {"ms": round(t, 4), "gvox_s": round(n / (t * 1e-3) / 1e9, 2),
+                                                        "pct_hbm": round(100 * 2 * n / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 2)}
+    return out
+
+
+def bench_small_1k(pm, torch, dev, reps=50):
+    """The reference's CPU-runnable case: 1024^2 uint8, SESquare2D iter 1, dilation and erosion."""
+    from poutre2_amd import synth
+    a = torch.from_numpy(synth.uniform_u8((1024, 1024), 1)).to(dev)
+    b = torch.empty_like(a)
+    stream = torch.cuda.current_stream(dev)
+    out = {}
+    for name, fn in (("dilate", pm.dilate), ("erode", pm.erode)):
+        fn(a, pm.static_nl_se.square2d, 1, out=b)
+        t = timed(lambda: fn(a, pm.static_nl_se.square2d, 1, out=b), reps, stream) / reps
+        out[name] = {"us": round(t * 1e3, 2), "gpx_s": round(a.numel() / (t * 1e-3) / 1e9, 2)}
+    return out
+
+
 def bench_batched(pm, torch, dev, reps=3):
     """512 uint8 images, H, W ~ U{64..2048}, square side ~ U{3,5,..,31}: dilation + opening per
     image in one launch vs the per-image path; 256 h-maxima reconstructions (sizes
@@ -339,7 +378,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--sections", default="dilate,ops,recon,batch,autograd")
+    ap.add_argument("--sections", default="dilate,ops,recon,3d,small,batch,autograd")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc passes")
     ap.add_argument("--pmc-child", type=int, default=0, help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -368,6 +407,12 @@ def main():
     if "recon" in sections:
         log("bench: 4K reconstruction / leveling")
         extra["geodesy_4k"] = bench_recon_4k(pm, torch, dev)
+    if "3d" in sections:
+        log("bench: 512^3 cross3d / square3d")
+        extra["volume_512"] = bench_3d(pm, torch, dev)
+    if "small" in sections:
+        log("bench: 1024^2 square2d iter 1")
+        extra["small_1k"] = bench_small_1k(pm, torch, dev)
     if "batch" in sections:
         log("bench: batched 512 dilate+open, 256 reconstructions")
         extra["batched"] = bench_batched(pm, torch, dev)

@@ -135,7 +135,7 @@ template <int K> static int compose(const pm_image *in, int se, int iter, pm_ima
   const int64_t n = numel(*in);
   if (n == 0) return PM_OK;
   if (iter == 0) {
-    if (K == C_GRAD) PM_HIP(hipMemsetAsync(out->data, 0, (size_t)n * ptype_size(in->ptype), s));
+    if (K == C_GRAD) return fill_bytes(out->data, 0, (size_t)n * ptype_size(in->ptype), s);
     else PM_HIP(hipMemcpyAsync(out->data, in->data, (size_t)n * ptype_size(in->ptype), hipMemcpyDeviceToDevice, s));
     return PM_OK;
   }

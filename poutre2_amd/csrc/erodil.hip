@@ -11,9 +11,12 @@
 // Host drivers map the reference's Common_NL_SE iterations onto these:
 //  SESquare2D^k = (2k+1)^2 rectangle (x pass then y pass), segments^k = lines of radius k,
 //  SESquare3D^k = box (x, y, z passes), crosses^k = k neighbour-list passes.
+// uint8 fast paths: squares go to the fused kernel of sq_u8.hip; the z passes of boxes and
+// z segments and the 3D cross go to the z-streaming kernels of vol_u8.hip.
 #include "pm_common.hpp"
 #include "erodil.hpp"
 #include "sq_u8.hpp"
+#include "vol_u8.hpp"
 
 #include <algorithm>
 #include <type_traits>
@@ -242,6 +245,19 @@ int rect_pass(const T *in, T *out, int64_t B, int64_t H, int64_t W, int64_t ry, 
   return PM_OK;
 }
 
+// window of 2rz+1 planes along z on a D x H x W volume
+template <typename T, bool DIL>
+static int zline_pass(const T *in, T *out, Dims d, int64_t rz, hipStream_t s)
+{
+  if constexpr (std::is_same<T, uint8_t>::value) {
+    const int64_t k = rz < d.D ? rz : d.D;  // a window longer than the volume is the whole column
+    if (k >= 1 && k <= VOL_ZWIN_KMAX && vol_u8_applicable(in, out, d.D, d.H, d.W))
+      return zwin_u8<DIL>(in, out, d.D, d.H, d.W, (int)k, s);
+  }
+  // z pass = y pass on the (D, H*W) view
+  return line_pass<T, DIL>(in, out, 1, d.D, d.H * d.W, 1, rz, s);
+}
+
 // box (2rz+1) x (2ry+1) x (2rx+1) on a D x H x W volume
 template <typename T, bool DIL>
 int box_pass(const T *in, T *out, Dims d, int64_t rz, int64_t ry, int64_t rx, hipStream_t s)
@@ -253,9 +269,7 @@ int box_pass(const T *in, T *out, Dims d, int64_t rz, int64_t ry, int64_t rx, hi
   PM_TRY(tmp.alloc((size_t)n * sizeof(T), s));
   T *t = reinterpret_cast<T *>(tmp.ptr);
   PM_TRY((rect_pass<T, DIL>(in, t, d.D, d.H, d.W, ry, rx, s)));
-  // z pass = y pass on the (D, H*W) view
-  PM_TRY((line_pass<T, DIL>(t, out, 1, d.D, d.H * d.W, 1, rz, s)));
-  return PM_OK;
+  return zline_pass<T, DIL>(t, out, d, rz, s);
 }
 
 // ============================================================ SE tables
@@ -325,6 +339,24 @@ static int nl_iter(const T *in, T *out, Dims d, const NLArgs &a, int k, hipStrea
   return PM_OK;
 }
 
+// SECross3D^k on a uint8 volume: k passes of the z-streaming cross, ping-ponging so that
+// the last one writes out
+template <bool DIL> static int cross3_iter(const uint8_t *in, uint8_t *out, Dims d, int k, hipStream_t s)
+{
+  if (k == 1) return cross3_u8<DIL>(in, out, d.D, d.H, d.W, s);
+  Scratch tmp;
+  PM_TRY(tmp.alloc((size_t)(d.D * d.H * d.W), s));
+  uint8_t *bufs[2] = {out, (uint8_t *)tmp.ptr};
+  int dst = (k % 2 == 1) ? 0 : 1;
+  const uint8_t *src = in;
+  for (int i = 0; i < k; i++) {
+    PM_TRY(cross3_u8<DIL>(src, bufs[dst], d.D, d.H, d.W, s));
+    src = bufs[dst];
+    dst ^= 1;
+  }
+  return PM_OK;
+}
+
 // One Common_NL_SE iterated `iter` times (ero_dil_static.cpp:229-305); iter >= 1.
 template <typename T, bool DIL>
 int static_iter(const T *in, T *out, Dims d, int rank, int se, int iter, hipStream_t s)
@@ -337,10 +369,13 @@ int static_iter(const T *in, T *out, Dims d, int rank, int se, int iter, hipStre
   case PM_SE_SEGMENT_Y_2D: return line_pass<T, DIL>(in, out, 1, d.H, d.W, 1, k, s);
   case PM_SE_SEGMENT_X_3D: return line_pass<T, DIL>(in, out, 1, d.D * d.H, d.W, 0, k, s);
   case PM_SE_SEGMENT_Y_3D: return line_pass<T, DIL>(in, out, d.D, d.H, d.W, 1, k, s);
-  case PM_SE_SEGMENT_Z_3D: return line_pass<T, DIL>(in, out, 1, d.D, d.H * d.W, 1, k, s);
+  case PM_SE_SEGMENT_Z_3D: return zline_pass<T, DIL>(in, out, d, k, s);
   case PM_SE_SQUARE_3D: return box_pass<T, DIL>(in, out, d, k, k, k, s);
   case PM_SE_CROSS_2D:
   case PM_SE_CROSS_3D: {
+    if constexpr (std::is_same<T, uint8_t>::value) {
+      if (se == PM_SE_CROSS_3D && vol_u8_applicable(in, out, d.D, d.H, d.W)) return cross3_iter<DIL>(in, out, d, iter, s);
+    }
     NLArgs a;
     PM_TRY(se_nl(se, a));
     return nl_iter<T, DIL>(in, out, d, a, iter, s);

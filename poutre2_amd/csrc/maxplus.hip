@@ -242,7 +242,7 @@ extern "C" int pm_maxplus_bwd(const float *x, const float *gy, int64_t B, int64_
   const int64_t nb = mp_blocks(B, H, W, tiles_x, per);
   hipStream_t s = as_stream(stream);
   if (nb == 0) {
-    if (!flat) PM_HIP(hipMemsetAsync(gw, 0, sizeof(float) * K * K, s));
+    if (!flat) return fill_bytes(gw, 0, sizeof(float) * K * K, s);
     return PM_OK;
   }
   if (nb > INT32_MAX) PM_FAIL(PM_E_SIZE, "maxplus: batch too large");

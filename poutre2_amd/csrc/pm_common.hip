@@ -1,4 +1,6 @@
 // pm_common.hip -- error state, argument checks and scratch allocation shared by all entry points.
+#include <mutex>
+
 #include "pm_common.hpp"
 
 #include <cstdarg>
@@ -66,11 +68,50 @@ int check_binary(const pm_image *a, const pm_image *b, const pm_image *out, cons
   return PM_OK;
 }
 
+// Scratch memory comes from one memory pool per device, owned by the library, whose
+// release threshold is unlimited: memory freed by a call stays in the pool for the next
+// one instead of being returned to the driver at every synchronisation (which made every
+// call with a scratch image pay a fresh mapping).  The pool holds at most the peak scratch
+// of the calls made so far on that device, until the process ends.
+static hipMemPool_t scratch_pool(int dev, hipError_t &err)
+{
+  static std::mutex mu;
+  static hipMemPool_t pools[PM_MAX_DEVICES] = {};
+  err = hipSuccess;
+  if (dev < 0 || dev >= PM_MAX_DEVICES) {
+    err = hipErrorInvalidDevice;
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  if (!pools[dev]) {
+    hipMemPoolProps props = {};
+    props.allocType = hipMemAllocationTypePinned;
+    props.handleTypes = hipMemHandleTypeNone;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = dev;
+    hipMemPool_t p = nullptr;
+    err = hipMemPoolCreate(&p, &props);
+    if (err != hipSuccess) return nullptr;
+    uint64_t keep = UINT64_MAX;
+    err = hipMemPoolSetAttribute(p, hipMemPoolAttrReleaseThreshold, &keep);
+    if (err != hipSuccess) {
+      (void)hipMemPoolDestroy(p);
+      return nullptr;
+    }
+    pools[dev] = p;
+  }
+  return pools[dev];
+}
+
 int Scratch::alloc(size_t bytes, hipStream_t s)
 {
   stream = s;
   if (bytes == 0) bytes = 256;
-  hipError_t e = hipMallocAsync(&ptr, bytes, s);
+  int dev = 0;
+  PM_HIP(hipGetDevice(&dev));
+  hipError_t e = hipSuccess;
+  hipMemPool_t pool = scratch_pool(dev, e);
+  if (pool) e = hipMallocFromPoolAsync(&ptr, bytes, pool, s);
   if (e != hipSuccess) {
     ptr = nullptr;
     PM_FAIL(PM_E_HIP, "scratch allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
@@ -81,6 +122,34 @@ int Scratch::alloc(size_t bytes, hipStream_t s)
 Scratch::~Scratch()
 {
   if (ptr) (void)hipFreeAsync(ptr, stream);
+}
+
+// byte fill with 16-byte stores where a 16-byte block of the buffer is whole (a kernel, not
+// hipMemsetAsync, so that callers may capture it in a graph)
+__global__ void fill_u8_kernel(uint8_t *p, uint8_t v, uint64_t n)
+{
+  const uint64_t lead = (uint64_t)((16 - ((uintptr_t)p & 15)) & 15);  // bytes before the first aligned block
+  const uint64_t nblk = n > lead ? (n - lead) / 16 : 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint32_t w = v * 0x01010101u;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nblk; i += stride)
+    reinterpret_cast<uint4 *>(p + lead)[i] = make_uint4(w, w, w, w);
+  // the unaligned head and the tail
+  const uint64_t tail0 = lead + nblk * 16;
+  const uint64_t nedge = (lead < n ? lead : n) + (n - (tail0 < n ? tail0 : n));
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nedge; i += stride) {
+    const uint64_t h = lead < n ? lead : n;
+    p[i < h ? i : tail0 + (i - h)] = v;
+  }
+}
+
+int fill_bytes(void *p, uint8_t v, size_t n, hipStream_t s)
+{
+  if (n == 0) return PM_OK;
+  hipLaunchKernelGGL(fill_u8_kernel, dim3(grid1d((int64_t)(n / 16 + 1), 256, 4096)), dim3(256), 0, s, (uint8_t *)p, v,
+                     (uint64_t)n);
+  PM_HIP(hipGetLastError());
+  return PM_OK;
 }
 
 }  // namespace pm

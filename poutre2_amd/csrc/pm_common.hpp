@@ -134,6 +134,11 @@ struct Scratch {
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+constexpr int PM_MAX_DEVICES = 64;
+
+// Fill n bytes at device pointer p with v (a kernel; safe under graph capture).
+int fill_bytes(void *p, uint8_t v, size_t n, hipStream_t s);
+
 inline unsigned grid1d(int64_t n, int block, int64_t cap = 1 << 20)
 {
   int64_t g = (n + block - 1) / block;

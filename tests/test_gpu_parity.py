@@ -85,6 +85,17 @@ def test_static_3d(dtype, se, it):
         np.testing.assert_array_equal(h(f(g(img), PM_SE[se], it)), oracle.static_iter(img, se, it, dil))
 
 
+@pytest.mark.parametrize("se", ["SECross3D", "SESquare3D", "SESegmentZ3D"])
+@pytest.mark.parametrize("shape,it", [((11, 13, 48), 1), ((11, 13, 48), 2), ((7, 9, 32), 3), ((20, 5, 16), 8),
+                                      ((20, 5, 16), 9), ((3, 17, 64), 4), ((1, 6, 16), 2)])
+def test_static_3d_u8_volume_kernels(se, shape, it):
+    """uint8 volumes with W % 16 == 0 take the z-streaming kernels (z radius <= 8; 9 falls back)."""
+    img = rand(shape, np.uint8, 5 + it)
+    for dil in (True, False):
+        f = pm.dilate if dil else pm.erode
+        np.testing.assert_array_equal(h(f(g(img), PM_SE[se], it)), oracle.static_iter(img, se, it, dil))
+
+
 @pytest.mark.parametrize("n,it", [(1, 1), (101, 1), (101, 4), (5000, 40)])
 def test_static_1d(n, it):
     img = rand((n,), np.uint8, 3)
@@ -184,6 +195,19 @@ def test_open_close_gradient(se, it):
     np.testing.assert_array_equal(h(pm.opening(g(img), PM_SE[se], it)), oracle.static_iter(e, se, it, True))
     np.testing.assert_array_equal(h(pm.closing(g(img), PM_SE[se], it)), oracle.static_iter(d, se, it, False))
     np.testing.assert_array_equal(h(pm.gradient(g(img), PM_SE[se], it)), oracle.ssub_image(d, e))
+
+
+@pytest.mark.parametrize("off,n", [(0, 4096), (3, 1), (5, 15), (7, 100), (1, 70 * 61 + 3)])
+def test_gradient_iter0_fills_zero(off, n):
+    """iter = 0 gives an all-zero gradient: the fill kernel writes exactly the view's bytes
+    (unaligned head and tail included) and nothing around it."""
+    buf = torch.full((n + 32,), 9, dtype=torch.uint8, device=DEV)
+    src = torch.full((n,), 200, dtype=torch.uint8, device=DEV)
+    out = buf[off:off + n]
+    pm.gradient(src, pm.static_nl_se.segx1d, 0, out=out)
+    b = buf.cpu().numpy()
+    assert (b[off:off + n] == 0).all()
+    assert (b[:off] == 9).all() and (b[off + n:] == 9).all()
 
 
 # ----------------------------------------------------------------------------- geodesy
