@@ -202,11 +202,19 @@ def bench_batched(pm, torch, dev, reps=3):
             pm.opening(a, pm.static_nl_se.square2d, r, out=o)
     per_image()
     tp = timed(per_image, 1, stream)
-    res = {"dilate_open_512": {"mpx": round(npx / 1e6, 1), "ms": round(tb, 3),
-                               "gpx_s": round(npx / (tb * 1e-3) / 1e9, 2),
-                               "pct_hbm": round(100 * 3 * npx / (tb * 1e-3) / 1e9 / HBM_PEAK_GBS, 2),
-                               "per_image_ms": round(tp, 3), "speedup_vs_per_image": round(tp / tb, 2)}}
-    del imgs, dil, opn
+    # the same images packed back to back in one buffer (descriptors computed by numpy)
+    flat = torch.cat([a.reshape(-1) for a in imgs])
+    shapes = [tuple(a.shape) for a in imgs]
+    dflat, oflat = torch.empty_like(flat), torch.empty_like(flat)
+    fp = lambda: pm.batch_dilate_open_packed(flat, shapes, radii, dflat, oflat)  # noqa: E731
+    fp()
+    tpk = timed(fp, reps, stream) / reps
+    res = {"dilate_open_512": {"mpx": round(npx / 1e6, 1), "ms": round(tpk, 3),
+                               "gpx_s": round(npx / (tpk * 1e-3) / 1e9, 2),
+                               "pct_hbm": round(100 * 3 * npx / (tpk * 1e-3) / 1e9 / HBM_PEAK_GBS, 2),
+                               "list_api_ms": round(tb, 3), "api": "packed buffer (batch_dilate_open_packed)",
+                               "per_image_ms": round(tp, 3), "speedup_vs_per_image": round(tp / tpk, 2)}}
+    del imgs, dil, opn, flat, dflat, oflat
     g = synth.rng(6)
     masks, markers = [], []
     for i in range(256):
@@ -225,9 +233,17 @@ def bench_batched(pm, torch, dev, reps=3):
             pm.reconstruction(pm.RecType.dilate, mk, ms, pm.static_nl_se.square2d, out=o)
     per_rec()
     tpr = timed(per_rec, 1, stream)
-    res["reconstruct_256"] = {"mpx": round(rpx / 1e6, 1), "ms": round(tr, 3), "gpx_s": round(rpx / (tr * 1e-3) / 1e9, 2),
-                              "pct_hbm": round(100 * 3 * rpx / (tr * 1e-3) / 1e9 / HBM_PEAK_GBS, 2),
-                              "per_image_ms": round(tpr, 3), "speedup_vs_per_image": round(tpr / tr, 2)}
+    mkf, msf = torch.cat([m.reshape(-1) for m in markers]), torch.cat([m.reshape(-1) for m in masks])
+    rshapes = [tuple(m.shape) for m in masks]
+    of = torch.empty_like(msf)
+    frp = lambda: pm.batch_reconstruction_packed(pm.RecType.dilate, mkf, msf, rshapes, pm.static_nl_se.square2d, of)  # noqa: E731
+    frp()
+    trp = timed(frp, reps, stream) / reps
+    res["reconstruct_256"] = {"mpx": round(rpx / 1e6, 1), "ms": round(trp, 3),
+                              "gpx_s": round(rpx / (trp * 1e-3) / 1e9, 2),
+                              "pct_hbm": round(100 * 3 * rpx / (trp * 1e-3) / 1e9 / HBM_PEAK_GBS, 2),
+                              "list_api_ms": round(tr, 3), "api": "packed buffer (batch_reconstruction_packed)",
+                              "per_image_ms": round(tpr, 3), "speedup_vs_per_image": round(tpr / trp, 2)}
     return res
 
 

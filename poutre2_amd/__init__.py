@@ -18,6 +18,8 @@ Differences from pypoutre, all additive:
 import ctypes
 import enum
 
+import numpy as np
+
 from . import _capi
 from ._capi import PoutreError
 
@@ -60,15 +62,24 @@ def _torch():
     return torch
 
 
+_PTYPES = None
+# numpy record layout of pm_image (include/poutre_amd.h): void *data; int32 ptype, rank; int64 shape[3]
+_PM_IMAGE_NP = np.dtype([("data", "<u8"), ("ptype", "<i4"), ("rank", "<i4"), ("shape", "<i8", (3,))])
+
+
 def _ptype(t):
-    torch = _torch()
-    m = {torch.uint8: _capi.PTYPE_U8, torch.int32: _capi.PTYPE_I32, torch.float32: _capi.PTYPE_F32,
-         torch.int64: _capi.PTYPE_I64, torch.float64: _capi.PTYPE_D64}
-    if hasattr(torch, "uint16"):
-        m[torch.uint16] = _capi.PTYPE_U16
-    if t.dtype not in m:
+    global _PTYPES
+    if _PTYPES is None:
+        torch = _torch()
+        m = {torch.uint8: _capi.PTYPE_U8, torch.int32: _capi.PTYPE_I32, torch.float32: _capi.PTYPE_F32,
+             torch.int64: _capi.PTYPE_I64, torch.float64: _capi.PTYPE_D64}
+        if hasattr(torch, "uint16"):
+            m[torch.uint16] = _capi.PTYPE_U16
+        _PTYPES = m
+    p = _PTYPES.get(t.dtype)
+    if p is None:
         raise PoutreError(-2, f"Unsupported PType for dtype {t.dtype}")
-    return m[t.dtype]
+    return p
 
 
 def _desc(t):
@@ -85,6 +96,35 @@ def _desc(t):
     for i, s in enumerate(t.shape):
         d.shape[i] = s
     return d
+
+
+def _descs(ts):
+    """pm_image array for a list of device tensors, built as one numpy record array (the
+    per-image Python work of a 512-image batch would otherwise rival its kernel time)."""
+    n = len(ts)
+    rec = np.zeros(n, dtype=_PM_IMAGE_NP)
+    data, ptype, rank, shape = [], [], [], []
+    for t in ts:
+        if not t.is_cuda:
+            raise PoutreError(-6, "images must be device tensors (HBM-resident)")
+        if not t.is_contiguous():
+            raise PoutreError(-6, "images must be contiguous")
+        d = t.dim()
+        if d < 1 or d > 3:
+            raise PoutreError(-4, f"Unsupported number of dims {d}")
+        data.append(t.data_ptr())
+        ptype.append(_ptype(t))
+        rank.append(d)
+        shape.append(tuple(t.shape) + (0,) * (3 - d))
+    rec["data"] = data
+    rec["ptype"] = ptype
+    rec["rank"] = rank
+    rec["shape"] = shape
+    return rec
+
+
+def _rec_ptr(rec):
+    return ctypes.cast(rec.ctypes.data, ctypes.POINTER(_capi.PmImage))
 
 
 def _stream(t):
@@ -312,11 +352,62 @@ def batch_dilate_open(images, radii, dil_outs=None, open_outs=None):
     open_outs = [_out(im, None) for im in images] if open_outs is None else open_outs
     if n == 0:
         return dil_outs, open_outs
-    arr = _capi.PmImage * n
-    di, dd, do = arr(*[_desc(t) for t in images]), arr(*[_desc(t) for t in dil_outs]), arr(*[_desc(t) for t in open_outs])
-    rr = (ctypes.c_int * n)(*[int(r) for r in radii])
-    _call("pm_batch_dilate_open", n, di, rr, dd, do, _stream(images[0]))
+    di, dd, do = _descs(images), _descs(dil_outs), _descs(open_outs)
+    rr = np.asarray([int(r) for r in radii], dtype=np.int32)
+    _call("pm_batch_dilate_open", n, _rec_ptr(di), rr.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _rec_ptr(dd),
+          _rec_ptr(do), _stream(images[0]))
     return dil_outs, open_outs
+
+
+def _packed(flat, shapes, what):
+    """pm_image records of 2D images stored back to back (row-major) in one 1D device tensor."""
+    if not flat.is_cuda or flat.dim() != 1 or not flat.is_contiguous():
+        raise PoutreError(-6, f"{what}: packed images must be one contiguous 1D device tensor")
+    shp = np.asarray(shapes, dtype=np.int64).reshape(-1, 2)
+    if (shp < 0).any():
+        raise PoutreError(-6, f"{what}: negative image size")
+    sizes = shp[:, 0] * shp[:, 1]
+    offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    if int(sizes.sum()) > flat.numel():
+        raise PoutreError(-6, f"{what}: packed buffer holds {flat.numel()} elements, the images need {int(sizes.sum())}")
+    rec = np.zeros(len(shp), dtype=_PM_IMAGE_NP)
+    rec["data"] = np.uint64(flat.data_ptr()) + (offsets * flat.element_size()).astype(np.uint64)
+    rec["ptype"] = _ptype(flat)
+    rec["rank"] = 2
+    rec["shape"][:, :2] = shp
+    return rec
+
+
+def batch_dilate_open_packed(flat, shapes, radii, dil_flat=None, open_flat=None):
+    """batch_dilate_open for images stored back to back in one 1D uint8 device tensor `flat`
+    (image i: shapes[i] = (H_i, W_i), row-major, at offset sum_{j<i} H_j W_j).  The outputs
+    use the same packing.  The descriptors are computed with numpy in one pass, so the host
+    cost of a call does not grow with per-image Python work.  Returns (dil_flat, open_flat)."""
+    n = len(shapes)
+    if len(radii) != n:
+        raise PoutreError(-6, "batch: one radius per image")
+    dil_flat = _out(flat, dil_flat)
+    open_flat = _out(flat, open_flat)
+    if n == 0:
+        return dil_flat, open_flat
+    di, dd, do = _packed(flat, shapes, "batch"), _packed(dil_flat, shapes, "batch"), _packed(open_flat, shapes, "batch")
+    rr = np.ascontiguousarray(radii, dtype=np.int32)
+    _call("pm_batch_dilate_open", n, _rec_ptr(di), rr.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _rec_ptr(dd),
+          _rec_ptr(do), _stream(flat))
+    return dil_flat, open_flat
+
+
+def batch_reconstruction_packed(rec_type, marker_flat, mask_flat, shapes, se, out_flat=None):
+    """batch_reconstruction for images packed as in batch_dilate_open_packed."""
+    n = len(shapes)
+    out_flat = _out(mask_flat, out_flat)
+    if n == 0:
+        return out_flat
+    dm, dk, do = (_packed(marker_flat, shapes, "batch"), _packed(mask_flat, shapes, "batch"),
+                  _packed(out_flat, shapes, "batch"))
+    _call("pm_batch_reconstruct", n, int(rec_type), _rec_ptr(dm), _rec_ptr(dk), int(se), _rec_ptr(do),
+          _stream(mask_flat))
+    return out_flat
 
 
 def batch_reconstruction(rec_type, markers, masks, se, outs=None):
@@ -325,7 +416,7 @@ def batch_reconstruction(rec_type, markers, masks, se, outs=None):
     outs = [_out(m, None) for m in masks] if outs is None else outs
     if n == 0:
         return outs
-    arr = _capi.PmImage * n
-    dm, dk, do = arr(*[_desc(t) for t in markers]), arr(*[_desc(t) for t in masks]), arr(*[_desc(t) for t in outs])
-    _call("pm_batch_reconstruct", n, int(rec_type), dm, dk, int(se), do, _stream(masks[0]))
+    dm, dk, do = _descs(markers), _descs(masks), _descs(outs)
+    _call("pm_batch_reconstruct", n, int(rec_type), _rec_ptr(dm), _rec_ptr(dk), int(se), _rec_ptr(do),
+          _stream(masks[0]))
     return outs

@@ -49,59 +49,17 @@ __device__ __forceinline__ TileSched tile_sched(int b, int G, int ntiles)
   return {lo + j, gx, hi};
 }
 
-// Tile width (output columns): 256 up to r = 16, 128 above (RC = 0: run-time radius of
-// run length K).  Wider tiles make longer contiguous row segments for HBM; above r = 16
-// the registers and LDS of the wider region cost more than they save (measured on
-// 8192x8192: r = 1 37.9 -> 34.9 us, r = 15 42.5 -> 41.4 us, r = 31 55.5 -> 67.4 us).
-constexpr int sq_tw(int K, int RC) { return (RC > 0 ? RC <= 16 : K < 32) ? 256 : 128; }
-
 template <int K, bool DIL, bool ALIGNED, int RC>
 __global__ __launch_bounds__(FTHREADS) void sq_u8_kernel(const uint8_t *__restrict__ in, uint8_t *__restrict__ out,
                                                          int H, int W, int r_rt, int tiles_x, int tiles_per_img,
                                                          int ntiles)
 {
   constexpr int TW = sq_tw(K, RC);
-  // region chunks (16 bytes) per thread: enough for the largest radius of each K
-  constexpr int MAXC = sq_maxc(K, TW);
-  constexpr int R16 = sq_r16<K>();
-  constexpr int AW = TW + 2 * R16;  // region width in bytes (multiple of 16)
-  constexpr int cpr = AW >> 4;
   extern __shared__ __align__(16) uint8_t smem[];
-  const int r = RC > 0 ? RC : r_rt;
-  const int AH = FT + 2 * r;  // region height
-  uint8_t *A = smem;
-  const int tid = threadIdx.x;
-  const int nchunk = AH * cpr;
-  const uint32_t nv = DIL ? 0u : 0xFFFFFFFFu;
-
   // A tile's region is read with all its loads in flight, then copied to LDS.  The
   // workgroups resident on a CU overlap each other's loads and compute; prefetching the
   // next tile into registers instead was slower (r = 1: 34.9 vs 33.5 us, r = 7: 40.2 vs
   // 36.5 us, r = 31: 55.7 vs 52.1 us on 8192x8192), its registers costing occupancy.
-  uint4 pre[MAXC];
-  auto fetch = [&](int tile) {
-    const int img = tile / tiles_per_img;
-    const int t_in = tile - img * tiles_per_img;
-    const int ty = t_in / tiles_x, tx = t_in - ty * tiles_x;
-    const uint8_t *src = in + (int64_t)img * H * W;
-#pragma unroll
-    for (int i = 0; i < MAXC; i++) {
-      const int c = tid + i * FTHREADS;
-      pre[i] = make_uint4(nv, nv, nv, nv);
-      if (c < nchunk) {
-        const int ri = c / cpr, cc = c - ri * cpr;
-        const int gy = ty * FT - r + ri, gx = tx * TW - R16 + (cc << 4);
-        if (ALIGNED) {
-          // W is a multiple of 16, so a 16-byte chunk is all inside or all outside
-          if ((unsigned)gy < (unsigned)H && (unsigned)gx < (unsigned)W)
-            pre[i] = *reinterpret_cast<const uint4 *>(src + (int64_t)gy * W + gx);
-        } else {
-          pre[i] = load_chunk_any(src, H, W, gy, gx, DIL ? 0 : 255);
-        }
-      }
-    }
-  };
-
   const TileSched ts = tile_sched(blockIdx.x, gridDim.x, ntiles);
   for (int tile = ts.first; tile < ts.end; tile += ts.step) {
     const int img = tile / tiles_per_img;
@@ -109,46 +67,29 @@ __global__ __launch_bounds__(FTHREADS) void sq_u8_kernel(const uint8_t *__restri
     const int ty = t_in / tiles_x, tx = t_in - ty * tiles_x;
     const int y0 = ty * FT, x0 = tx * TW;
     const int64_t ibase = (int64_t)img * H * W;
-
-    // ---- 1. region -> registers -> LDS
-    fetch(tile);
-#pragma unroll
-    for (int i = 0; i < MAXC; i++) {
-      const int c = tid + i * FTHREADS;
-      if (c < nchunk) {
-        const int ri = c / cpr, cc = c - ri * cpr;
-        *reinterpret_cast<uint4 *>(A + ri * AW + (cc << 4)) = pre[i];
-      }
-    }
+    sq_tile_load<K, DIL, ALIGNED, RC, TW>(smem, in + ibase, H, W, r_rt, y0, x0);
     __syncthreads();
-
-    sq_tile_passes<K, DIL, ALIGNED, RC, TW>(smem, out + ibase, H, W, r, y0, x0);
+    sq_tile_passes<K, DIL, ALIGNED, RC, TW>(smem, out + ibase, H, W, r_rt, y0, x0);
     __syncthreads();  // A/T are rewritten by the next tile
   }
 }
 
 typedef void (*sq_kernel_t)(const uint8_t *, uint8_t *, int, int, int, int, int, int);
 
-// Kernel for radius r: aligned rows get the variant specialised on r (1..32), others the
-// run-length variant with a run-time radius.
+// Kernel for radius r (1..32), specialised on r, for 16-byte aligned rows or any rows.
 template <bool DIL, int... RS>
 static sq_kernel_t sq_pick(int r, bool aligned, std::integer_sequence<int, RS...>)
 {
   static constexpr sq_kernel_t spec[] = {sq_u8_kernel<sq_k_of(RS + 1), DIL, true, RS + 1>...};
-  if (aligned) return spec[r - 1];
-  switch (sq_k_of(r)) {
-  case 32: return sq_u8_kernel<32, DIL, false, 0>;
-  case 16: return sq_u8_kernel<16, DIL, false, 0>;
-  case 8: return sq_u8_kernel<8, DIL, false, 0>;
-  default: return sq_u8_kernel<4, DIL, false, 0>;
-  }
+  static constexpr sq_kernel_t spec_any[] = {sq_u8_kernel<sq_k_of(RS + 1), DIL, false, RS + 1>...};
+  return aligned ? spec[r - 1] : spec_any[r - 1];
 }
 
 template <bool DIL>
 static int sq_u8_once(const uint8_t *in, uint8_t *out, int64_t B, int64_t H, int64_t W, int r, hipStream_t s)
 {
   const bool aligned = (W % 16) == 0 && ((uintptr_t)in % 16) == 0 && ((uintptr_t)out % 16) == 0;
-  const int TW = sq_tw(sq_k_of(r), aligned ? r : 0);
+  const int TW = sq_tw(sq_k_of(r), r);
   const int tiles_x = (int)((W + TW - 1) / TW), tiles_y = (int)((H + FT - 1) / FT);
   const int64_t per = (int64_t)tiles_x * tiles_y;
   const int64_t ntiles = per * B;

@@ -63,16 +63,44 @@ __device__ __forceinline__ uint4 load16_any(const uint8_t *p)
                     __builtin_amdgcn_alignbyte(v.w, v.z, sh), __builtin_amdgcn_alignbyte(v4, v.w, sh));
 }
 
-// One 16-byte region chunk starting at image column gx of row gy (any width / alignment):
-// bytes outside the image get the neutral value nb.  Chunks wholly inside a row take
-// load16_any, chunks straddling the left or right edge are read bytewise.
-__device__ __forceinline__ uint4 load_chunk_any(const uint8_t *__restrict__ src, int H, int W, int gy, int gx,
-                                                uint8_t nb)
+// Shift the 16 bytes of (v0..v3) by b bytes towards higher (LEFT = true: out byte j =
+// in byte j - b) or lower (out byte j = in byte j + b) positions, 0 <= b <= 15, filling
+// with the byte value of fill.  b may differ between lanes.
+template <bool LEFT>
+__device__ __forceinline__ uint4 shift16(const uint4 &v, int b, uint32_t fill)
+{
+  const uint32_t in[4] = {v.x, v.y, v.z, v.w};
+  const int q = b >> 2, rem = b & 3;
+  // word m of the input shifted by q whole words (fill outside)
+  auto wsh = [&](int m) {
+    const int i = LEFT ? m - q : m + q;
+    uint32_t x = fill;
+#pragma unroll
+    for (int t = 0; t < 4; t++) x = (i == t) ? in[t] : x;
+    return x;
+  };
+  uint32_t o[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    if (LEFT) {
+      const uint32_t hi = wsh(k), lo = wsh(k - 1);  // out = high rem bytes of lo, then low 4-rem of hi
+      o[k] = rem ? __builtin_amdgcn_alignbyte(hi, lo, 4 - rem) : hi;
+    } else {
+      const uint32_t lo = wsh(k), hi = wsh(k + 1);  // out = bytes rem.. of lo, then low rem of hi
+      o[k] = rem ? __builtin_amdgcn_alignbyte(hi, lo, rem) : lo;
+    }
+  }
+  return make_uint4(o[0], o[1], o[2], o[3]);
+}
+
+// The 16-byte region chunk starting at image column gx < W < gx + 16 of row `row` (the
+// chunk that straddles the right edge; tile columns are multiples of 16 bytes apart from
+// column 0, so no chunk straddles the left edge): the row's last 16 bytes shifted into
+// place with neutral fill; rows narrower than 16 bytes are read bytewise.
+__device__ __forceinline__ uint4 load_edge_chunk(const uint8_t *__restrict__ row, int W, int gx, uint8_t nb)
 {
   const uint32_t nv = nb * 0x01010101u;
-  if ((unsigned)gy >= (unsigned)H || gx + 16 <= 0 || gx >= W) return make_uint4(nv, nv, nv, nv);
-  const uint8_t *p = src + (int64_t)gy * W + gx;
-  if (gx >= 0 && gx + 16 <= W) return load16_any(p);
+  if (W >= 16) return shift16<false>(load16_any(row + (W - 16)), gx + 16 - W, nv);
   uint32_t wd[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
@@ -80,7 +108,7 @@ __device__ __forceinline__ uint4 load_chunk_any(const uint8_t *__restrict__ src,
 #pragma unroll
     for (int bb = 0; bb < 4; bb++) {
       const int x = gx + 4 * k + bb;
-      const uint32_t byte = ((unsigned)x < (unsigned)W) ? p[4 * k + bb] : nb;
+      const uint32_t byte = ((unsigned)x < (unsigned)W) ? row[x] : nb;
       acc |= byte << (8 * bb);
     }
     wd[k] = acc;
@@ -88,44 +116,16 @@ __device__ __forceinline__ uint4 load_chunk_any(const uint8_t *__restrict__ src,
   return make_uint4(wd[0], wd[1], wd[2], wd[3]);
 }
 
-// Store 4*NW bytes w (little-endian across words) at any address dst.  When dst is not
-// 4-aligned, the bytes before the first dword boundary and after the last one are written
-// with byte / short stores and the NW-1 dwords in between with dword stores of realigned
-// words; sub-dword stores only touch their own bytes, so neighbouring rows and tiles that
-// share a dword do not race.
-template <int NW> __device__ __forceinline__ void store_any(uint8_t *dst, const uint32_t (&w)[NW])
-{
-  const uint32_t s = (uint32_t)((uintptr_t)dst & 3);
-  if (s == 0) {
-#pragma unroll
-    for (int m = 0; m < NW; m++) reinterpret_cast<uint32_t *>(dst)[m] = w[m];
-    return;
-  }
-  const uint32_t hs = 4 - s;  // head bytes
-  uint32_t *mid = reinterpret_cast<uint32_t *>(dst + hs);
-#pragma unroll
-  for (int m = 0; m < NW - 1; m++) mid[m] = __builtin_amdgcn_alignbyte(w[m + 1], w[m], hs);
-  uint8_t *tail = dst + hs + 4 * (NW - 1);
-  const uint32_t tw = w[NW - 1] >> (8 * hs);  // the last s bytes
-  if (s == 1) {
-    dst[0] = (uint8_t)w[0];
-    *reinterpret_cast<uint16_t *>(dst + 1) = (uint16_t)(w[0] >> 8);
-    tail[0] = (uint8_t)tw;
-  } else if (s == 2) {
-    *reinterpret_cast<uint16_t *>(dst) = (uint16_t)w[0];
-    *reinterpret_cast<uint16_t *>(tail) = (uint16_t)tw;
-  } else {
-    dst[0] = (uint8_t)w[0];
-    *reinterpret_cast<uint16_t *>(tail) = (uint16_t)tw;
-    tail[2] = (uint8_t)(tw >> 16);
-  }
-}
-
 // Radius r and run length K of the fused square kernel: K is the largest power of two
 // <= min(2r, 32) (4 for r = 1), and the region is padded by R16 = 16 columns on each side
 // for K <= 16 (r <= 15) and by 32 for K = 32 (16 <= r <= 32).
 constexpr int sq_k_of(int r) { return r >= 16 ? 32 : r >= 8 ? 16 : r >= 4 ? 8 : 4; }
 template <int K> constexpr int sq_r16() { return K == 32 ? 32 : 16; }
+// Tile width (output columns): 256 up to r = 16, 128 above (RC = 0: run-time radius of
+// run length K).  Wider tiles make longer contiguous row segments for HBM; above r = 16
+// the registers and LDS of the wider region cost more than they save (measured on
+// 8192x8192: r = 1 37.9 -> 34.9 us, r = 15 42.5 -> 41.4 us, r = 31 55.5 -> 67.4 us).
+constexpr int sq_tw(int K, int RC) { return (RC > 0 ? RC <= 16 : K < 32) ? 256 : 128; }
 // largest radius served by run length K
 constexpr int sq_rmax_of(int K) { return K == 4 ? 3 : K == 8 ? 7 : K == 16 ? 15 : 32; }
 // vertical-pass tasks (column quad x run of K rows) per thread for tiles TW wide: the
@@ -141,6 +141,61 @@ constexpr int sq_maxc(int K, int TW)
 {
   const int r = sq_rmax_of(K), R16 = K == 32 ? 32 : 16;
   return ((FT + 2 * r) * ((TW + 2 * R16) >> 4) + FTHREADS - 1) / FTHREADS;
+}
+
+// Store the 4*NW bytes w of one run at any address dst, as dwords at 4-aligned addresses.
+// With s = dst & 3 != 0, the s bytes before dst end the previous lane's run (whose last
+// word is prev, fetched by a DPP lane shift) and are written by this lane together with its
+// own first 4-s bytes, so that a lane issues one multi-dword store per run.  A run that
+// starts a row segment (first) writes its first 4-s bytes with sub-dword stores instead, and
+// a run that no following run overlaps (tail) also writes its own last s bytes.  Every byte
+// is written once; sub-dword stores touch only their own bytes.
+// s is the same for every lane of the wave (rows 4*q2 + c of one tile differ by multiples
+// of 4*W bytes and runs by multiples of 4), so it is read once into a scalar register and
+// its cases are uniform branches.
+template <int NW>
+__device__ __forceinline__ void store_run_shared(uint8_t *dst, const uint32_t (&w)[NW], uint32_t prev, bool first,
+                                                 bool tail)
+{
+  const uint32_t s = __builtin_amdgcn_readfirstlane((uint32_t)((uintptr_t)dst & 3));
+  if (s == 0) {
+#pragma unroll
+    for (int m = 0; m < NW; m++) reinterpret_cast<uint32_t *>(dst)[m] = w[m];
+    return;
+  }
+  const uint32_t hs = 4 - s;
+  uint32_t *a = reinterpret_cast<uint32_t *>(dst - s);  // 4-aligned
+  uint32_t o[NW];
+  o[0] = __builtin_amdgcn_alignbyte(w[0], prev, hs);
+#pragma unroll
+  for (int m = 1; m < NW; m++) o[m] = __builtin_amdgcn_alignbyte(w[m], w[m - 1], hs);
+  if (!first) {
+#pragma unroll
+    for (int m = 0; m < NW; m++) a[m] = o[m];
+  } else {
+    if (s == 1) {
+      dst[0] = (uint8_t)w[0];
+      *reinterpret_cast<uint16_t *>(dst + 1) = (uint16_t)(w[0] >> 8);
+    } else if (s == 2) {
+      *reinterpret_cast<uint16_t *>(dst) = (uint16_t)w[0];
+    } else {
+      dst[0] = (uint8_t)w[0];
+    }
+#pragma unroll
+    for (int m = 1; m < NW; m++) a[m] = o[m];
+  }
+  if (tail) {
+    uint8_t *t = reinterpret_cast<uint8_t *>(a + NW);
+    const uint32_t tw = w[NW - 1] >> (8 * hs);  // the last s bytes
+    if (s == 1) {
+      t[0] = (uint8_t)tw;
+    } else if (s == 2) {
+      *reinterpret_cast<uint16_t *>(t) = (uint16_t)tw;
+    } else {
+      *reinterpret_cast<uint16_t *>(t) = (uint16_t)tw;
+      t[2] = (uint8_t)(tw >> 16);
+    }
+  }
 }
 
 // Windowed max/min of 2r+1 rows for K consecutive output rows of one 4-byte column quad.
@@ -294,6 +349,14 @@ __device__ __forceinline__ void sq_tile_passes(uint8_t *smem, uint8_t *__restric
         for (int c = 0; c < 4; c++) rows[c][g] = w[c];
       }
       uint8_t *d0 = dst_img + (int64_t)(y0 + 4 * q2) * W + gx;
+      // rows of any alignment: the last word of the previous lane's run of each row (lanes
+      // k2 - 1 and k2 are neighbouring runs of one row segment); taken while every lane of
+      // the wave is active
+      uint32_t prevw[4] = {0u, 0u, 0u, 0u};
+      if (!ALIGNED) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) prevw[c] = __builtin_amdgcn_mov_dpp(rows[c][K / 4 - 1], 0x138, 0xf, 0xf, false);
+      }
       if (ALIGNED && full) {
 #pragma unroll
         for (int c = 0; c < 4; c++) {
@@ -329,7 +392,7 @@ __device__ __forceinline__ void sq_tile_passes(uint8_t *smem, uint8_t *__restric
               if (gx + 4 * g < W) *reinterpret_cast<uint32_t *>(dst + 4 * g) = rows[c][g];
           }
         } else if (gx + K <= W) {
-          store_any<K / 4>(dst, rows[c]);
+          store_run_shared<K / 4>(dst, rows[c], prevw[c], k2 == 0, k2 == NR - 1 || gx + 2 * K > W);
         } else {
           // the run crosses the right edge: byte stores, clipped
 #pragma unroll
@@ -343,35 +406,60 @@ __device__ __forceinline__ void sq_tile_passes(uint8_t *smem, uint8_t *__restric
   }
 }
 
-// Copy the input region of a tile into smem (any width / alignment, load_chunk_any).
-template <int K, bool DIL, int TW = FT>
-__device__ __forceinline__ void sq_load_region(uint8_t *smem, const uint8_t *__restrict__ src, int H, int W, int r,
-                                               int y0, int x0)
+// Copy the input region of the tile at (y0, x0) -- rows y0-r .. y0+FT+r-1, columns
+// x0-R16 .. x0+TW+R16-1 -- into smem (row pitch TW + 2*R16) with all of a thread's
+// 16-byte loads in flight before the LDS writes.  Pixels outside the image get the neutral
+// value.  ALIGNED: W % 16 == 0 and src 16-byte aligned (a chunk is wholly inside or
+// outside the image); otherwise load16_any, and the chunk that straddles the right edge
+// of the image comes from load_edge_chunk.
+template <int K, bool DIL, bool ALIGNED, int RC = 0, int TW = FT>
+__device__ __forceinline__ void sq_tile_load(uint8_t *smem, const uint8_t *__restrict__ src, int H, int W, int r_rt,
+                                             int y0, int x0)
 {
+  constexpr int MAXC = sq_maxc(K, TW);
   constexpr int R16 = sq_r16<K>();
   constexpr int AW = TW + 2 * R16, cpr = AW >> 4;
-  const int AH = FT + 2 * r;
-  constexpr int BATCH = 4;
-  const int n = AH * cpr;
-  for (int c0 = threadIdx.x; c0 < n; c0 += BATCH * FTHREADS) {
-    uint4 v[BATCH];
+  const int r = RC > 0 ? RC : r_rt;
+  const int nchunk = (FT + 2 * r) * cpr;
+  const uint32_t nv = DIL ? 0u : 0xFFFFFFFFu;
+  const int gx0 = x0 - R16;  // image column of region column 0 (a multiple of 16)
+  // any width: the chunk of each region row that straddles the right edge (column W) is
+  // skipped by the main loop and filled by the edge loop below
+  const int ce = ((W - gx0) & 15) ? (W - gx0) >> 4 : -1;
+  uint4 v[MAXC];
 #pragma unroll
-    for (int u = 0; u < BATCH; u++) {
-      const int c = c0 + u * FTHREADS;
+  for (int i = 0; i < MAXC; i++) {
+    const int c = threadIdx.x + i * FTHREADS;
+    v[i] = make_uint4(nv, nv, nv, nv);
+    if (c < nchunk) {
       const int ri = c / cpr, cc = c - ri * cpr;
-      // c >= n: a row below the region (gy >= H is not guaranteed), so clamp it out explicitly
-      v[u] = load_chunk_any(src, c < n ? H : 0, W, y0 - r + ri, x0 - R16 + (cc << 4), DIL ? 0 : 255);
-    }
-#pragma unroll
-    for (int u = 0; u < BATCH; u++) {
-      const int c = c0 + u * FTHREADS;
-      if (c < n) {
-        const int ri = c / cpr, cc = c - ri * cpr;
-        *reinterpret_cast<uint4 *>(smem + ri * AW + (cc << 4)) = v[u];
+      const int gy = y0 - r + ri, gx = gx0 + (cc << 4);
+      if (ALIGNED) {
+        if ((unsigned)gy < (unsigned)H && (unsigned)gx < (unsigned)W)
+          v[i] = *reinterpret_cast<const uint4 *>(src + (int64_t)gy * W + gx);
+      } else {
+        if ((unsigned)gy < (unsigned)H && gx >= 0 && gx + 16 <= W) v[i] = load16_any(src + (int64_t)gy * W + gx);
       }
     }
   }
+#pragma unroll
+  for (int i = 0; i < MAXC; i++) {
+    const int c = threadIdx.x + i * FTHREADS;
+    if (c < nchunk) {
+      const int ri = c / cpr, cc = c - ri * cpr;
+      if (ALIGNED || cc != ce) *reinterpret_cast<uint4 *>(smem + ri * AW + (cc << 4)) = v[i];
+    }
+  }
+  if (!ALIGNED && ce >= 0 && ce < cpr) {
+    for (int ri = threadIdx.x; ri < FT + 2 * r; ri += FTHREADS) {
+      const int gy = y0 - r + ri;
+      uint4 e = make_uint4(nv, nv, nv, nv);
+      if ((unsigned)gy < (unsigned)H) e = load_edge_chunk(src + (int64_t)gy * W, W, gx0 + (ce << 4), DIL ? 0 : 255);
+      *reinterpret_cast<uint4 *>(smem + ri * AW + (ce << 4)) = e;
+    }
+  }
 }
+
 
 constexpr int SQ_TILE_RMAX = 32;
 // LDS bytes of one workgroup for radius r and tile width TW (region A and transposed

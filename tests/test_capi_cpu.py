@@ -61,3 +61,13 @@ def test_version_and_error_without_gpu():
     b.data = 2
     assert L.pm_dilate_static(ctypes.byref(a), 2, -1, ctypes.byref(b), None) == -6
     assert L.pm_dilate_static(ctypes.byref(a), 10, 1, ctypes.byref(b), None) == -4
+
+
+def test_batch_descriptor_record_matches_pm_image():
+    """The numpy record array used for batched calls has pm_image's exact layout."""
+    import ctypes
+    import poutre2_amd as pm
+    from poutre2_amd import _capi
+    assert pm._PM_IMAGE_NP.itemsize == ctypes.sizeof(_capi.PmImage)
+    for f in ("data", "ptype", "rank", "shape"):
+        assert pm._PM_IMAGE_NP.fields[f][1] == getattr(_capi.PmImage, f).offset

@@ -69,3 +69,47 @@ def test_batch_reconstruction_matches_oracle(se, rec):
     torch.cuda.synchronize()
     for mk, ms, o in zip(markers, masks, outs):
         np.testing.assert_array_equal(o.cpu().numpy(), oracle.reconstruct(rec, mk, ms, se))
+
+
+@pytest.mark.parametrize("seed,n,hmax,rmax", [(11, 10, 300, 15), (12, 6, 40, 32)])
+def test_batch_packed_matches_list_api(seed, n, hmax, rmax):
+    """Images packed back to back in one buffer give the same bits as the list API."""
+    imgs, radii = _batch(n, seed, hmax, rmax)
+    flat = torch.from_numpy(np.concatenate([a.reshape(-1) for a in imgs])).to(DEV)
+    shapes = [a.shape for a in imgs]
+    d_flat, o_flat = pm.batch_dilate_open_packed(flat, shapes, radii)
+    dils, opens = pm.batch_dilate_open([torch.from_numpy(a).to(DEV) for a in imgs], radii)
+    torch.cuda.synchronize()
+    off = 0
+    for a, d, o in zip(imgs, dils, opens):
+        k = a.size
+        assert torch.equal(d_flat[off:off + k].view(a.shape), d)
+        assert torch.equal(o_flat[off:off + k].view(a.shape), o)
+        off += k
+
+
+def test_batch_reconstruction_packed_matches_oracle():
+    rng = np.random.default_rng(9)
+    markers, masks = [], []
+    for i in range(7):
+        H, W = int(rng.integers(5, 200)), int(rng.integers(5, 200))
+        mask = synth.blurred_noise_mask(H, W, 200 + i, sigma=3.0, radius=12)
+        markers.append(synth.hmax_marker(mask, 16))
+        masks.append(mask)
+    shapes = [m.shape for m in masks]
+    mk = torch.from_numpy(np.concatenate([m.reshape(-1) for m in markers])).to(DEV)
+    ms = torch.from_numpy(np.concatenate([m.reshape(-1) for m in masks])).to(DEV)
+    out = pm.batch_reconstruction_packed(pm.RecType.dilate, mk, ms, shapes, pm.static_nl_se.square2d)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    off = 0
+    for a, b in zip(markers, masks):
+        k = b.size
+        np.testing.assert_array_equal(got[off:off + k].reshape(b.shape), oracle.reconstruct("dilate", a, b, "SESquare2D"))
+        off += k
+
+
+def test_batch_packed_rejects_short_buffer():
+    flat = torch.zeros(50, dtype=torch.uint8, device=DEV)
+    with pytest.raises(pm.PoutreError):
+        pm.batch_dilate_open_packed(flat, [(10, 10)], [1])
