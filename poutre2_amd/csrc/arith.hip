@@ -2,8 +2,11 @@
 // arith_op_t.hpp:84-445): sup/inf of two images, saturated subtraction of images,
 // saturated addition/subtraction of a constant (clamped at numeric_limits<T>::min()/max(),
 // accumulated in a wider type), plus the opening / closing / gradient compositions.
+#include <type_traits>
+
 #include "arith.hpp"
 #include "erodil.hpp"
+#include "sq_u8.hpp"
 
 namespace pm {
 
@@ -145,6 +148,13 @@ template <int K> static int compose(const pm_image *in, int se, int iter, pm_ima
       using T = typename decltype(tag)::type;
       const T *src = (const T *)in->data;
       T *dst = (T *)out->data;
+      if constexpr (std::is_same<T, uint8_t>::value) {
+        // uint8 gradient by a square: one fused launch (no scratch image, no subtraction
+        // pass).  Above r = 15 its doubled registers leave one workgroup per CU and the
+        // composition is faster (8192^2, r = 31: 0.143 vs 0.133 ms; r = 7: 0.053 vs 0.105 ms).
+        if (K == C_GRAD && se == PM_SE_SQUARE_2D && rank == 2 && iter <= SQ_U8_GRAD_RMAX)
+          return sq_u8_gradient(src, dst, 1, d.H, d.W, iter, s);
+      }
       Scratch tmp;
       PM_TRY(tmp.alloc((size_t)n * sizeof(T), s));
       T *t = (T *)tmp.ptr;

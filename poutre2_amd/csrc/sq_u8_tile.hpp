@@ -265,6 +265,97 @@ __device__ __forceinline__ void vh_run(LD &&row, int r_rt, uint32_t (&e)[K], uin
   }
 }
 
+// Horizontal pass of one task (rows 4*q2 .. 4*q2+3 of the tile, output columns
+// k2*K .. k2*K+K-1) over the transposed buffer T, whose row c0 holds output column 0's
+// window start: rows[c][g] = output word g of tile row 4*q2 + c.
+template <int K, bool DIL, int RC>
+__device__ __forceinline__ void sq_pass3_rows(const uint8_t *T, int r, int c0, int q2, int k2, uint32_t (&rows)[4][K / 4])
+{
+  uint32_t e[K], o[K];
+  // T row R0 + t, word q2: the swizzle key (R >> 2) & 31 is (m + u) & 31 with
+  // u = ((R0 & 3) + t) >> 2 known at compile time when r is, so the rows of one key
+  // group share a base address and differ by immediate offsets.
+  const int R0 = k2 * K + c0;
+  const int m = k2 * (K / 4) + (c0 >> 2), s0 = c0 & 3;
+  const uint8_t *Tb = T + R0 * FT;
+  vh_run<K, DIL, RC>(
+      [&](int t) {
+        const int u = (s0 + t) >> 2;
+        return lds_word((Tb + ((q2 ^ ((m + u) & 31)) << 2)) + t * FT);
+      },
+      r, e, o);
+#pragma unroll
+  for (int g = 0; g < K / 4; g++) {
+    uint32_t w[4];
+    transpose4(e + 4 * g, o + 4 * g, w);
+#pragma unroll
+    for (int c = 0; c < 4; c++) rows[c][g] = w[c];
+  }
+}
+
+// Store rows[c] (K bytes of tile row 4*q2 + c at output column k2*K) to the image.  Must be
+// called with every lane of the wave active (the unaligned path shifts words across lanes).
+template <int K, bool ALIGNED, int TW>
+__device__ __forceinline__ void sq_store_rows(const uint32_t (&rows)[4][K / 4], uint8_t *__restrict__ dst_img, int H,
+                                              int W, int y0, int x0, int q2, int k2, bool full)
+{
+  constexpr int NR = TW / K;
+  const int gx = x0 + k2 * K;
+  uint8_t *d0 = dst_img + (int64_t)(y0 + 4 * q2) * W + gx;
+  // rows of any alignment: the last word of the previous lane's run of each row (lanes
+  // k2 - 1 and k2 are neighbouring runs of one row segment)
+  uint32_t prevw[4] = {0u, 0u, 0u, 0u};
+  if (!ALIGNED) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) prevw[c] = __builtin_amdgcn_mov_dpp(rows[c][K / 4 - 1], 0x138, 0xf, 0xf, false);
+  }
+  if (ALIGNED && full) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      uint8_t *dst = d0 + (int64_t)c * W;
+      if (K >= 16) {
+#pragma unroll
+        for (int hh = 0; hh < K / 16; hh++)
+          *reinterpret_cast<uint4 *>(dst + 16 * hh) =
+              make_uint4(rows[c][4 * hh], rows[c][4 * hh + 1], rows[c][4 * hh + 2], rows[c][4 * hh + 3]);
+      } else {
+#pragma unroll
+        for (int g = 0; g < K / 4; g++) reinterpret_cast<uint32_t *>(dst)[g] = rows[c][g];
+      }
+    }
+    return;
+  }
+  // W is a multiple of 16 when ALIGNED: a 16-byte (or 4-byte) chunk starting inside the row is whole
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int gy = y0 + 4 * q2 + c;
+    if (gy >= H) continue;
+    uint8_t *dst = d0 + (int64_t)c * W;
+    if (ALIGNED) {
+      if (K >= 16) {
+#pragma unroll
+        for (int hh = 0; hh < K / 16; hh++)
+          if (gx + 16 * hh < W)
+            *reinterpret_cast<uint4 *>(dst + 16 * hh) =
+                make_uint4(rows[c][4 * hh], rows[c][4 * hh + 1], rows[c][4 * hh + 2], rows[c][4 * hh + 3]);
+      } else {
+#pragma unroll
+        for (int g = 0; g < K / 4; g++)
+          if (gx + 4 * g < W) *reinterpret_cast<uint32_t *>(dst + 4 * g) = rows[c][g];
+      }
+    } else if (gx + K <= W) {
+      store_run_shared<K / 4>(dst, rows[c], prevw[c], k2 == 0, k2 == NR - 1 || gx + 2 * K > W);
+    } else {
+      // the run crosses the right edge: byte stores, clipped
+#pragma unroll
+      for (int g = 0; g < K / 4; g++)
+#pragma unroll
+        for (int bb = 0; bb < 4; bb++)
+          if (gx + 4 * g + bb < W) dst[4 * g + bb] = (uint8_t)(rows[c][g] >> (8 * bb));
+    }
+  }
+}
+
 // Passes 2 and 3 of one tile of FT rows and TW columns whose input region (rows
 // y0-r .. y0+FT+r-1, columns x0-R16 .. x0+TW+R16-1 of the image) is in smem (row pitch
 // AW = TW + 2*R16).  dst is the output image (row pitch W).  ALIGNED: W % 16 == 0 and dst
@@ -321,87 +412,13 @@ __device__ __forceinline__ void sq_tile_passes(uint8_t *smem, uint8_t *__restric
   {
     constexpr int NR = TW / K;
     constexpr int ntask = (FT / 4) * NR;
-    const int c0 = R16 - r;  // T row of output column 0
     // tile wholly inside the image: no bounds checks on the stores
     const bool full = (y0 + FT <= H) && (x0 + TW <= W);
     for (int task = tid; task < ntask; task += FTHREADS) {
       const int q2 = task / NR, k2 = task - q2 * NR;
-      uint32_t e[K], o[K];
-      // T row R0 + t, word q2: the swizzle key (R >> 2) & 31 is (m + u) & 31 with
-      // u = ((R0 & 3) + t) >> 2 known at compile time when r is, so the rows of one key
-      // group share a base address and differ by immediate offsets.
-      const int R0 = k2 * K + c0;
-      const int m = k2 * (K / 4) + (c0 >> 2), s0 = c0 & 3;
-      const uint8_t *Tb = T + R0 * FT;
-      vh_run<K, DIL, RC>(
-          [&](int t) {
-            const int u = (s0 + t) >> 2;
-            return lds_word((Tb + ((q2 ^ ((m + u) & 31)) << 2)) + t * FT);
-          },
-          r, e, o);
-      const int gx = x0 + k2 * K;
       uint32_t rows[4][K / 4];
-#pragma unroll
-      for (int g = 0; g < K / 4; g++) {
-        uint32_t w[4];
-        transpose4(e + 4 * g, o + 4 * g, w);
-#pragma unroll
-        for (int c = 0; c < 4; c++) rows[c][g] = w[c];
-      }
-      uint8_t *d0 = dst_img + (int64_t)(y0 + 4 * q2) * W + gx;
-      // rows of any alignment: the last word of the previous lane's run of each row (lanes
-      // k2 - 1 and k2 are neighbouring runs of one row segment); taken while every lane of
-      // the wave is active
-      uint32_t prevw[4] = {0u, 0u, 0u, 0u};
-      if (!ALIGNED) {
-#pragma unroll
-        for (int c = 0; c < 4; c++) prevw[c] = __builtin_amdgcn_mov_dpp(rows[c][K / 4 - 1], 0x138, 0xf, 0xf, false);
-      }
-      if (ALIGNED && full) {
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-          uint8_t *dst = d0 + (int64_t)c * W;
-          if (K >= 16) {
-#pragma unroll
-            for (int hh = 0; hh < K / 16; hh++)
-              *reinterpret_cast<uint4 *>(dst + 16 * hh) =
-                  make_uint4(rows[c][4 * hh], rows[c][4 * hh + 1], rows[c][4 * hh + 2], rows[c][4 * hh + 3]);
-          } else {
-#pragma unroll
-            for (int g = 0; g < K / 4; g++) reinterpret_cast<uint32_t *>(dst)[g] = rows[c][g];
-          }
-        }
-        continue;
-      }
-      // W is a multiple of 16 when ALIGNED: a 16-byte (or 4-byte) chunk starting inside the row is whole
-#pragma unroll
-      for (int c = 0; c < 4; c++) {
-        const int gy = y0 + 4 * q2 + c;
-        if (gy >= H) continue;
-        uint8_t *dst = d0 + (int64_t)c * W;
-        if (ALIGNED) {
-          if (K >= 16) {
-#pragma unroll
-            for (int hh = 0; hh < K / 16; hh++)
-              if (gx + 16 * hh < W)
-                *reinterpret_cast<uint4 *>(dst + 16 * hh) =
-                    make_uint4(rows[c][4 * hh], rows[c][4 * hh + 1], rows[c][4 * hh + 2], rows[c][4 * hh + 3]);
-          } else {
-#pragma unroll
-            for (int g = 0; g < K / 4; g++)
-              if (gx + 4 * g < W) *reinterpret_cast<uint32_t *>(dst + 4 * g) = rows[c][g];
-          }
-        } else if (gx + K <= W) {
-          store_run_shared<K / 4>(dst, rows[c], prevw[c], k2 == 0, k2 == NR - 1 || gx + 2 * K > W);
-        } else {
-          // the run crosses the right edge: byte stores, clipped
-#pragma unroll
-          for (int g = 0; g < K / 4; g++)
-#pragma unroll
-            for (int bb = 0; bb < 4; bb++)
-              if (gx + 4 * g + bb < W) dst[4 * g + bb] = (uint8_t)(rows[c][g] >> (8 * bb));
-        }
-      }
+      sq_pass3_rows<K, DIL, RC>(T, r, R16 - r, q2, k2, rows);
+      sq_store_rows<K, ALIGNED, TW>(rows, dst_img, H, W, y0, x0, q2, k2, full);
     }
   }
 }
@@ -460,6 +477,158 @@ __device__ __forceinline__ void sq_tile_load(uint8_t *smem, const uint8_t *__res
   }
 }
 
+
+// ---------------------------------------------------------------- fused gradient
+// Gradient = dilation - erosion by the same square.  One region serves both: pixels outside
+// the image are replaced by the nearest image pixel (rows and columns clamped).  A box
+// window that reaches outside the image always contains the clamped-to pixel itself, so
+// max and min over the clamped region equal max and min over the window's image part, i.e.
+// the reference's neutral padding for both operators at once.
+
+// value of the 16-byte region chunk at column gx of image row `row` with columns clamped
+// to [0, W)
+template <bool ALIGNED>
+__device__ __forceinline__ uint4 load_chunk_clamped(const uint8_t *__restrict__ row, int W, int gx)
+{
+  if (gx + 16 <= 0) {
+    const uint32_t b = row[0] * 0x01010101u;
+    return make_uint4(b, b, b, b);
+  }
+  if (gx >= W) {
+    const uint32_t b = row[W - 1] * 0x01010101u;
+    return make_uint4(b, b, b, b);
+  }
+  if (gx >= 0 && gx + 16 <= W) return ALIGNED ? *reinterpret_cast<const uint4 *>(row + gx) : load16_any(row + gx);
+  // straddles the right edge (chunks never straddle column 0)
+  const uint8_t last = row[W - 1];
+  if (W >= 16) return shift16<false>(load16_any(row + (W - 16)), gx + 16 - W, last * 0x01010101u);
+  uint32_t wd[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    uint32_t acc = 0;
+#pragma unroll
+    for (int bb = 0; bb < 4; bb++) {
+      const int x = gx + 4 * k + bb;
+      const uint32_t byte = x < 0 ? row[0] : x < W ? row[x] : last;
+      acc |= byte << (8 * bb);
+    }
+    wd[k] = acc;
+  }
+  return make_uint4(wd[0], wd[1], wd[2], wd[3]);
+}
+
+// region of the tile at (y0, x0) with clamped rows and columns (see above)
+template <int K, bool ALIGNED, int RC, int TW>
+__device__ __forceinline__ void sq_tile_load_clamped(uint8_t *smem, const uint8_t *__restrict__ src, int H, int W,
+                                                     int r_rt, int y0, int x0)
+{
+  constexpr int MAXC = sq_maxc(K, TW);
+  constexpr int R16 = sq_r16<K>();
+  constexpr int AW = TW + 2 * R16, cpr = AW >> 4;
+  const int r = RC > 0 ? RC : r_rt;
+  const int nchunk = (FT + 2 * r) * cpr;
+  const int gx0 = x0 - R16;
+  uint4 v[MAXC];
+#pragma unroll
+  for (int i = 0; i < MAXC; i++) {
+    const int c = threadIdx.x + i * FTHREADS;
+    v[i] = make_uint4(0u, 0u, 0u, 0u);
+    if (c < nchunk) {
+      const int ri = c / cpr, cc = c - ri * cpr;
+      const int gy = min(max(y0 - r + ri, 0), H - 1);
+      v[i] = load_chunk_clamped<ALIGNED>(src + (int64_t)gy * W, W, gx0 + (cc << 4));
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < MAXC; i++) {
+    const int c = threadIdx.x + i * FTHREADS;
+    if (c < nchunk) {
+      const int ri = c / cpr, cc = c - ri * cpr;
+      *reinterpret_cast<uint4 *>(smem + ri * AW + (cc << 4)) = v[i];
+    }
+  }
+}
+
+// LDS bytes of the gradient tile: region / max-transposed buffer, then the min-transposed one
+inline size_t sq_grad_lds_bytes(int r, int TW)
+{
+  const int R16 = sq_k_of(r) == 32 ? 32 : 16;
+  const size_t a = (size_t)(FT + 2 * r) * (TW + 2 * R16), t = (size_t)(TW + 2 * R16) * FT;
+  return (a > t ? a : t) + t;
+}
+
+// Passes 2 and 3 of the gradient: the vertical pass computes max and min of each task from
+// the same LDS words, the two transposed results go to T_max (reusing the region's
+// storage) and T_min (after it), and the horizontal pass subtracts min from max (bytewise,
+// with no borrow since max >= min in every byte).
+template <int K, bool ALIGNED, int RC, int TW>
+__device__ __forceinline__ void sq_tile_grad_passes(uint8_t *smem, uint8_t *__restrict__ dst_img, int H, int W, int r_rt,
+                                                    int y0, int x0)
+{
+  constexpr int R16 = sq_r16<K>();
+  constexpr int AW = TW + 2 * R16;
+  const int r = RC > 0 ? RC : r_rt;
+  uint8_t *A = smem;
+  uint8_t *TM = smem;
+  const size_t a_bytes = (size_t)(FT + 2 * r) * AW, t_bytes = (size_t)AW * FT;
+  uint8_t *Tm = smem + (a_bytes > t_bytes ? a_bytes : t_bytes);
+  const int tid = threadIdx.x;
+  {
+    constexpr int MAXT = sq_maxt(K, TW);
+    const int q0 = (R16 - r) >> 2, q1 = (R16 + TW + r + 3) >> 2;
+    const int nq = q1 - q0;
+    const int ntask = nq * (FT / K);
+    uint32_t rmax[MAXT][K], rmin[MAXT][K];
+#pragma unroll
+    for (int sl = 0; sl < MAXT; sl++) {
+      const int task = tid + sl * FTHREADS;
+      if (task < ntask) {
+        const int k = task / nq, q = q0 + (task - k * nq);
+        uint32_t e[K], o[K];
+        const uint8_t *base = A + (k * K) * AW + 4 * q;
+        vh_run<K, true, RC>([&](int t) { return lds_word(base + t * AW); }, r, e, o);
+#pragma unroll
+        for (int g = 0; g < K / 4; g++) transpose4(e + 4 * g, o + 4 * g, &rmax[sl][4 * g]);
+        vh_run<K, false, RC>([&](int t) { return lds_word(base + t * AW); }, r, e, o);
+#pragma unroll
+        for (int g = 0; g < K / 4; g++) transpose4(e + 4 * g, o + 4 * g, &rmin[sl][4 * g]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int sl = 0; sl < MAXT; sl++) {
+      const int task = tid + sl * FTHREADS;
+      if (task < ntask) {
+        const int k = task / nq, q = q0 + (task - k * nq);
+#pragma unroll
+        for (int g = 0; g < K / 4; g++)
+#pragma unroll
+          for (int c = 0; c < 4; c++) {
+            const int off = t_off(4 * q + c, (k * K >> 2) + g);
+            *reinterpret_cast<uint32_t *>(TM + off) = rmax[sl][4 * g + c];
+            *reinterpret_cast<uint32_t *>(Tm + off) = rmin[sl][4 * g + c];
+          }
+      }
+    }
+  }
+  __syncthreads();
+  {
+    constexpr int NR = TW / K;
+    constexpr int ntask = (FT / 4) * NR;
+    const bool full = (y0 + FT <= H) && (x0 + TW <= W);
+    for (int task = tid; task < ntask; task += FTHREADS) {
+      const int q2 = task / NR, k2 = task - q2 * NR;
+      uint32_t rows[4][K / 4], lo[4][K / 4];
+      sq_pass3_rows<K, true, RC>(TM, r, R16 - r, q2, k2, rows);
+      sq_pass3_rows<K, false, RC>(Tm, r, R16 - r, q2, k2, lo);
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int g = 0; g < K / 4; g++) rows[c][g] -= lo[c][g];
+      sq_store_rows<K, ALIGNED, TW>(rows, dst_img, H, W, y0, x0, q2, k2, full);
+    }
+  }
+}
 
 constexpr int SQ_TILE_RMAX = 32;
 // LDS bytes of one workgroup for radius r and tile width TW (region A and transposed

@@ -310,6 +310,30 @@ def test_square_u8_fused(shape, r):
         np.testing.assert_array_equal(h(f(g(img), pm.static_nl_se.square2d, r)), exp)
 
 
+@pytest.mark.parametrize("shape", [(256, 272), (130, 48), (77, 1001), (3, 5), (129, 130), (300, 16), (20, 33)])
+@pytest.mark.parametrize("r", [1, 2, 3, 4, 7, 8, 15, 16, 31, 32, 33])
+def test_gradient_u8_fused(shape, r):
+    """uint8 gradient by a square (one fused launch for r <= 32) == dilation - erosion."""
+    img = rand(shape, np.uint8, 80 + r)
+    d = oracle.static_iter(oracle.line_x(img, r, True), "SESegmentY2D", min(r, shape[0]), True)
+    e = oracle.static_iter(oracle.line_x(img, r, False), "SESegmentY2D", min(r, shape[0]), False)
+    np.testing.assert_array_equal(h(pm.gradient(g(img), pm.static_nl_se.square2d, r)), oracle.ssub_image(d, e))
+
+
+def test_gradient_u8_fused_unaligned_view():
+    base = torch.from_numpy(rand((70 * 61 + 8,), np.uint8, 90)).to(DEV)
+    view = base[3:3 + 70 * 61].view(70, 61)
+    obuf = torch.full((70 * 61 + 8,), 5, dtype=torch.uint8, device=DEV)
+    oview = obuf[1:1 + 70 * 61].view(70, 61)
+    img = view.cpu().numpy()
+    pm.gradient(view, pm.static_nl_se.square2d, 2, out=oview)
+    d = oracle.static_iter(img, "SESquare2D", 2, True)
+    e = oracle.static_iter(img, "SESquare2D", 2, False)
+    np.testing.assert_array_equal(h(oview), oracle.ssub_image(d, e))
+    rest = obuf.cpu().numpy()
+    assert rest[0] == 5 and (rest[1 + 70 * 61:] == 5).all()
+
+
 @pytest.mark.parametrize("it", [1, 2, 5])
 def test_square3d_u8_fused(it):
     img = rand((6, 40, 48), np.uint8, 70)
