@@ -166,13 +166,35 @@ int pm_maxplus_bwd(const float *x, const float *gy, int64_t B, int64_t H, int64_
 /* ---- batched launches (no reference counterpart: the reference processes one image per call).
  * pm_batch_dilate_open: for each i < n, dil_outs[i] = Dilate(ins[i], SESquare2D, radii[i]) and
  * open_outs[i] = Dilate(Erode(ins[i], SESquare2D, radii[i]), SESquare2D, radii[i]), all images
- * uint8 2D of any sizes, radii in [1, 32], in one cooperative persistent launch.
+ * uint8 2D of any sizes, radii in [1, 32]: two persistent launches per distinct radius,
+ * whatever the number of images.
  * pm_batch_reconstruct: n reconstructions (geo::Reconstruction semantics, uint8 2D, 3x3 SEs)
  * sharing one sequence of global rounds. */
 int pm_batch_dilate_open(int n, const pm_image *ins, const int *radii, pm_image *dil_outs, pm_image *open_outs,
                          void *stream);
 int pm_batch_reconstruct(int n, int rec_type, const pm_image *markers, const pm_image *masks, int se, pm_image *outs,
                          void *stream);
+
+/* ---- batch plans (no reference counterpart): pm_batch_dilate_open captured once into a HIP
+ * graph, with its scratch and descriptor table allocated at creation; a launch is one
+ * hipGraphLaunch on the caller's stream, with no allocation and no per-image host work.  The
+ * plan keeps the image pointers it was created with; launches on one stream are ordered,
+ * launches of one plan on different streams at the same time race on its scratch. */
+typedef struct pm_batch_plan pm_batch_plan;
+int pm_batch_plan_create(int n, const pm_image *ins, const int *radii, pm_image *dil_outs, pm_image *open_outs,
+                         pm_batch_plan **plan);
+int pm_batch_plan_launch(pm_batch_plan *plan, void *stream);
+int pm_batch_plan_destroy(pm_batch_plan *plan);
+
+/* ---- autotuner (no reference counterpart).  The fused uint8 square kernel has two tile
+ * widths (128 and 256 columns) per radius; pm_autotune times both for every radius 1..32 on a
+ * 4096 x 4096 image with 16-byte aligned rows and on a 4096 x 4093 one with unaligned rows
+ * (device memory allocated and freed by the call; the stream is synchronised) and keeps the
+ * faster in a per-device table used by every later call.  pm_tuning_get returns the width in
+ * use for (aligned != 0, r); pm_tuning_set sets it (128, 256, or 0 = built-in rule). */
+int pm_autotune(void *stream);
+int pm_tuning_get(int aligned, int r);
+int pm_tuning_set(int aligned, int r, int tile_width);
 
 #ifdef __cplusplus
 }

@@ -420,3 +420,103 @@ def batch_reconstruction(rec_type, markers, masks, se, outs=None):
     _call("pm_batch_reconstruct", n, int(rec_type), _rec_ptr(dm), _rec_ptr(dk), int(se), _rec_ptr(do),
           _stream(masks[0]))
     return outs
+
+
+class BatchPlan:
+    """batch_dilate_open captured once into a HIP graph (pm_batch_plan_create): its scratch and
+    descriptor table are allocated at creation, and run() is one graph launch on the current
+    stream with no allocation and no per-image host work.  The plan keeps references to its
+    tensors; their contents may change between runs, their storage may not."""
+
+    def __init__(self, images, radii, dil_outs=None, open_outs=None):
+        n = len(images)
+        if len(radii) != n:
+            raise PoutreError(-6, "batch: one radius per image")
+        self.dil_outs = [_out(im, None) for im in images] if dil_outs is None else dil_outs
+        self.open_outs = [_out(im, None) for im in images] if open_outs is None else open_outs
+        self._keep = (list(images), self.dil_outs, self.open_outs)
+        recs = (_descs(images), _descs(self.dil_outs), _descs(self.open_outs))
+        self._create(n, recs, radii, images[0] if n else None)
+
+    @classmethod
+    def packed(cls, flat, shapes, radii, dil_flat=None, open_flat=None):
+        """Plan over images packed back to back in one 1D uint8 tensor (see batch_dilate_open_packed)."""
+        self = cls.__new__(cls)
+        dil_flat = _out(flat, dil_flat)
+        open_flat = _out(flat, open_flat)
+        self.dil_outs, self.open_outs = dil_flat, open_flat
+        self._keep = (flat, dil_flat, open_flat)
+        recs = (_packed(flat, shapes, "batch"), _packed(dil_flat, shapes, "batch"), _packed(open_flat, shapes, "batch"))
+        self._create(len(shapes), recs, radii, flat)
+        return self
+
+    def _create(self, n, recs, radii, ref):
+        self._plan = None
+        self._device = ref.device if ref is not None else None
+        rr = np.ascontiguousarray([int(r) for r in radii], dtype=np.int32)
+        h = ctypes.c_void_p()
+        di, dd, do = recs
+        _call("pm_batch_plan_create", n, _rec_ptr(di), rr.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _rec_ptr(dd),
+              _rec_ptr(do), ctypes.byref(h))
+        self._plan = h
+
+    def run(self):
+        if self._plan is None:
+            raise PoutreError(-6, "batch plan: closed")
+        stream = ctypes.c_void_p(_torch().cuda.current_stream(self._device).cuda_stream)
+        _call("pm_batch_plan_launch", self._plan, stream)
+        return self.dil_outs, self.open_outs
+
+    def close(self):
+        if getattr(self, "_plan", None) is not None:
+            _capi.lib().pm_batch_plan_destroy(self._plan)
+            self._plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+
+# ---------------------------------------------------------------- autotuner
+def tuning():
+    """The fused uint8 square kernel's tile width per radius 1..32 on the current device:
+    {"aligned": [...], "any": [...]} (rows 16-byte aligned, rows of any alignment)."""
+    L = _capi.lib()
+    out = {}
+    for key, al in (("aligned", 1), ("any", 0)):
+        vals = []
+        for r in range(1, 33):
+            v = L.pm_tuning_get(al, r)
+            if v < 0:
+                _capi.check(v)
+            vals.append(v)
+        out[key] = vals
+    return out
+
+
+def autotune(cache=None):
+    """Pick the fused square kernel's tile width for every radius by timing (pm_autotune).
+    With `cache` (a JSON path): load the table from it when it was written for this GPU's
+    name, otherwise measure and write it.  Returns the table (see tuning())."""
+    import json
+    import os
+    torch = _torch()
+    name = torch.cuda.get_device_properties(torch.cuda.current_device()).name
+    if cache is not None and os.path.exists(cache):
+        with open(cache) as f:
+            data = json.load(f)
+        if data.get("device") == name and data.get("version") == version():
+            for key, al in (("aligned", 1), ("any", 0)):
+                for r, w in enumerate(data["tile_width"][key], start=1):
+                    _call("pm_tuning_set", al, r, int(w))
+            return tuning()
+    _call("pm_autotune", ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    table = tuning()
+    if cache is not None:
+        d = os.path.dirname(os.path.abspath(cache))
+        os.makedirs(d, exist_ok=True)
+        with open(cache, "w") as f:
+            json.dump({"device": name, "version": version(), "tile_width": table}, f)
+    return table

@@ -73,6 +73,12 @@ SIGNATURES = {
     "pm_maxplus_bwd": [_VP, _VP, _I64, _I64, _I64, _VP, _I, _I, _VP, _VP, _VP],
     "pm_batch_dilate_open": [_I, _P, ctypes.POINTER(_I), _P, _P, _VP],
     "pm_batch_reconstruct": [_I, _I, _P, _P, _I, _P, _VP],
+    "pm_batch_plan_create": [_I, _P, ctypes.POINTER(_I), _P, _P, ctypes.POINTER(ctypes.c_void_p)],
+    "pm_batch_plan_launch": [ctypes.c_void_p, _VP],
+    "pm_batch_plan_destroy": [ctypes.c_void_p],
+    "pm_autotune": [_VP],
+    "pm_tuning_get": [_I, _I],
+    "pm_tuning_set": [_I, _I, _I],
 }
 
 

@@ -24,21 +24,22 @@ static batch_kernel_t batch_kernel(int r, int phase)
   return batch_kernel_range<25>(r, phase);
 }
 
-}  // namespace pm
 
-
-using namespace pm;
-
-extern "C" int pm_batch_dilate_open(int n, const pm_image *ins, const int *radii, pm_image *dil_outs,
-                                    pm_image *open_outs, void *stream)
-{
-  clear_error();
-  if (n < 0 || (n > 0 && (!ins || !radii || !dil_outs || !open_outs))) PM_FAIL(PM_E_ARG, "batch: bad arguments");
-  if (n == 0) return PM_OK;
-  hipStream_t s = as_stream(stream);
-  constexpr int NG = SQ_TILE_RMAX;  // one group per radius 1..32
+// A batch laid out for launching: images grouped by radius, tile counts, descriptor table
+// (host copy) and the scratch bytes the eroded images need.
+struct BatchLayout {
+  static constexpr int NG = SQ_TILE_RMAX;  // one group per radius 1..32
   std::vector<BatchImage> grp[NG];
-  int64_t tiles[NG] = {}, tmp_bytes = 0;
+  int64_t tiles[NG] = {};
+  int64_t tmp_bytes = 0;
+  size_t first[NG] = {};
+  std::vector<BatchImage> all;
+};
+
+static int batch_layout(int n, const pm_image *ins, const int *radii, pm_image *dil_outs, pm_image *open_outs,
+                        BatchLayout &L)
+{
+  if (n < 0 || (n > 0 && (!ins || !radii || !dil_outs || !open_outs))) PM_FAIL(PM_E_ARG, "batch: bad arguments");
   for (int i = 0; i < n; i++) {
     PM_TRY(check_unary(&ins[i], &dil_outs[i], "batch dilate"));
     PM_TRY(check_unary(&ins[i], &open_outs[i], "batch open"));
@@ -59,51 +60,167 @@ extern "C" int pm_batch_dilate_open(int n, const pm_image *ins, const int *radii
     b.W = (int)W;
     b.r = radii[i];
     b.tiles_x = (int)((W + TW - 1) / TW);
-    b.tile0 = (int)tiles[g];
-    tiles[g] += (int64_t)b.tiles_x * ((H + FT - 1) / FT);
-    if (tiles[g] > INT32_MAX / 4) PM_FAIL(PM_E_SIZE, "batch: too many tiles");
-    tmp_bytes += (H * W + 255) & ~(int64_t)255;
-    grp[g].push_back(b);
+    b.tile0 = (int)L.tiles[g];
+    L.tiles[g] += (int64_t)b.tiles_x * ((H + FT - 1) / FT);
+    if (L.tiles[g] > INT32_MAX / 4) PM_FAIL(PM_E_SIZE, "batch: too many tiles");
+    L.tmp_bytes += (H * W + 255) & ~(int64_t)255;
+    L.grp[g].push_back(b);
   }
-  if (tmp_bytes == 0) return PM_OK;
-  // scratch: eroded images (256-byte aligned each), then the descriptor tables
-  size_t desc_off = (size_t)tmp_bytes, desc_bytes = 0;
-  for (int g = 0; g < NG; g++) desc_bytes += grp[g].size() * sizeof(BatchImage);
-  Scratch sc;
-  PM_TRY(sc.alloc(desc_off + desc_bytes, s));
-  uint8_t *base = (uint8_t *)sc.ptr;
+  return PM_OK;
+}
+
+// Scratch layout: eroded images (256-byte aligned each), then the descriptor table.
+static size_t batch_scratch_bytes(const BatchLayout &L)
+{
+  size_t d = 0;
+  for (int g = 0; g < BatchLayout::NG; g++) d += L.grp[g].size() * sizeof(BatchImage);
+  return (size_t)L.tmp_bytes + d;
+}
+
+// point the descriptors at scratch `base` and build the flat host table
+static BatchImage *batch_bind(BatchLayout &L, uint8_t *base)
+{
   int64_t off = 0;
-  for (int g = 0; g < NG; g++)
-    for (auto &b : grp[g]) {
+  for (int g = 0; g < BatchLayout::NG; g++)
+    for (auto &b : L.grp[g]) {
       b.tmp = base + off;
       off += ((int64_t)b.H * b.W + 255) & ~(int64_t)255;
     }
-  std::vector<BatchImage> all;
-  all.reserve(desc_bytes / sizeof(BatchImage));
-  size_t first[NG];
-  for (int g = 0; g < NG; g++) {
-    first[g] = all.size();
-    all.insert(all.end(), grp[g].begin(), grp[g].end());
+  L.all.clear();
+  for (int g = 0; g < BatchLayout::NG; g++) {
+    L.first[g] = L.all.size();
+    L.all.insert(L.all.end(), L.grp[g].begin(), L.grp[g].end());
   }
-  BatchImage *dimg = (BatchImage *)(base + desc_off);
-  PM_HIP(hipMemcpyAsync(dimg, all.data(), desc_bytes, hipMemcpyHostToDevice, s));
+  return (BatchImage *)(base + L.tmp_bytes);
+}
+
+// enqueue the two phases of every radius group on s (kernel launches only, so that the
+// sequence can be captured into a graph)
+static int batch_enqueue(const BatchLayout &L, const BatchImage *dimg, hipStream_t s)
+{
   int dev = 0, cus = 0;
   PM_HIP(hipGetDevice(&dev));
   PM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   for (int phase = 0; phase < 2; phase++)
-    for (int g = 0; g < NG; g++) {
-      if (grp[g].empty()) continue;
+    for (int g = 0; g < BatchLayout::NG; g++) {
+      if (L.grp[g].empty()) continue;
       const int r = g + 1;
       const batch_kernel_t k = batch_kernel(r, phase);
       const size_t lds = sq_tile_lds_bytes(r, sq_tw(sq_k_of(r), r));
       int occ = 0;
       PM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, FTHREADS, lds));
-      const int64_t items = phase == 0 ? 2 * tiles[g] : tiles[g];
+      const int64_t items = phase == 0 ? 2 * L.tiles[g] : L.tiles[g];
       int64_t G = (int64_t)(occ > 0 ? occ : 1) * cus;
       if (G > items) G = items;
-      hipLaunchKernelGGL(k, dim3((unsigned)G), dim3(FTHREADS), lds, s, dimg + first[g], (int)grp[g].size(),
+      hipLaunchKernelGGL(k, dim3((unsigned)G), dim3(FTHREADS), lds, s, dimg + L.first[g], (int)L.grp[g].size(),
                          (int)items);
       PM_HIP(hipGetLastError());
     }
+  return PM_OK;
+}
+
+}  // namespace pm
+
+using namespace pm;
+
+extern "C" int pm_batch_dilate_open(int n, const pm_image *ins, const int *radii, pm_image *dil_outs,
+                                    pm_image *open_outs, void *stream)
+{
+  clear_error();
+  BatchLayout L;
+  PM_TRY(batch_layout(n, ins, radii, dil_outs, open_outs, L));
+  if (L.tmp_bytes == 0) return PM_OK;
+  hipStream_t s = as_stream(stream);
+  Scratch sc;
+  PM_TRY(sc.alloc(batch_scratch_bytes(L), s));
+  BatchImage *dimg = batch_bind(L, (uint8_t *)sc.ptr);
+  PM_HIP(hipMemcpyAsync(dimg, L.all.data(), L.all.size() * sizeof(BatchImage), hipMemcpyHostToDevice, s));
+  return batch_enqueue(L, dimg, s);
+}
+
+// A batch captured once into a graph: its scratch and descriptor table are allocated at
+// creation and live with the plan, so a launch is one hipGraphLaunch with no allocation and
+// no host-side work per image.
+struct pm_batch_plan {
+  int device = 0;
+  void *scratch = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int64_t pixels = 0;
+};
+
+static void plan_free(pm_batch_plan *p)
+{
+  if (!p) return;
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->scratch) (void)hipFree(p->scratch);
+  delete p;
+}
+
+extern "C" int pm_batch_plan_create(int n, const pm_image *ins, const int *radii, pm_image *dil_outs,
+                                    pm_image *open_outs, pm_batch_plan **plan)
+{
+  clear_error();
+  if (!plan) PM_FAIL(PM_E_ARG, "batch plan: null plan pointer");
+  *plan = nullptr;
+  BatchLayout L;
+  PM_TRY(batch_layout(n, ins, radii, dil_outs, open_outs, L));
+  pm_batch_plan *p = new pm_batch_plan();
+  PM_HIP(hipGetDevice(&p->device));
+  const size_t bytes = batch_scratch_bytes(L);
+  hipError_t e = hipMalloc(&p->scratch, bytes ? bytes : 256);
+  if (e != hipSuccess) {
+    plan_free(p);
+    PM_FAIL(PM_E_HIP, "batch plan: scratch allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+  }
+  BatchImage *dimg = batch_bind(L, (uint8_t *)p->scratch);
+  if (!L.all.empty()) {
+    e = hipMemcpy(dimg, L.all.data(), L.all.size() * sizeof(BatchImage), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      plan_free(p);
+      PM_FAIL(PM_E_HIP, "batch plan: descriptor upload failed: %s", hipGetErrorString(e));
+    }
+  }
+  // capture the launches on a private stream (default priority)
+  hipStream_t cs = nullptr;
+  e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  int rc = PM_OK;
+  if (e == hipSuccess) {
+    rc = batch_enqueue(L, dimg, cs);
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(cs, &g);
+    p->graph = g;
+    if (rc == PM_OK && e2 != hipSuccess) e = e2;
+  }
+  if (cs) (void)hipStreamDestroy(cs);
+  if (rc == PM_OK && e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+  if (rc != PM_OK || e != hipSuccess) {
+    plan_free(p);
+    if (rc != PM_OK) return rc;
+    PM_FAIL(PM_E_HIP, "batch plan: graph capture failed: %s", hipGetErrorString(e));
+  }
+  for (int g = 0; g < BatchLayout::NG; g++)
+    for (const auto &b : L.grp[g]) p->pixels += (int64_t)b.H * b.W;
+  *plan = p;
+  return PM_OK;
+}
+
+extern "C" int pm_batch_plan_launch(pm_batch_plan *plan, void *stream)
+{
+  clear_error();
+  if (!plan || !plan->exec) PM_FAIL(PM_E_ARG, "batch plan: null plan");
+  int dev = 0;
+  PM_HIP(hipGetDevice(&dev));
+  if (dev != plan->device) PM_FAIL(PM_E_ARG, "batch plan: created on device %d, launched on %d", plan->device, dev);
+  PM_HIP(hipGraphLaunch(plan->exec, as_stream(stream)));
+  return PM_OK;
+}
+
+extern "C" int pm_batch_plan_destroy(pm_batch_plan *plan)
+{
+  clear_error();
+  plan_free(plan);
   return PM_OK;
 }

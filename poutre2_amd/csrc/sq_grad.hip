@@ -1,4 +1,4 @@
-// sq_grad.hip -- fused morphological gradient of uint8 images by a (2r+1)^2 square, r <= 32:
+// sq_grad.hip -- fused morphological gradient of uint8 images by a (2r+1)^2 square, r <= 15:
 // out = Dilate(in, SESquare2D, r) - Erode(in, SESquare2D, r) in one launch (the composition
 // of ero_dil_static.cpp:229-305 with arith_saturated_sub_image; max >= min, so the
 // subtraction never saturates).  One read of the input, one write of the output; the tile
@@ -56,13 +56,14 @@ int sq_u8_gradient(const uint8_t *in, uint8_t *out, int64_t B, int64_t H, int64_
 {
   const int64_t n = B * H * W;
   if (n == 0) return PM_OK;
-  if (r < 1 || r > SQ_U8_RMAX) PM_FAIL(PM_E_ARG, "fused gradient radius %lld outside [1, %d]", (long long)r, (int)SQ_U8_RMAX);
+  if (r < 1 || r > SQ_U8_GRAD_RMAX)
+    PM_FAIL(PM_E_ARG, "fused gradient radius %lld outside [1, %d]", (long long)r, (int)SQ_U8_GRAD_RMAX);
   if (H > INT32_MAX / 2 || W > INT32_MAX / 2) PM_FAIL(PM_E_ARG, "image too large for the fused gradient");
   const int tiles_x = (int)((W + GRAD_TW - 1) / GRAD_TW), tiles_y = (int)((H + FT - 1) / FT);
   const int64_t per = (int64_t)tiles_x * tiles_y, ntiles = per * B;
   if (ntiles > INT32_MAX) PM_FAIL(PM_E_ARG, "image too large for the fused gradient");
   const bool aligned = (W % 16) == 0 && ((uintptr_t)in % 16) == 0 && ((uintptr_t)out % 16) == 0;
-  const grad_kernel_t k = grad_pick((int)r, aligned, std::make_integer_sequence<int, (int)SQ_U8_RMAX>{});
+  const grad_kernel_t k = grad_pick((int)r, aligned, std::make_integer_sequence<int, (int)SQ_U8_GRAD_RMAX>{});
   const size_t lds = sq_grad_lds_bytes((int)r, GRAD_TW);
   int dev = 0, cus = 0, occ = 0;
   PM_HIP(hipGetDevice(&dev));

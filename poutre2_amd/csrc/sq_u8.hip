@@ -27,78 +27,37 @@
 // The kernel is persistent (as many workgroups as fit on the chip, each looping over
 // tiles).  Tiles are mapped XCD-aware: each XCD walks a contiguous tile range, so
 // neighbouring tiles share their halo rows in that XCD's L2.
+#include <atomic>
 #include <utility>
 
 #include "pm_common.hpp"
 #include "sq_u8.hpp"
+#include "sq_u8_kernel.hpp"
 #include "sq_u8_tile.hpp"
 
 namespace pm {
 
-// Persistent tile schedule.  Workgroups are dispatched round-robin over the 8 XCDs
-// (workgroup b on XCD b % 8); XCD x owns the contiguous tile range [x*n/8, (x+1)*n/8) so
-// that neighbouring tiles, which share halo rows, are read through the same L2.
-struct TileSched {
-  int first, step, end;
-};
-__device__ __forceinline__ TileSched tile_sched(int b, int G, int ntiles)
+// ---------------------------------------------------------------- tile-width table
+// Tile width per device, alignment class and radius: 0 = not tuned (the built-in rule
+// sq_tw), otherwise 128 or 256 as chosen by pm_autotune (or set by pm_tuning_set).
+static std::atomic<int> g_tw[PM_MAX_DEVICES][2][SQ_U8_RMAX + 1];
+
+static int tuned_tw(int dev, bool aligned, int r)
 {
-  if ((G & 7) != 0 || ntiles < 8) return {b, G, ntiles};
-  const int x = b & 7, j = b >> 3, gx = G >> 3;
-  const int lo = (int)((int64_t)x * ntiles / 8), hi = (int)((int64_t)(x + 1) * ntiles / 8);
-  return {lo + j, gx, hi};
+  const int v = (dev >= 0 && dev < PM_MAX_DEVICES) ? g_tw[dev][aligned ? 1 : 0][r].load(std::memory_order_relaxed) : 0;
+  return v ? v : sq_tw(sq_k_of(r), r);
 }
 
-template <int K, bool DIL, bool ALIGNED, int RC>
-__global__ __launch_bounds__(FTHREADS) void sq_u8_kernel(const uint8_t *__restrict__ in, uint8_t *__restrict__ out,
-                                                         int H, int W, int r_rt, int tiles_x, int tiles_per_img,
-                                                         int ntiles)
+static int sq_launch(bool dil, bool aligned, int tw, const uint8_t *in, uint8_t *out, int64_t B, int64_t H, int64_t W,
+                     int r, int cus, hipStream_t s)
 {
-  constexpr int TW = sq_tw(K, RC);
-  extern __shared__ __align__(16) uint8_t smem[];
-  // A tile's region is read with all its loads in flight, then copied to LDS.  The
-  // workgroups resident on a CU overlap each other's loads and compute; prefetching the
-  // next tile into registers instead was slower (r = 1: 34.9 vs 33.5 us, r = 7: 40.2 vs
-  // 36.5 us, r = 31: 55.7 vs 52.1 us on 8192x8192), its registers costing occupancy.
-  const TileSched ts = tile_sched(blockIdx.x, gridDim.x, ntiles);
-  for (int tile = ts.first; tile < ts.end; tile += ts.step) {
-    const int img = tile / tiles_per_img;
-    const int t_in = tile - img * tiles_per_img;
-    const int ty = t_in / tiles_x, tx = t_in - ty * tiles_x;
-    const int y0 = ty * FT, x0 = tx * TW;
-    const int64_t ibase = (int64_t)img * H * W;
-    sq_tile_load<K, DIL, ALIGNED, RC, TW>(smem, in + ibase, H, W, r_rt, y0, x0);
-    __syncthreads();
-    sq_tile_passes<K, DIL, ALIGNED, RC, TW>(smem, out + ibase, H, W, r_rt, y0, x0);
-    __syncthreads();  // A/T are rewritten by the next tile
-  }
-}
-
-typedef void (*sq_kernel_t)(const uint8_t *, uint8_t *, int, int, int, int, int, int);
-
-// Kernel for radius r (1..32), specialised on r, for 16-byte aligned rows or any rows.
-template <bool DIL, int... RS>
-static sq_kernel_t sq_pick(int r, bool aligned, std::integer_sequence<int, RS...>)
-{
-  static constexpr sq_kernel_t spec[] = {sq_u8_kernel<sq_k_of(RS + 1), DIL, true, RS + 1>...};
-  static constexpr sq_kernel_t spec_any[] = {sq_u8_kernel<sq_k_of(RS + 1), DIL, false, RS + 1>...};
-  return aligned ? spec[r - 1] : spec_any[r - 1];
-}
-
-template <bool DIL>
-static int sq_u8_once(const uint8_t *in, uint8_t *out, int64_t B, int64_t H, int64_t W, int r, hipStream_t s)
-{
-  const bool aligned = (W % 16) == 0 && ((uintptr_t)in % 16) == 0 && ((uintptr_t)out % 16) == 0;
-  const int TW = sq_tw(sq_k_of(r), r);
-  const int tiles_x = (int)((W + TW - 1) / TW), tiles_y = (int)((H + FT - 1) / FT);
+  const int tiles_x = (int)((W + tw - 1) / tw), tiles_y = (int)((H + FT - 1) / FT);
   const int64_t per = (int64_t)tiles_x * tiles_y;
   const int64_t ntiles = per * B;
   if (ntiles > INT32_MAX) PM_FAIL(PM_E_ARG, "image too large for the fused square kernel");
-  const size_t lds = sq_tile_lds_bytes(r, TW);
-  int dev = 0, cus = 0;
-  PM_HIP(hipGetDevice(&dev));
-  PM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  const sq_kernel_t k = sq_pick<DIL>(r, aligned, std::make_integer_sequence<int, (int)SQ_U8_RMAX>{});
+  const size_t lds = sq_tile_lds_bytes(r, tw);
+  const sq_kernel_t k = dil ? (aligned ? sq_kernel_for<true, true>(r, tw) : sq_kernel_for<true, false>(r, tw))
+                            : (aligned ? sq_kernel_for<false, true>(r, tw) : sq_kernel_for<false, false>(r, tw));
   int occ = 0;
   PM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, FTHREADS, lds));
   int64_t g = (int64_t)(occ > 0 ? occ : 1) * cus;
@@ -108,6 +67,16 @@ static int sq_u8_once(const uint8_t *in, uint8_t *out, int64_t B, int64_t H, int
                      (int)ntiles);
   PM_HIP(hipGetLastError());
   return PM_OK;
+}
+
+template <bool DIL>
+static int sq_u8_once(const uint8_t *in, uint8_t *out, int64_t B, int64_t H, int64_t W, int r, hipStream_t s)
+{
+  const bool aligned = (W % 16) == 0 && ((uintptr_t)in % 16) == 0 && ((uintptr_t)out % 16) == 0;
+  int dev = 0, cus = 0;
+  PM_HIP(hipGetDevice(&dev));
+  PM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  return sq_launch(DIL, aligned, tuned_tw(dev, aligned, r), in, out, B, H, W, r, cus, s);
 }
 
 bool sq_u8_applicable(const void *, const void *, int64_t H, int64_t W)
@@ -150,3 +119,87 @@ template int sq_u8<true>(const uint8_t *, uint8_t *, int64_t, int64_t, int64_t, 
 template int sq_u8<false>(const uint8_t *, uint8_t *, int64_t, int64_t, int64_t, int64_t, hipStream_t);
 
 }  // namespace pm
+
+// ---------------------------------------------------------------- autotuner
+using namespace pm;
+
+extern "C" int pm_autotune(void *stream)
+{
+  clear_error();
+  hipStream_t s = as_stream(stream);
+  int dev = 0, cus = 0;
+  PM_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= PM_MAX_DEVICES) PM_FAIL(PM_E_HIP, "device %d outside the tuning table", dev);
+  PM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  // a 4096 x 4096 image (aligned rows) and a 4096 x 4093 one at odd offsets (any rows);
+  // min/max work does not depend on the pixel values
+  const int64_t H = 4096, W = 4096, Wu = 4093;
+  const size_t bytes = (size_t)H * W + 64;
+  uint8_t *a = nullptr, *b = nullptr;
+  PM_HIP(hipMalloc((void **)&a, bytes));
+  hipError_t e = hipMalloc((void **)&b, bytes);
+  if (e != hipSuccess) {
+    (void)hipFree(a);
+    PM_FAIL(PM_E_HIP, "autotune: %s", hipGetErrorString(e));
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = fill_bytes(a, 77, bytes, s);
+  if (rc == PM_OK && hipEventCreate(&e0) != hipSuccess) rc = PM_E_HIP;
+  if (rc == PM_OK && hipEventCreate(&e1) != hipSuccess) rc = PM_E_HIP;
+  for (int al = 1; al >= 0 && rc == PM_OK; al--) {
+    const uint8_t *in = al ? a : a + 1;
+    uint8_t *out = al ? b : b + 3;
+    const int64_t w = al ? W : Wu;
+    for (int r = 1; r <= (int)SQ_U8_RMAX && rc == PM_OK; r++) {
+      float best = 1e30f;
+      int best_tw = 0;
+      for (int tw : {128, 256}) {
+        rc = sq_launch(true, al != 0, tw, in, out, 1, H, w, r, cus, s);  // warm-up
+        if (rc != PM_OK) break;
+        if (hipEventRecord(e0, s) != hipSuccess) { rc = PM_E_HIP; break; }
+        for (int it = 0; it < 3 && rc == PM_OK; it++) rc = sq_launch(true, al != 0, tw, in, out, 1, H, w, r, cus, s);
+        if (rc != PM_OK) break;
+        float ms = 0.f;
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+          rc = PM_E_HIP;
+          break;
+        }
+        if (ms < best) {
+          best = ms;
+          best_tw = tw;
+        }
+      }
+      if (rc == PM_OK) g_tw[dev][al][r].store(best_tw, std::memory_order_relaxed);
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (rc != PM_OK && pm_last_error()[0] == 0) PM_FAIL(PM_E_HIP, "autotune: a timing call failed");
+  return rc;
+}
+
+extern "C" int pm_tuning_get(int aligned, int r)
+{
+  clear_error();
+  if (r < 1 || r > (int)SQ_U8_RMAX) PM_FAIL(PM_E_ARG, "tuning: radius %d outside [1, %d]", r, (int)SQ_U8_RMAX);
+  int dev = 0;
+  PM_HIP(hipGetDevice(&dev));
+  return tuned_tw(dev, aligned != 0, r);
+}
+
+extern "C" int pm_tuning_set(int aligned, int r, int tile_width)
+{
+  clear_error();
+  if (r < 1 || r > (int)SQ_U8_RMAX) PM_FAIL(PM_E_ARG, "tuning: radius %d outside [1, %d]", r, (int)SQ_U8_RMAX);
+  if (tile_width != 0 && tile_width != 128 && tile_width != 256)
+    PM_FAIL(PM_E_ARG, "tuning: tile width must be 128, 256 or 0 (built-in rule)");
+  int dev = 0;
+  PM_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= PM_MAX_DEVICES) PM_FAIL(PM_E_HIP, "device %d outside the tuning table", dev);
+  g_tw[dev][aligned ? 1 : 0][r].store(tile_width, std::memory_order_relaxed);
+  return PM_OK;
+}

@@ -12,8 +12,8 @@ bool sq_u8_applicable(const void *in, const void *out, int64_t H, int64_t W);
 // (2r+1)^2 square on B images of H x W; r > SQ_U8_RMAX runs several launches.
 template <bool DIL> int sq_u8(const uint8_t *in, uint8_t *out, int64_t B, int64_t H, int64_t W, int64_t r, hipStream_t s);
 
-// Dilate - Erode by the (2r+1)^2 square on B images of H x W in one launch, 1 <= r <= SQ_U8_RMAX
-// (the gradient entry point uses it up to SQ_U8_GRAD_RMAX)
+// Dilate - Erode by the (2r+1)^2 square on B images of H x W in one launch,
+// 1 <= r <= SQ_U8_GRAD_RMAX
 constexpr int64_t SQ_U8_GRAD_RMAX = 15;
 int sq_u8_gradient(const uint8_t *in, uint8_t *out, int64_t B, int64_t H, int64_t W, int64_t r, hipStream_t s);
 

@@ -113,3 +113,45 @@ def test_batch_packed_rejects_short_buffer():
     flat = torch.zeros(50, dtype=torch.uint8, device=DEV)
     with pytest.raises(pm.PoutreError):
         pm.batch_dilate_open_packed(flat, [(10, 10)], [1])
+
+
+def test_batch_plan_replays_match_eager():
+    """A graph-captured plan gives the eager call's bits on every replay, with eager work and
+    synchronisations between replays and the inputs changed in place."""
+    imgs, radii = _batch(9, 21, 300, 15)
+    ts = [torch.from_numpy(a).to(DEV) for a in imgs]
+    plan = pm.BatchPlan(ts, radii)
+    try:
+        for rep in range(4):
+            for t in ts:
+                t.random_(0, 256)
+            dils, opens = plan.run()
+            _ = torch.stack([t.float().mean() for t in ts]).sum().item()  # eager work between replays
+            torch.cuda.synchronize()
+            ed, eo = pm.batch_dilate_open(ts, radii)
+            torch.cuda.synchronize()
+            for a, b in zip(dils, ed):
+                assert torch.equal(a, b), rep
+            for a, b in zip(opens, eo):
+                assert torch.equal(a, b), rep
+    finally:
+        plan.close()
+
+
+def test_batch_plan_packed_matches_oracle():
+    imgs, radii = _batch(6, 22, 200, 32)
+    flat = torch.from_numpy(np.concatenate([a.reshape(-1) for a in imgs])).to(DEV)
+    plan = pm.BatchPlan.packed(flat, [a.shape for a in imgs], radii)
+    d, o = plan.run()
+    d, o = plan.run()
+    torch.cuda.synchronize()
+    d, o = d.cpu().numpy(), o.cpu().numpy()
+    off = 0
+    for a, r in zip(imgs, radii):
+        k = a.size
+        np.testing.assert_array_equal(d[off:off + k].reshape(a.shape), _square(a, r, True))
+        np.testing.assert_array_equal(o[off:off + k].reshape(a.shape), _square(_square(a, r, False), r, True))
+        off += k
+    plan.close()
+    with pytest.raises(pm.PoutreError):
+        plan.run()
