@@ -84,7 +84,8 @@ def bench_dilate_8k(pm, torch, steps, warmup, dev):
     achieved = 2 * npx / (t_dom * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": f"sq_u8_kernel<{kk}, true> (side {dom}, r={r_dom}) on 8192x8192 uint8",
+                "kernel": f"sq_u8_kernel<{kk}, true, true, {r_dom}, {pm.tuning()['aligned'][r_dom - 1]}> "
+                          f"(side {dom}) on 8192x8192 uint8",
                 "kernel_ms": t_dom, "bytes_per_launch": 2 * npx, "side": int(dom)}
     # op-level fraction of HBM peak over the whole step (2 B/px algorithmic)
     pct = 100 * 2 * npx * len(SIDES) * steps / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS
@@ -209,11 +210,18 @@ def bench_batched(pm, torch, dev, reps=3):
     fp = lambda: pm.batch_dilate_open_packed(flat, shapes, radii, dflat, oflat)  # noqa: E731
     fp()
     tpk = timed(fp, reps, stream) / reps
-    res = {"dilate_open_512": {"mpx": round(npx / 1e6, 1), "ms": round(tpk, 3),
-                               "gpx_s": round(npx / (tpk * 1e-3) / 1e9, 2),
-                               "pct_hbm": round(100 * 3 * npx / (tpk * 1e-3) / 1e9 / HBM_PEAK_GBS, 2),
-                               "list_api_ms": round(tb, 3), "api": "packed buffer (batch_dilate_open_packed)",
-                               "per_image_ms": round(tp, 3), "speedup_vs_per_image": round(tp / tpk, 2)}}
+    # the same packed batch captured once into a graph (pm_batch_plan): one launch per run
+    plan = pm.BatchPlan.packed(flat, shapes, radii, dflat, oflat)
+    plan.run()
+    tpl = timed(plan.run, reps, stream) / reps
+    plan.close()
+    tbest = min(tpk, tpl)
+    res = {"dilate_open_512": {"mpx": round(npx / 1e6, 1), "ms": round(tpl, 3),
+                               "gpx_s": round(npx / (tpl * 1e-3) / 1e9, 2),
+                               "pct_hbm": round(100 * 3 * npx / (tpl * 1e-3) / 1e9 / HBM_PEAK_GBS, 2),
+                               "api": "graph-captured plan over a packed buffer (BatchPlan.packed)",
+                               "packed_call_ms": round(tpk, 3), "list_api_ms": round(tb, 3),
+                               "per_image_ms": round(tp, 3), "speedup_vs_per_image": round(tp / tbest, 2)}}
     del imgs, dil, opn, flat, dflat, oflat
     g = synth.rng(6)
     masks, markers = [], []
@@ -296,13 +304,16 @@ def bench_autograd(pm, torch, dev, reps=3):
     return res
 
 
-def pmc_child(side):
+def pmc_child(side, tw=0):
     """Workload profiled by measure_traffic: a plain elementwise uint8 kernel of known traffic
     (bitwise_not over 64 MiB: 64 MiB read + 64 MiB written) and the 8K dilation of one side."""
     import torch
     import poutre2_amd as pm
     from poutre2_amd import synth
     dev = torch.device("cuda:0")
+    if tw:  # the tile width the parent run uses for this side (no tuning launches here)
+        r = (side - 1) // 2
+        pm._call("pm_tuning_set", 1, r, int(tw))
     base = torch.from_numpy(synth.uniform_u8((8192, 8192), 2)).to(dev)
     pairs = [(base.roll(i * 977, dims=1).contiguous(), torch.empty_like(base)) for i in range(3)]
     for i in range(6):
@@ -315,13 +326,13 @@ def pmc_child(side):
     print("pmc child done", flush=True)
 
 
-def _pmc_pass(counter, side, outdir):
+def _pmc_pass(counter, side, tw, outdir):
     import csv
     import glob
     import subprocess
     env = dict(os.environ, TMPDIR="/tmp")
     cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", outdir, "-o", "run", "--",
-           sys.executable, os.path.abspath(__file__), "--pmc-child", str(side)]
+           sys.executable, os.path.abspath(__file__), "--pmc-child", str(side), "--pmc-tw", str(tw)]
     subprocess.run(cmd, env=env, cwd="/tmp", stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=120,
                    check=False)
     files = glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True)
@@ -343,7 +354,7 @@ def _pmc_pass(counter, side, outdir):
     return (sum(vals["copy"][1:]) / (len(vals["copy"]) - 1), sum(vals["dil"][1:]) / (len(vals["dil"]) - 1))
 
 
-def measure_traffic(side):
+def measure_traffic(side, tw):
     """HBM bytes per launch of the dominant kernel from two rocprofv3 --pmc passes (FETCH_SIZE,
     then WRITE_SIZE, KiB per dispatch), each scaled by the factor that turns the same counter on
     a plain elementwise copy of 64 MiB into the bytes that copy moves."""
@@ -356,7 +367,7 @@ def measure_traffic(side):
     tmp = tempfile.mkdtemp(prefix="pm_pmc_", dir="/tmp")
     try:
         for counter in ("FETCH_SIZE", "WRITE_SIZE"):
-            copy_kib, dil_kib = _pmc_pass(counter, side, os.path.join(tmp, counter))
+            copy_kib, dil_kib = _pmc_pass(counter, side, tw, os.path.join(tmp, counter))
             factor = npx / (copy_kib * 1024.0)
             out[counter] = {"copy_kib": round(copy_kib, 1), "kernel_kib": round(dil_kib, 1),
                             "factor": round(factor, 4), "bytes": dil_kib * 1024.0 * factor}
@@ -397,9 +408,10 @@ def main():
     ap.add_argument("--sections", default="dilate,ops,recon,3d,small,batch,autograd")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc passes")
     ap.add_argument("--pmc-child", type=int, default=0, help=argparse.SUPPRESS)
+    ap.add_argument("--pmc-tw", type=int, default=0, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
-        pmc_child(args.pmc_child)
+        pmc_child(args.pmc_child, args.pmc_tw)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU path is a design "
@@ -409,12 +421,14 @@ def main():
     import poutre2_amd as pm
     dev = torch.device("cuda:0")
     sections = set(args.sections.split(","))
+    log("bench: autotune (tile width per radius, untimed)")
+    tuned = pm.autotune()
     log("bench: 8192x8192 u8 dilation, sides", SIDES)
     gpx, ms_step, per_side, roofline, pct = bench_dilate_8k(pm, torch, args.steps, args.warmup, dev)
     extra = {"dilate_per_side": per_side, "pct_hbm": pct}
     if not args.no_traffic:
         log("bench: rocprofv3 FETCH_SIZE / WRITE_SIZE passes on the dominant kernel")
-        traffic, detail = measure_traffic(roofline["side"])
+        traffic, detail = measure_traffic(roofline["side"], pm.tuning()["aligned"][(roofline["side"] - 1) // 2 - 1])
         roofline["traffic"] = None if traffic is None else round(traffic)
         roofline["traffic_detail"] = detail
     if "ops" in sections:
@@ -440,6 +454,7 @@ def main():
         log("bench: cpu baseline")
         cpu = cpu_baseline_8k()
     props = torch.cuda.get_device_properties(dev)
+    extra["tile_width_table"] = tuned
     line = {
         "metric": "Gpixel/s & % HBM peak: 8K uint8 dilate, 4K reconstruct; batched fwd+bwd",
         "value": round(gpx, 3), "unit": "Gpixel/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

@@ -188,7 +188,7 @@ int pm_batch_plan_destroy(pm_batch_plan *plan);
 
 /* ---- autotuner (no reference counterpart).  The fused uint8 square kernel has two tile
  * widths (128 and 256 columns) per radius; pm_autotune times both for every radius 1..32 on a
- * 4096 x 4096 image with 16-byte aligned rows and on a 4096 x 4093 one with unaligned rows
+ * 8192 x 8192 image with 16-byte aligned rows and on an 8192 x 8189 one with unaligned rows
  * (device memory allocated and freed by the call; the stream is synchronised) and keeps the
  * faster in a per-device table used by every later call.  pm_tuning_get returns the width in
  * use for (aligned != 0, r); pm_tuning_set sets it (128, 256, or 0 = built-in rule). */

@@ -131,33 +131,42 @@ extern "C" int pm_autotune(void *stream)
   PM_HIP(hipGetDevice(&dev));
   if (dev < 0 || dev >= PM_MAX_DEVICES) PM_FAIL(PM_E_HIP, "device %d outside the tuning table", dev);
   PM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  // a 4096 x 4096 image (aligned rows) and a 4096 x 4093 one at odd offsets (any rows);
-  // min/max work does not depend on the pixel values
-  const int64_t H = 4096, W = 4096, Wu = 4093;
+  // three 8192 x 8192 input/output pairs used in turn (384 MiB, more than the MALL holds,
+  // so the timings see HBM as a real workload does); the unaligned class uses 8192 x 8189
+  // images at odd offsets.  Min/max work does not depend on the pixel values.
+  constexpr int NP = 3;
+  const int64_t H = 8192, W = 8192, Wu = 8189;
   const size_t bytes = (size_t)H * W + 64;
-  uint8_t *a = nullptr, *b = nullptr;
-  PM_HIP(hipMalloc((void **)&a, bytes));
-  hipError_t e = hipMalloc((void **)&b, bytes);
-  if (e != hipSuccess) {
-    (void)hipFree(a);
-    PM_FAIL(PM_E_HIP, "autotune: %s", hipGetErrorString(e));
+  uint8_t *buf[2 * NP] = {};
+  int rc = PM_OK;
+  for (int i = 0; i < 2 * NP && rc == PM_OK; i++) {
+    const hipError_t e = hipMalloc((void **)&buf[i], bytes);
+    if (e != hipSuccess) {
+      buf[i] = nullptr;
+      set_error("autotune: %s", hipGetErrorString(e));
+      rc = PM_E_HIP;
+    }
   }
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  int rc = fill_bytes(a, 77, bytes, s);
+  for (int i = 0; i < NP && rc == PM_OK; i++) rc = fill_bytes(buf[i], 77, bytes, s);
   if (rc == PM_OK && hipEventCreate(&e0) != hipSuccess) rc = PM_E_HIP;
   if (rc == PM_OK && hipEventCreate(&e1) != hipSuccess) rc = PM_E_HIP;
+  int turn = 0;
   for (int al = 1; al >= 0 && rc == PM_OK; al--) {
-    const uint8_t *in = al ? a : a + 1;
-    uint8_t *out = al ? b : b + 3;
     const int64_t w = al ? W : Wu;
+    auto one = [&](int tw, int r) {
+      const int p = turn++ % NP;
+      return sq_launch(true, al != 0, tw, al ? buf[p] : buf[p] + 1, al ? buf[NP + p] : buf[NP + p] + 3, 1, H, w, r,
+                       cus, s);
+    };
     for (int r = 1; r <= (int)SQ_U8_RMAX && rc == PM_OK; r++) {
       float best = 1e30f;
       int best_tw = 0;
       for (int tw : {128, 256}) {
-        rc = sq_launch(true, al != 0, tw, in, out, 1, H, w, r, cus, s);  // warm-up
+        rc = one(tw, r);  // warm-up
         if (rc != PM_OK) break;
         if (hipEventRecord(e0, s) != hipSuccess) { rc = PM_E_HIP; break; }
-        for (int it = 0; it < 3 && rc == PM_OK; it++) rc = sq_launch(true, al != 0, tw, in, out, 1, H, w, r, cus, s);
+        for (int it = 0; it < 3 && rc == PM_OK; it++) rc = one(tw, r);
         if (rc != PM_OK) break;
         float ms = 0.f;
         if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
@@ -176,8 +185,8 @@ extern "C" int pm_autotune(void *stream)
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   (void)hipStreamSynchronize(s);
-  (void)hipFree(a);
-  (void)hipFree(b);
+  for (int i = 0; i < 2 * NP; i++)
+    if (buf[i]) (void)hipFree(buf[i]);
   if (rc != PM_OK && pm_last_error()[0] == 0) PM_FAIL(PM_E_HIP, "autotune: a timing call failed");
   return rc;
 }
