@@ -165,6 +165,32 @@ def bench_3d(pm, torch, dev, reps=3):
     return out
 
 
+def bench_types(pm, torch, dev, reps=5):
+    """4096x4096 uint16 / int32 / float32 dilation by squares of side 3 and 15 (van Herk x and y
+    line passes through a scratch image); %HBM at 2 * sizeof(T) B/px algorithmic."""
+    import numpy as np
+    from poutre2_amd import synth
+    g = synth.rng(9)
+    out = {}
+    for name, npt in (("uint16", np.uint16), ("int32", np.int32), ("float32", np.float32)):
+        if npt == np.float32:
+            a_np = g.standard_normal((4096, 4096), dtype=np.float32)
+        else:
+            a_np = g.integers(0, 60000, size=(4096, 4096)).astype(npt)
+        a = torch.from_numpy(a_np).to(dev)
+        b = torch.empty_like(a)
+        stream = torch.cuda.current_stream(dev)
+        per = {}
+        for side in (3, 15):
+            r = (side - 1) // 2
+            pm.dilate(a, pm.static_nl_se.square2d, r, out=b)
+            t = timed(lambda: pm.dilate(a, pm.static_nl_se.square2d, r, out=b), reps, stream) / reps
+            per[str(side)] = {"ms": round(t, 4), "gpx_s": round(a.numel() / (t * 1e-3) / 1e9, 2),
+                              "pct_hbm": round(100 * 2 * a.element_size() * a.numel() / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 2)}
+        out[name] = per
+    return out
+
+
 def bench_small_1k(pm, torch, dev, reps=50):
     """The reference's CPU-runnable case: 1024^2 uint8, SESquare2D iter 1, dilation and erosion."""
     from poutre2_amd import synth
@@ -405,7 +431,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--sections", default="dilate,ops,recon,3d,small,batch,autograd")
+    ap.add_argument("--sections", default="dilate,ops,recon,3d,types,small,batch,autograd")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc passes")
     ap.add_argument("--pmc-child", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--pmc-tw", type=int, default=0, help=argparse.SUPPRESS)
@@ -440,6 +466,9 @@ def main():
     if "3d" in sections:
         log("bench: 512^3 cross3d / square3d")
         extra["volume_512"] = bench_3d(pm, torch, dev)
+    if "types" in sections:
+        log("bench: 4096^2 uint16/int32/float32 dilation")
+        extra["types_4k"] = bench_types(pm, torch, dev)
     if "small" in sections:
         log("bench: 1024^2 square2d iter 1")
         extra["small_1k"] = bench_small_1k(pm, torch, dev)

@@ -4,10 +4,10 @@
 //  * nl_kernel      : max/min over an arbitrary neighbour list (the runtime-SE path,
 //                     ero_dil_runtime_nl_se_t.hpp:48-97), out-of-bounds taps skipped.
 //  * line_kernel    : max/min over a centred segment of 2r+1 along x or y, computed in an
-//                     LDS tile by log2 doubling (M_2k[i] = op(M_k[i], M_k[i+k])) and one
-//                     final combine, so the work per pixel grows with log(r), not r.
-//                     Replaces the van Herk row passes and the transpose-based Y pass
-//                     (ero_dil_line_se_t.hpp:49-280) without any transpose.
+//                     LDS tile (halo r on both sides) by van Herk / Gil-Werman block
+//                     prefix and suffix maxima, three ops per pixel whatever r is: the
+//                     reference's van Herk row passes (ero_dil_line_se_t.hpp:49-89) for
+//                     every type, and its Y pass (:248-280) without the transposes.
 // Host drivers map the reference's Common_NL_SE iterations onto these:
 //  SESquare2D^k = (2k+1)^2 rectangle (x pass then y pass), segments^k = lines of radius k,
 //  SESquare3D^k = box (x, y, z passes), crosses^k = k neighbour-list passes.
@@ -99,27 +99,36 @@ __global__ __launch_bounds__(256) void line_kernel(const T *__restrict__ in, T *
   }
   __syncthreads();
 
-  // doubling: after the level with shift k, buf holds M_2k (max over 2k consecutive taps)
+  // van Herk / Gil-Werman: cut the region into blocks of w = 2r+1 taps along the axis; g is
+  // the running max from each block's start (into buf1), h the running max from each
+  // block's end (in place in buf0); every window [j, j+2r] spans at most two blocks, so
+  // out[j] = op(h[j], g[j + 2r]).  Three ops per output whatever r is.
   const int w = 2 * r + 1;
-  int p = 1;
-  while (2 * p <= w) p *= 2;
   const int strideA = (AXIS == 0) ? 1 : C;   // step along the axis
   const int strideC = (AXIS == 0) ? Lr : 1;  // step across
-  for (int k = 1; k < p; k *= 2) {
-    const int len = Lr - 2 * k + 1;  // valid outputs of this level
-    const int n = len * C;
-    for (int e = threadIdx.x; e < n; e += 256) {
-      int i, c;
-      if (AXIS == 0) { c = e / len; i = e - c * len; }
-      else { i = e / C; c = e - i * C; }
-      const int o = i * strideA + c * strideC;
-      buf1[o] = L::op(buf0[o], buf0[o + k * strideA]);
+  {
+    const int nblk = (Lr + w - 1) / w;
+    const int ntask = nblk * C;
+    for (int e = threadIdx.x; e < ntask; e += 256) {
+      const int b = e / C, c = e - b * C;
+      const int i0 = b * w, i1 = min(i0 + w, Lr);
+      const int oc = c * strideC;
+      T acc = L::neutral();
+      for (int i = i0; i < i1; i++) {
+        const int o = i * strideA + oc;
+        acc = L::op(acc, buf0[o]);
+        buf1[o] = acc;
+      }
+      acc = L::neutral();
+      for (int i = i1 - 1; i >= i0; i--) {
+        const int o = i * strideA + oc;
+        acc = L::op(acc, buf0[o]);
+        buf0[o] = acc;
+      }
     }
-    __syncthreads();
-    T *t = buf0; buf0 = buf1; buf1 = t;
   }
-  // out[j] = op(M_p[j], M_p[j + w - p]) in region coordinates
-  const int sh = (w - p) * strideA;
+  __syncthreads();
+  const int sh = (w - 1) * strideA;
   const int n_out = S * C;
   if (AXIS == 0) {
     const int y0 = b_across * C, x0 = b_along * S;
@@ -128,7 +137,7 @@ __global__ __launch_bounds__(256) void line_kernel(const T *__restrict__ in, T *
       const int y = y0 + c, x = x0 + j;
       if (y < H && x < W) {
         const int o = j * strideA + c * strideC;
-        out[base + (int64_t)y * W + x] = L::op(buf0[o], buf0[o + sh]);
+        out[base + (int64_t)y * W + x] = L::op(buf0[o], buf1[o + sh]);
       }
     }
   } else {
@@ -138,7 +147,7 @@ __global__ __launch_bounds__(256) void line_kernel(const T *__restrict__ in, T *
       const int y = y0 + j, x = x0 + c;
       if (y < H && x < W) {
         const int o = j * strideA + c * strideC;
-        out[base + (int64_t)y * W + x] = L::op(buf0[o], buf0[o + sh]);
+        out[base + (int64_t)y * W + x] = L::op(buf0[o], buf1[o + sh]);
       }
     }
   }
