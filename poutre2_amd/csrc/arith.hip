@@ -6,6 +6,7 @@
 
 #include "arith.hpp"
 #include "erodil.hpp"
+#include "sq_open.hpp"
 #include "sq_u8.hpp"
 
 namespace pm {
@@ -154,6 +155,11 @@ template <int K> static int compose(const pm_image *in, int se, int iter, pm_ima
         // composition is faster (8192^2, r = 31: 0.143 vs 0.133 ms; r = 7: 0.053 vs 0.105 ms).
         if (K == C_GRAD && se == PM_SE_SQUARE_2D && rank == 2 && iter <= SQ_U8_GRAD_RMAX)
           return sq_u8_gradient(src, dst, 1, d.H, d.W, iter, s);
+        // opening / closing by a square: one fused launch, the eroded (dilated) image stays
+        // in LDS
+        if (K != C_GRAD && se == PM_SE_SQUARE_2D && rank == 2 && iter <= SQ_U8_OPEN_RMAX &&
+            d.H <= INT32_MAX / 2 && d.W <= INT32_MAX / 2)
+          return sq_u8_open_close(K == C_OPEN, src, dst, 1, d.H, d.W, iter, s);
       }
       Scratch tmp;
       PM_TRY(tmp.alloc((size_t)n * sizeof(T), s));

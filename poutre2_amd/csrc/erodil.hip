@@ -16,6 +16,7 @@
 #include "pm_common.hpp"
 #include "erodil.hpp"
 #include "sq_u8.hpp"
+#include "sq_gen.hpp"
 #include "vol_u8.hpp"
 
 #include <algorithm>
@@ -243,6 +244,8 @@ int rect_pass(const T *in, T *out, int64_t B, int64_t H, int64_t W, int64_t ry, 
   if (rx <= 0) return line_pass<T, DIL>(in, out, B, H, W, 1, ry, s);
   if constexpr (std::is_same<T, uint8_t>::value) {
     if (ry == rx && sq_u8_applicable(in, out, H, W)) return sq_u8<DIL>(in, out, B, H, W, ry, s);
+  } else {
+    if (ry == rx && sq_gen_applicable(H, W)) return sq_gen<T, DIL>(in, out, B, H, W, ry, s);
   }
   const int64_t n = B * H * W;
   if (n == 0) return PM_OK;

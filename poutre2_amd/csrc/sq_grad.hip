@@ -36,9 +36,11 @@ __global__ __launch_bounds__(FTHREADS) void sq_grad_kernel(const uint8_t *__rest
     const int ty = t_in / tiles_x, tx = t_in - ty * tiles_x;
     const int y0 = ty * FT, x0 = tx * GRAD_TW;
     const int64_t ibase = (int64_t)img * H * W;
-    sq_tile_load_clamped<K, ALIGNED, RC, GRAD_TW>(smem, in + ibase, H, W, RC, y0, x0);
+    // K = 16 (r >= 8): indices recomputed per tile (fewer registers) are faster; below, the
+    // hoisted ones (see tile_tid)
+    sq_tile_load_clamped<K, ALIGNED, RC, GRAD_TW, (K >= 16)>(smem, in + ibase, H, W, RC, y0, x0);
     __syncthreads();
-    sq_tile_grad_passes<K, ALIGNED, RC, GRAD_TW>(smem, out + ibase, H, W, RC, y0, x0);
+    sq_tile_grad_passes<K, ALIGNED, RC, GRAD_TW, (K >= 16)>(smem, out + ibase, H, W, RC, y0, x0);
     __syncthreads();
   }
 }

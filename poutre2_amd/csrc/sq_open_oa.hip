@@ -1,0 +1,20 @@
+// sq_open_oa.hip -- fused opening kernels, rows of 16 byte aligned, every radius
+// 1..SQ_U8_OPEN_RMAX (sq_open.hpp).
+#include <utility>
+
+#include "sq_open.hpp"
+
+namespace pm {
+
+template <int... RS> static open_kernel_t pick(int r, std::integer_sequence<int, RS...>)
+{
+  static constexpr open_kernel_t tab[] = {sq_open_kernel<sq_k_of(RS + 1), true, true, RS + 1, OPEN_TW>...};
+  return tab[r - 1];
+}
+
+template <> open_kernel_t sq_open_kernel_for<true, true>(int r)
+{
+  return pick(r, std::make_integer_sequence<int, (int)SQ_U8_OPEN_RMAX>{});
+}
+
+}  // namespace pm

@@ -43,6 +43,21 @@ __device__ __forceinline__ void transpose4(const uint32_t *e, const uint32_t *o,
   w[3] = __builtin_amdgcn_perm(ob, oa, 0x07060302u);
 }
 
+// The thread index, optionally (OPQ) as a value the compiler cannot prove loop-invariant.
+// In a persistent tile loop every LDS and region offset derived from the thread index alone
+// is otherwise hoisted out of the loop and kept live across all passes of every tile: fewer
+// instructions per tile, more VGPRs.  Which is faster depends on whether the registers limit
+// occupancy (LDS often does first).  Measured on 8192x8192 (us, hoisted vs opaque):
+// dilation r = 7 / 31: 38.5 / 47.7 vs 40.8 / 52.1 (hoisted kept); fused opening r = 7: 75.4
+// vs 59.7 (opaque passes, 205 -> 86 VGPRs); fused gradient r = 15: 81.0 vs 67.3 (opaque load
+// and passes), r = 7: 59.5 vs 63.1.
+template <bool OPQ> __device__ __forceinline__ int tile_tid()
+{
+  int t = threadIdx.x;
+  if constexpr (OPQ) __asm__ volatile("" : "+v"(t));
+  return t;
+}
+
 __device__ __forceinline__ uint32_t lds_word(const uint8_t *p) { return *reinterpret_cast<const uint32_t *>(p); }
 
 // Four dwords at a 4-byte aligned address: one global_load_dwordx4 (multi-dword accesses
@@ -268,10 +283,13 @@ __device__ __forceinline__ void vh_run(LD &&row, int r_rt, uint32_t (&e)[K], uin
 // Horizontal pass of one task (rows 4*q2 .. 4*q2+3 of the tile, output columns
 // k2*K .. k2*K+K-1) over the transposed buffer T, whose row c0 holds output column 0's
 // window start: rows[c][g] = output word g of tile row 4*q2 + c.
+// Windowed max/min along the rows of the transposed buffer T for the task (word q2, output
+// rows k2*K .. k2*K+K-1 of T's line order, whose row c0 holds line 0's window start): the
+// high-byte pairs e / o of vh_run.
 template <int K, bool DIL, int RC>
-__device__ __forceinline__ void sq_pass3_rows(const uint8_t *T, int r, int c0, int q2, int k2, uint32_t (&rows)[4][K / 4])
+__device__ __forceinline__ void sq_tline(const uint8_t *T, int r, int c0, int q2, int k2, uint32_t (&e)[K],
+                                         uint32_t (&o)[K])
 {
-  uint32_t e[K], o[K];
   // T row R0 + t, word q2: the swizzle key (R >> 2) & 31 is (m + u) & 31 with
   // u = ((R0 & 3) + t) >> 2 known at compile time when r is, so the rows of one key
   // group share a base address and differ by immediate offsets.
@@ -284,6 +302,13 @@ __device__ __forceinline__ void sq_pass3_rows(const uint8_t *T, int r, int c0, i
         return lds_word((Tb + ((q2 ^ ((m + u) & 31)) << 2)) + t * FT);
       },
       r, e, o);
+}
+
+template <int K, bool DIL, int RC>
+__device__ __forceinline__ void sq_pass3_rows(const uint8_t *T, int r, int c0, int q2, int k2, uint32_t (&rows)[4][K / 4])
+{
+  uint32_t e[K], o[K];
+  sq_tline<K, DIL, RC>(T, r, c0, q2, k2, e, o);
 #pragma unroll
   for (int g = 0; g < K / 4; g++) {
     uint32_t w[4];
@@ -429,24 +454,28 @@ __device__ __forceinline__ void sq_tile_passes(uint8_t *smem, uint8_t *__restric
 // value.  ALIGNED: W % 16 == 0 and src 16-byte aligned (a chunk is wholly inside or
 // outside the image); otherwise load16_any, and the chunk that straddles the right edge
 // of the image comes from load_edge_chunk.
-template <int K, bool DIL, bool ALIGNED, int RC = 0, int TW = FT>
+// PAD (a multiple of 16, default R16) replaces R16 as the column pad.
+template <int K, bool DIL, bool ALIGNED, int RC = 0, int TW = FT, int PAD = sq_r16<K>(), bool OPQ = false>
 __device__ __forceinline__ void sq_tile_load(uint8_t *smem, const uint8_t *__restrict__ src, int H, int W, int r_rt,
                                              int y0, int x0)
 {
-  constexpr int MAXC = sq_maxc(K, TW);
-  constexpr int R16 = sq_r16<K>();
+  constexpr int MAXC = PAD == sq_r16<K>() ? sq_maxc(K, TW)
+                                          : ((FT + 2 * (RC > 0 ? RC : sq_rmax_of(K))) * ((TW + 2 * PAD) >> 4) +
+                                             FTHREADS - 1) / FTHREADS;
+  constexpr int R16 = PAD;
   constexpr int AW = TW + 2 * R16, cpr = AW >> 4;
   const int r = RC > 0 ? RC : r_rt;
   const int nchunk = (FT + 2 * r) * cpr;
   const uint32_t nv = DIL ? 0u : 0xFFFFFFFFu;
   const int gx0 = x0 - R16;  // image column of region column 0 (a multiple of 16)
+  const int tid = tile_tid<OPQ>();
   // any width: the chunk of each region row that straddles the right edge (column W) is
   // skipped by the main loop and filled by the edge loop below
   const int ce = ((W - gx0) & 15) ? (W - gx0) >> 4 : -1;
   uint4 v[MAXC];
 #pragma unroll
   for (int i = 0; i < MAXC; i++) {
-    const int c = threadIdx.x + i * FTHREADS;
+    const int c = tid + i * FTHREADS;
     v[i] = make_uint4(nv, nv, nv, nv);
     if (c < nchunk) {
       const int ri = c / cpr, cc = c - ri * cpr;
@@ -461,14 +490,14 @@ __device__ __forceinline__ void sq_tile_load(uint8_t *smem, const uint8_t *__res
   }
 #pragma unroll
   for (int i = 0; i < MAXC; i++) {
-    const int c = threadIdx.x + i * FTHREADS;
+    const int c = tid + i * FTHREADS;
     if (c < nchunk) {
       const int ri = c / cpr, cc = c - ri * cpr;
       if (ALIGNED || cc != ce) *reinterpret_cast<uint4 *>(smem + ri * AW + (cc << 4)) = v[i];
     }
   }
   if (!ALIGNED && ce >= 0 && ce < cpr) {
-    for (int ri = threadIdx.x; ri < FT + 2 * r; ri += FTHREADS) {
+    for (int ri = tid; ri < FT + 2 * r; ri += FTHREADS) {
       const int gy = y0 - r + ri;
       uint4 e = make_uint4(nv, nv, nv, nv);
       if ((unsigned)gy < (unsigned)H) e = load_edge_chunk(src + (int64_t)gy * W, W, gx0 + (ce << 4), DIL ? 0 : 255);
@@ -518,7 +547,7 @@ __device__ __forceinline__ uint4 load_chunk_clamped(const uint8_t *__restrict__ 
 }
 
 // region of the tile at (y0, x0) with clamped rows and columns (see above)
-template <int K, bool ALIGNED, int RC, int TW>
+template <int K, bool ALIGNED, int RC, int TW, bool OPQ = false>
 __device__ __forceinline__ void sq_tile_load_clamped(uint8_t *smem, const uint8_t *__restrict__ src, int H, int W,
                                                      int r_rt, int y0, int x0)
 {
@@ -528,10 +557,11 @@ __device__ __forceinline__ void sq_tile_load_clamped(uint8_t *smem, const uint8_
   const int r = RC > 0 ? RC : r_rt;
   const int nchunk = (FT + 2 * r) * cpr;
   const int gx0 = x0 - R16;
+  const int tid = tile_tid<OPQ>();
   uint4 v[MAXC];
 #pragma unroll
   for (int i = 0; i < MAXC; i++) {
-    const int c = threadIdx.x + i * FTHREADS;
+    const int c = tid + i * FTHREADS;
     v[i] = make_uint4(0u, 0u, 0u, 0u);
     if (c < nchunk) {
       const int ri = c / cpr, cc = c - ri * cpr;
@@ -541,7 +571,7 @@ __device__ __forceinline__ void sq_tile_load_clamped(uint8_t *smem, const uint8_
   }
 #pragma unroll
   for (int i = 0; i < MAXC; i++) {
-    const int c = threadIdx.x + i * FTHREADS;
+    const int c = tid + i * FTHREADS;
     if (c < nchunk) {
       const int ri = c / cpr, cc = c - ri * cpr;
       *reinterpret_cast<uint4 *>(smem + ri * AW + (cc << 4)) = v[i];
@@ -561,7 +591,7 @@ inline size_t sq_grad_lds_bytes(int r, int TW)
 // the same LDS words, the two transposed results go to T_max (reusing the region's
 // storage) and T_min (after it), and the horizontal pass subtracts min from max (bytewise,
 // with no borrow since max >= min in every byte).
-template <int K, bool ALIGNED, int RC, int TW>
+template <int K, bool ALIGNED, int RC, int TW, bool OPQ = false>
 __device__ __forceinline__ void sq_tile_grad_passes(uint8_t *smem, uint8_t *__restrict__ dst_img, int H, int W, int r_rt,
                                                     int y0, int x0)
 {
@@ -572,7 +602,7 @@ __device__ __forceinline__ void sq_tile_grad_passes(uint8_t *smem, uint8_t *__re
   uint8_t *TM = smem;
   const size_t a_bytes = (size_t)(FT + 2 * r) * AW, t_bytes = (size_t)AW * FT;
   uint8_t *Tm = smem + (a_bytes > t_bytes ? a_bytes : t_bytes);
-  const int tid = threadIdx.x;
+  const int tid = tile_tid<OPQ>();
   {
     constexpr int MAXT = sq_maxt(K, TW);
     const int q0 = (R16 - r) >> 2, q1 = (R16 + TW + r + 3) >> 2;

@@ -334,6 +334,65 @@ def test_gradient_u8_fused_unaligned_view():
     assert rest[0] == 5 and (rest[1 + 70 * 61:] == 5).all()
 
 
+def _sq_oracle(img, r, dil):
+    return oracle.static_iter(oracle.line_x(img, r, dil), "SESegmentY2D", min(r, img.shape[0]), dil)
+
+
+@pytest.mark.parametrize("shape", [(256, 272), (130, 48), (517, 1008), (77, 1001), (3, 5), (129, 130),
+                                   (300, 16), (20, 33), (700, 520), (125, 257), (16, 4096)])
+@pytest.mark.parametrize("r", [1, 2, 3, 4, 5, 7, 8, 9, 12, 15, 16])
+def test_open_close_u8_fused(shape, r):
+    """uint8 opening / closing by a square (one fused launch for r <= 7: the band of the
+    first operator stays in LDS) == the two-operator composition of the oracle."""
+    img = rand(shape, np.uint8, 100 + r)
+    e, d = _sq_oracle(img, r, False), _sq_oracle(img, r, True)
+    np.testing.assert_array_equal(h(pm.opening(g(img), pm.static_nl_se.square2d, r)), _sq_oracle(e, r, True))
+    np.testing.assert_array_equal(h(pm.closing(g(img), pm.static_nl_se.square2d, r)), _sq_oracle(d, r, False))
+
+
+@pytest.mark.parametrize("ioff,ooff", [(3, 1), (0, 2), (4, 0)])
+def test_open_close_u8_fused_unaligned_view(ioff, ooff):
+    H, W = 150, 301
+    base = torch.from_numpy(rand((H * W + 8,), np.uint8, 110)).to(DEV)
+    view = base[ioff:ioff + H * W].view(H, W)
+    img = view.cpu().numpy()
+    for opening in (True, False):
+        obuf = torch.full((H * W + 8,), 5, dtype=torch.uint8, device=DEV)
+        oview = obuf[ooff:ooff + H * W].view(H, W)
+        (pm.opening if opening else pm.closing)(view, pm.static_nl_se.square2d, 3, out=oview)
+        first = _sq_oracle(img, 3, not opening)
+        np.testing.assert_array_equal(h(oview), _sq_oracle(first, 3, opening))
+        rest = obuf.cpu().numpy()
+        assert (rest[:ooff] == 5).all() and (rest[ooff + H * W:] == 5).all()
+
+
+@pytest.mark.parametrize("dtype", [np.uint16, np.int32, np.int64, np.float32, np.float64])
+@pytest.mark.parametrize("shape", [(64, 64), (65, 70), (200, 130), (3, 5), (130, 257), (1, 300), (300, 1)])
+@pytest.mark.parametrize("r", [1, 2, 3, 4, 7, 8, 15, 16, 17, 40])
+def test_square_generic_fused(dtype, shape, r):
+    """uint16 / int32 / int64 / float32 / float64 square (one fused launch per radius up to 16,
+    chained launches above) == the oracle's x line pass then r y segments."""
+    img = rand(shape, dtype, 120 + r)
+    for dil in (True, False):
+        f = pm.dilate if dil else pm.erode
+        np.testing.assert_array_equal(h(f(g(img), pm.static_nl_se.square2d, r)), _sq_oracle(img, r, dil))
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.float64])
+def test_square_generic_fused_views(dtype):
+    """the fused generic kernel on views at element offsets (no alignment assumed)"""
+    H, W = 70, 93
+    base = torch.from_numpy(rand((H * W + 5,), dtype, 130)).to(DEV)
+    view = base[3:3 + H * W].view(H, W)
+    img = view.cpu().numpy()
+    obuf = torch.zeros((H * W + 5,), dtype=base.dtype, device=DEV)
+    oview = obuf[1:1 + H * W].view(H, W)
+    pm.dilate(view, pm.static_nl_se.square2d, 5, out=oview)
+    np.testing.assert_array_equal(h(oview), _sq_oracle(img, 5, True))
+    rest = obuf.cpu().numpy()
+    assert rest[0] == 0 and (rest[1 + H * W:] == 0).all()
+
+
 @pytest.mark.parametrize("it", [1, 2, 5])
 def test_square3d_u8_fused(it):
     img = rand((6, 40, 48), np.uint8, 70)
