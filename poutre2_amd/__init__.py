@@ -504,6 +504,8 @@ def autotune(cache=None):
     import os
     torch = _torch()
     name = torch.cuda.get_device_properties(torch.cuda.current_device()).name
+    if cache is not None:
+        cache = os.path.expanduser(cache)
     if cache is not None and os.path.exists(cache):
         with open(cache) as f:
             data = json.load(f)

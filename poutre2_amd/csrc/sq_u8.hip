@@ -4,8 +4,8 @@
 // uint8 (ero_dil_static.cpp:229-305, ero_dil_line_se_t.hpp:49-280): one launch reads
 // every input pixel once from HBM and writes every output once.
 //
-// One workgroup (256 threads) owns an output tile of 128 rows and TW = 256 (r <= 16) or
-// 128 columns.
+// One workgroup (256 threads) owns an output tile of 128 rows and TW = 128 or 256 columns
+// (per device and radius: the table pm_autotune fills, else the rule sq_tw).
 //  1. The input region (128+2r rows, TW+2*R16 columns, R16 = 16 for r <= 15 and 32 above,
 //     so that global loads of aligned rows are 16-byte aligned) is copied to LDS with 128-bit
 //     loads (dword-aligned loads realigned in registers when rows are not); pixels
