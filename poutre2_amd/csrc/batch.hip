@@ -3,11 +3,13 @@
 // pm_batch_dilate_open computes, for every image i of a batch,
 //   dil[i]  = Dilate(in[i], SESquare2D, r_i)                      (ero_dil_static.cpp:268-305)
 //   open[i] = Dilate(Erode(in[i], SESquare2D, r_i), SESquare2D, r_i)
-// The images are grouped by radius.  Per group, one persistent launch walks a work list
-// spanning all its images with two items per tile (the tile's dilation, and its erosion
-// into a scratch image), and a second launch dilates the eroded tiles; stream order
-// separates the two.  Every item runs the tile code of the single-image fused kernel
-// (sq_u8.hip) specialised for the group's radius, with loads and stores for any width.
+// The images are grouped by radius.  Per group up to r = SQ_U8_OPEN_RMAX, one persistent
+// launch walks a work list spanning all its images: the dilation tiles, then the fused
+// opening tiles (sq_open.hpp: erosion and dilation in LDS, nothing in between goes through
+// memory).  Larger radii take two launches: items for the dilation and for the erosion into
+// a scratch image, then the dilation of the eroded tiles (stream order separates them).
+// Every item runs the tile code of the single-image fused kernels specialised for the
+// group's radius, with loads and stores for any width and alignment.
 // The reference runs the two Erode/Dilate calls per image, one image at a time.
 #include <vector>
 
@@ -31,6 +33,8 @@ struct BatchLayout {
   static constexpr int NG = SQ_TILE_RMAX;  // one group per radius 1..32
   std::vector<BatchImage> grp[NG];
   int64_t tiles[NG] = {};
+  int64_t otiles[NG] = {};  // fused-opening tiles (groups r <= SQ_U8_OPEN_RMAX)
+  int64_t pixels = 0;
   int64_t tmp_bytes = 0;
   size_t first[NG] = {};
   std::vector<BatchImage> all;
@@ -61,9 +65,18 @@ static int batch_layout(int n, const pm_image *ins, const int *radii, pm_image *
     b.r = radii[i];
     b.tiles_x = (int)((W + TW - 1) / TW);
     b.tile0 = (int)L.tiles[g];
+    b.otile0 = 0;
     L.tiles[g] += (int64_t)b.tiles_x * ((H + FT - 1) / FT);
-    if (L.tiles[g] > INT32_MAX / 4) PM_FAIL(PM_E_SIZE, "batch: too many tiles");
-    L.tmp_bytes += (H * W + 255) & ~(int64_t)255;
+    if (radii[i] <= SQ_U8_OPEN_RMAX) {
+      // fused opening: tiles of open_fo(r) rows, same columns as the dilation tiles
+      const int fo = open_fo(radii[i]);
+      b.otile0 = (int)L.otiles[g];
+      L.otiles[g] += (int64_t)b.tiles_x * ((H + fo - 1) / fo);
+    } else {
+      L.tmp_bytes += (H * W + 255) & ~(int64_t)255;
+    }
+    if (L.tiles[g] + L.otiles[g] > INT32_MAX / 4) PM_FAIL(PM_E_SIZE, "batch: too many tiles");
+    L.pixels += H * W;
     L.grp[g].push_back(b);
   }
   return PM_OK;
@@ -83,6 +96,7 @@ static BatchImage *batch_bind(BatchLayout &L, uint8_t *base)
   int64_t off = 0;
   for (int g = 0; g < BatchLayout::NG; g++)
     for (auto &b : L.grp[g]) {
+      if (b.r <= SQ_U8_OPEN_RMAX) continue;  // fused opening: no eroded image
       b.tmp = base + off;
       off += ((int64_t)b.H * b.W + 255) & ~(int64_t)255;
     }
@@ -94,15 +108,30 @@ static BatchImage *batch_bind(BatchLayout &L, uint8_t *base)
   return (BatchImage *)(base + L.tmp_bytes);
 }
 
-// enqueue the two phases of every radius group on s (kernel launches only, so that the
-// sequence can be captured into a graph)
+// enqueue every radius group on s (kernel launches only, so that the sequence can be
+// captured into a graph): one fused launch for r <= SQ_U8_OPEN_RMAX, two phases above
 static int batch_enqueue(const BatchLayout &L, const BatchImage *dimg, hipStream_t s)
 {
   int dev = 0, cus = 0;
   PM_HIP(hipGetDevice(&dev));
   PM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  for (int g = 0; g < BatchLayout::NG && g < SQ_U8_OPEN_RMAX; g++) {
+    if (L.grp[g].empty()) continue;
+    const int r = g + 1;
+    const batch_fused_t k = batch_fused_for(r);
+    const size_t l1 = sq_tile_lds_bytes(r, sq_tw(sq_k_of(r), r)), l2 = sq_open_lds_bytes(r);
+    const size_t lds = l1 > l2 ? l1 : l2;
+    int occ = 0;
+    PM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, FTHREADS, lds));
+    const int64_t items = L.tiles[g] + L.otiles[g];
+    int64_t G = (int64_t)(occ > 0 ? occ : 1) * cus;
+    if (G > items) G = items;
+    hipLaunchKernelGGL(k, dim3((unsigned)G), dim3(FTHREADS), lds, s, dimg + L.first[g], (int)L.grp[g].size(),
+                       (int)L.tiles[g], (int)items);
+    PM_HIP(hipGetLastError());
+  }
   for (int phase = 0; phase < 2; phase++)
-    for (int g = 0; g < BatchLayout::NG; g++) {
+    for (int g = SQ_U8_OPEN_RMAX; g < BatchLayout::NG; g++) {
       if (L.grp[g].empty()) continue;
       const int r = g + 1;
       const batch_kernel_t k = batch_kernel(r, phase);
@@ -129,7 +158,7 @@ extern "C" int pm_batch_dilate_open(int n, const pm_image *ins, const int *radii
   clear_error();
   BatchLayout L;
   PM_TRY(batch_layout(n, ins, radii, dil_outs, open_outs, L));
-  if (L.tmp_bytes == 0) return PM_OK;
+  if (L.pixels == 0) return PM_OK;
   hipStream_t s = as_stream(stream);
   Scratch sc;
   PM_TRY(sc.alloc(batch_scratch_bytes(L), s));

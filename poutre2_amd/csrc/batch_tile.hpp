@@ -3,22 +3,27 @@
 // radius range in batch_r*.hip so that the instantiations compile in parallel.
 #pragma once
 #include "pm_common.hpp"
+#include "sq_open.hpp"
 #include "sq_u8_tile.hpp"
 
 namespace pm {
 
+// One image of a batch.  tile0: index of its first dilation tile (128 x TW) in its radius
+// group's work list; otile0: index of its first fused-opening tile (open_fo(r) x OPEN_TW,
+// radius groups up to SQ_U8_OPEN_RMAX; 0 otherwise); tmp: its eroded image (two-phase groups).
 struct BatchImage {
   const uint8_t *in;
   uint8_t *dil, *open, *tmp;
-  int H, W, r, tiles_x, tile0;
+  int H, W, r, tiles_x, tile0, otile0;
 };
 
-__device__ __forceinline__ int batch_find(const BatchImage *im, int n, int t)
+// the image whose tile range (by tile0, or by otile0 when OPEN) holds tile t
+template <bool OPEN = false> __device__ __forceinline__ int batch_find(const BatchImage *im, int n, int t)
 {
   int lo = 0, hi = n - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (im[mid].tile0 <= t) lo = mid;
+    if ((OPEN ? im[mid].otile0 : im[mid].tile0) <= t) lo = mid;
     else hi = mid - 1;
   }
   return lo;
@@ -56,7 +61,38 @@ __global__ __launch_bounds__(FTHREADS) void batch_sq_kernel(const BatchImage *__
   }
 }
 
+// Radius groups up to SQ_U8_OPEN_RMAX: one launch.  Items [0, ndil) are the dilation tiles,
+// items [ndil, nitems) the fused-opening tiles (sq_open.hpp) of the same images, so no
+// eroded image goes through memory and no second launch is needed.
+template <int R>
+__global__ __launch_bounds__(FTHREADS) void batch_fused_kernel(const BatchImage *__restrict__ im, int nimg, int ndil,
+                                                               int nitems)
+{
+  constexpr int K = sq_k_of(R), TW = sq_tw(K, R), FO = open_fo(R);
+  static_assert(R <= SQ_U8_OPEN_RMAX && TW == OPEN_TW, "fused batch geometry");
+  extern __shared__ __align__(16) uint8_t smem[];
+  for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
+    if (item < ndil) {
+      const BatchImage b = im[batch_find(im, nimg, item)];
+      const int tl = item - b.tile0, ty = tl / b.tiles_x, tx = tl - ty * b.tiles_x;
+      batch_tile<R, true>(smem, b.in, b.dil, b, ty * FT, tx * TW);
+    } else {
+      const int t = item - ndil;
+      const BatchImage b = im[batch_find<true>(im, nimg, t)];
+      const int tl = t - b.otile0, ty = tl / b.tiles_x, tx = tl - ty * b.tiles_x;
+      const int y0 = ty * FO, x0 = tx * OPEN_TW;
+      sq_tile_load<K, false, false, R, OPEN_TW, open_pad(R)>(smem, b.in, b.H, b.W, R, y0 - R, x0);
+      __syncthreads();
+      sq_open_tile<K, true, false, R, OPEN_TW>(smem, b.open, b.H, b.W, y0, x0);
+      __syncthreads();  // LDS is rewritten by the next item
+    }
+  }
+}
+
 typedef void (*batch_kernel_t)(const BatchImage *, int, int);
+typedef void (*batch_fused_t)(const BatchImage *, int, int, int);
+// fused kernel of radius r in [1, SQ_U8_OPEN_RMAX] (batch_open.hip)
+batch_fused_t batch_fused_for(int r);
 
 // kernel of radius r in [LO, LO + 7] and phase 0 / 1 (batch_r*.hip)
 template <int LO> batch_kernel_t batch_kernel_range(int r, int phase);

@@ -28,7 +28,8 @@ def _square(img, r, dil):
     return oracle.static_iter(oracle.line_x(img, r, dil), "SESegmentY2D", min(r, img.shape[0]), dil)
 
 
-@pytest.mark.parametrize("seed,n,hmax,rmax", [(1, 12, 300, 15), (2, 5, 40, 32), (3, 1, 129, 7), (4, 30, 70, 3)])
+@pytest.mark.parametrize("seed,n,hmax,rmax", [(1, 12, 300, 15), (2, 5, 40, 32), (3, 1, 129, 7), (4, 30, 70, 3),
+                                            (5, 8, 700, 7)])
 def test_batch_dilate_open_matches_oracle(seed, n, hmax, rmax):
     imgs, radii = _batch(n, seed, hmax, rmax)
     dils, opens = pm.batch_dilate_open([torch.from_numpy(a).to(DEV) for a in imgs], radii)
