@@ -179,7 +179,6 @@ __device__ __forceinline__ void sq_open_tile(uint8_t *smem, uint8_t *__restrict_
   // stored with lanes on consecutive 4-byte quads of one row
   {
     constexpr int NQ = TW / 4, NRUN = FO / K, ntask = NQ * NRUN;
-    static_assert(NQ % 64 == 0, "a wave covers whole rows (the unaligned store shifts words across lanes)");
     const bool full = y0 + FO <= H && x0 + TW <= W;
     for (int task = tid; task < ntask; task += FTHREADS) {
       const int k = task / NQ, w = task - k * NQ;
@@ -196,19 +195,12 @@ __device__ __forceinline__ void sq_open_tile(uint8_t *smem, uint8_t *__restrict_
         uint8_t *d = d0 + (int64_t)t * W;
         if (ALIGNED) {
           if (full || (y0 + k * K + t < H && gx < W)) *reinterpret_cast<uint32_t *>(d) = v;
-        } else {
-          // any alignment: every lane of the wave takes part in the lane shift
-          const uint32_t wv[1] = {v};
-          const uint32_t prev = __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, false);
-          if (y0 + k * K + t < H) {
-            if (gx + 4 <= W) {
-              store_run_shared<1>(d, wv, prev, w == 0, w == NQ - 1 || gx + 8 > W);
-            } else {
+        } else if (full || (y0 + k * K + t < H && gx + 4 <= W)) {
+          store4_any(d, v);  // any alignment: one dword store at the byte address
+        } else if (y0 + k * K + t < H) {
 #pragma unroll
-              for (int bb = 0; bb < 4; bb++)
-                if (gx + bb < W) d[bb] = (uint8_t)(v >> (8 * bb));
-            }
-          }
+          for (int bb = 0; bb < 4; bb++)
+            if (gx + bb < W) d[bb] = (uint8_t)(v >> (8 * bb));
         }
       }
     }

@@ -8,7 +8,7 @@
 // (per device and radius: the table pm_autotune fills, else the rule sq_tw).
 //  1. The input region (128+2r rows, TW+2*R16 columns, R16 = 16 for r <= 15 and 32 above,
 //     so that global loads of aligned rows are 16-byte aligned) is copied to LDS with 128-bit
-//     loads (dword-aligned loads realigned in registers when rows are not); pixels
+//     loads (at byte addresses when rows are not 16-byte aligned); pixels
 //     outside the image get the neutral value (0 for dilation, 255 for erosion).
 //  2. Vertical pass: a thread owns a 4-column quad and a run of K output rows.  It reads
 //     K+2r words down the column, splits every word into two u16x2 whose high bytes hold

@@ -60,23 +60,25 @@ template <bool OPQ> __device__ __forceinline__ int tile_tid()
 
 __device__ __forceinline__ uint32_t lds_word(const uint8_t *p) { return *reinterpret_cast<const uint32_t *>(p); }
 
-// Four dwords at a 4-byte aligned address: one global_load_dwordx4 (multi-dword accesses
-// need dword alignment only).
-typedef unsigned int u32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
+// 16 and 4 bytes at any byte address.  gfx950 under ROCm runs with unaligned memory
+// access enabled (the compiler emits these as single global_load/store_dwordx4 / _dword
+// instructions at the unaligned address); each access reads or writes exactly its own
+// bytes, so no neighbouring byte of another tile or row is touched.
+typedef unsigned int u32x4_a1 __attribute__((ext_vector_type(4), aligned(1)));
+typedef unsigned int u32_a1 __attribute__((aligned(1)));
 
-// 16 bytes at any address p, all of them inside the image.  The dwords are read from
-// p rounded down to 4 (the fifth dword only when p is not 4-aligned: it then holds at
-// least one of the 16 bytes, so every dword read contains an image byte and lies inside
-// the image's allocation) and realigned with v_alignbyte_b32.
+// 16 bytes at any address p, all of them inside the image
 __device__ __forceinline__ uint4 load16_any(const uint8_t *p)
 {
-  const uint32_t sh = (uint32_t)((uintptr_t)p & 3);
-  const uint32_t *a = reinterpret_cast<const uint32_t *>(p - sh);
-  const u32x4_a4 v = *reinterpret_cast<const u32x4_a4 *>(a);
-  const uint32_t v4 = sh ? a[4] : 0u;
-  return make_uint4(__builtin_amdgcn_alignbyte(v.y, v.x, sh), __builtin_amdgcn_alignbyte(v.z, v.y, sh),
-                    __builtin_amdgcn_alignbyte(v.w, v.z, sh), __builtin_amdgcn_alignbyte(v4, v.w, sh));
+  const u32x4_a1 v = *reinterpret_cast<const u32x4_a1 *>(p);
+  return make_uint4(v.x, v.y, v.z, v.w);
 }
+__device__ __forceinline__ void store16_any(uint8_t *p, uint32_t a, uint32_t b, uint32_t c, uint32_t d)
+{
+  const u32x4_a1 v = {a, b, c, d};
+  *reinterpret_cast<u32x4_a1 *>(p) = v;
+}
+__device__ __forceinline__ void store4_any(uint8_t *p, uint32_t a) { *reinterpret_cast<u32_a1 *>(p) = a; }
 
 // Shift the 16 bytes of (v0..v3) by b bytes towards higher (LEFT = true: out byte j =
 // in byte j - b) or lower (out byte j = in byte j + b) positions, 0 <= b <= 15, filling
@@ -156,61 +158,6 @@ constexpr int sq_maxc(int K, int TW)
 {
   const int r = sq_rmax_of(K), R16 = K == 32 ? 32 : 16;
   return ((FT + 2 * r) * ((TW + 2 * R16) >> 4) + FTHREADS - 1) / FTHREADS;
-}
-
-// Store the 4*NW bytes w of one run at any address dst, as dwords at 4-aligned addresses.
-// With s = dst & 3 != 0, the s bytes before dst end the previous lane's run (whose last
-// word is prev, fetched by a DPP lane shift) and are written by this lane together with its
-// own first 4-s bytes, so that a lane issues one multi-dword store per run.  A run that
-// starts a row segment (first) writes its first 4-s bytes with sub-dword stores instead, and
-// a run that no following run overlaps (tail) also writes its own last s bytes.  Every byte
-// is written once; sub-dword stores touch only their own bytes.
-// s is the same for every lane of the wave (rows 4*q2 + c of one tile differ by multiples
-// of 4*W bytes and runs by multiples of 4), so it is read once into a scalar register and
-// its cases are uniform branches.
-template <int NW>
-__device__ __forceinline__ void store_run_shared(uint8_t *dst, const uint32_t (&w)[NW], uint32_t prev, bool first,
-                                                 bool tail)
-{
-  const uint32_t s = __builtin_amdgcn_readfirstlane((uint32_t)((uintptr_t)dst & 3));
-  if (s == 0) {
-#pragma unroll
-    for (int m = 0; m < NW; m++) reinterpret_cast<uint32_t *>(dst)[m] = w[m];
-    return;
-  }
-  const uint32_t hs = 4 - s;
-  uint32_t *a = reinterpret_cast<uint32_t *>(dst - s);  // 4-aligned
-  uint32_t o[NW];
-  o[0] = __builtin_amdgcn_alignbyte(w[0], prev, hs);
-#pragma unroll
-  for (int m = 1; m < NW; m++) o[m] = __builtin_amdgcn_alignbyte(w[m], w[m - 1], hs);
-  if (!first) {
-#pragma unroll
-    for (int m = 0; m < NW; m++) a[m] = o[m];
-  } else {
-    if (s == 1) {
-      dst[0] = (uint8_t)w[0];
-      *reinterpret_cast<uint16_t *>(dst + 1) = (uint16_t)(w[0] >> 8);
-    } else if (s == 2) {
-      *reinterpret_cast<uint16_t *>(dst) = (uint16_t)w[0];
-    } else {
-      dst[0] = (uint8_t)w[0];
-    }
-#pragma unroll
-    for (int m = 1; m < NW; m++) a[m] = o[m];
-  }
-  if (tail) {
-    uint8_t *t = reinterpret_cast<uint8_t *>(a + NW);
-    const uint32_t tw = w[NW - 1] >> (8 * hs);  // the last s bytes
-    if (s == 1) {
-      t[0] = (uint8_t)tw;
-    } else if (s == 2) {
-      *reinterpret_cast<uint16_t *>(t) = (uint16_t)tw;
-    } else {
-      *reinterpret_cast<uint16_t *>(t) = (uint16_t)tw;
-      t[2] = (uint8_t)(tw >> 16);
-    }
-  }
 }
 
 // Windowed max/min of 2r+1 rows for K consecutive output rows of one 4-byte column quad.
@@ -318,45 +265,44 @@ __device__ __forceinline__ void sq_pass3_rows(const uint8_t *T, int r, int c0, i
   }
 }
 
-// Store rows[c] (K bytes of tile row 4*q2 + c at output column k2*K) to the image.  Must be
-// called with every lane of the wave active (the unaligned path shifts words across lanes).
+// Store rows[c] (K bytes of tile row 4*q2 + c at output column k2*K) to the image.
+// ALIGNED: W % 16 == 0 and dst 16-byte aligned; otherwise the same stores at byte addresses.
 template <int K, bool ALIGNED, int TW>
 __device__ __forceinline__ void sq_store_rows(const uint32_t (&rows)[4][K / 4], uint8_t *__restrict__ dst_img, int H,
                                               int W, int y0, int x0, int q2, int k2, bool full)
 {
-  constexpr int NR = TW / K;
   const int gx = x0 + k2 * K;
   uint8_t *d0 = dst_img + (int64_t)(y0 + 4 * q2) * W + gx;
-  // rows of any alignment: the last word of the previous lane's run of each row (lanes
-  // k2 - 1 and k2 are neighbouring runs of one row segment)
-  uint32_t prevw[4] = {0u, 0u, 0u, 0u};
-  if (!ALIGNED) {
+  auto run = [&](uint8_t *dst, int c) {
+    if (K >= 16) {
 #pragma unroll
-    for (int c = 0; c < 4; c++) prevw[c] = __builtin_amdgcn_mov_dpp(rows[c][K / 4 - 1], 0x138, 0xf, 0xf, false);
-  }
-  if (ALIGNED && full) {
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-      uint8_t *dst = d0 + (int64_t)c * W;
-      if (K >= 16) {
-#pragma unroll
-        for (int hh = 0; hh < K / 16; hh++)
+      for (int hh = 0; hh < K / 16; hh++) {
+        if (ALIGNED)
           *reinterpret_cast<uint4 *>(dst + 16 * hh) =
               make_uint4(rows[c][4 * hh], rows[c][4 * hh + 1], rows[c][4 * hh + 2], rows[c][4 * hh + 3]);
-      } else {
+        else
+          store16_any(dst + 16 * hh, rows[c][4 * hh], rows[c][4 * hh + 1], rows[c][4 * hh + 2], rows[c][4 * hh + 3]);
+      }
+    } else {
 #pragma unroll
-        for (int g = 0; g < K / 4; g++) reinterpret_cast<uint32_t *>(dst)[g] = rows[c][g];
+      for (int g = 0; g < K / 4; g++) {
+        if (ALIGNED) reinterpret_cast<uint32_t *>(dst)[g] = rows[c][g];
+        else store4_any(dst + 4 * g, rows[c][g]);
       }
     }
+  };
+  if (full) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) run(d0 + (int64_t)c * W, c);
     return;
   }
-  // W is a multiple of 16 when ALIGNED: a 16-byte (or 4-byte) chunk starting inside the row is whole
 #pragma unroll
   for (int c = 0; c < 4; c++) {
     const int gy = y0 + 4 * q2 + c;
     if (gy >= H) continue;
     uint8_t *dst = d0 + (int64_t)c * W;
     if (ALIGNED) {
+      // W is a multiple of 16: a 16-byte (or 4-byte) chunk starting inside the row is whole
       if (K >= 16) {
 #pragma unroll
         for (int hh = 0; hh < K / 16; hh++)
@@ -369,7 +315,7 @@ __device__ __forceinline__ void sq_store_rows(const uint32_t (&rows)[4][K / 4], 
           if (gx + 4 * g < W) *reinterpret_cast<uint32_t *>(dst + 4 * g) = rows[c][g];
       }
     } else if (gx + K <= W) {
-      store_run_shared<K / 4>(dst, rows[c], prevw[c], k2 == 0, k2 == NR - 1 || gx + 2 * K > W);
+      run(dst, c);
     } else {
       // the run crosses the right edge: byte stores, clipped
 #pragma unroll
