@@ -186,12 +186,12 @@ int pm_batch_plan_create(int n, const pm_image *ins, const int *radii, pm_image 
 int pm_batch_plan_launch(pm_batch_plan *plan, void *stream);
 int pm_batch_plan_destroy(pm_batch_plan *plan);
 
-/* ---- autotuner (no reference counterpart).  The fused uint8 square kernel has two tile
- * widths (128 and 256 columns) per radius; pm_autotune times both for every radius 1..32 on a
- * 8192 x 8192 image with 16-byte aligned rows and on an 8192 x 8189 one with unaligned rows
+/* ---- autotuner (no reference counterpart).  The fused uint8 square kernel has three tile
+ * widths (128, 192 and 256 columns) per radius; pm_autotune times each for every radius 1..32
+ * on 8192 x 8192 images with 16-byte aligned rows and on 8192 x 8189 ones with unaligned rows
  * (device memory allocated and freed by the call; the stream is synchronised) and keeps the
- * faster in a per-device table used by every later call.  pm_tuning_get returns the width in
- * use for (aligned != 0, r); pm_tuning_set sets it (128, 256, or 0 = built-in rule). */
+ * fastest in a per-device table used by every later call.  pm_tuning_get returns the width in
+ * use for (aligned != 0, r); pm_tuning_set sets it (128, 192, 256, or 0 = built-in rule). */
 int pm_autotune(void *stream);
 int pm_tuning_get(int aligned, int r);
 int pm_tuning_set(int aligned, int r, int tile_width);

@@ -4,7 +4,7 @@
 // uint8 (ero_dil_static.cpp:229-305, ero_dil_line_se_t.hpp:49-280): one launch reads
 // every input pixel once from HBM and writes every output once.
 //
-// One workgroup (256 threads) owns an output tile of 128 rows and TW = 128 or 256 columns
+// One workgroup (256 threads) owns an output tile of 128 rows and TW = 128, 192 or 256 columns
 // (per device and radius: the table pm_autotune fills, else the rule sq_tw).
 //  1. The input region (128+2r rows, TW+2*R16 columns, R16 = 16 for r <= 15 and 32 above,
 //     so that global loads of aligned rows are 16-byte aligned) is copied to LDS with 128-bit
@@ -39,7 +39,7 @@ namespace pm {
 
 // ---------------------------------------------------------------- tile-width table
 // Tile width per device, alignment class and radius: 0 = not tuned (the built-in rule
-// sq_tw), otherwise 128 or 256 as chosen by pm_autotune (or set by pm_tuning_set).
+// sq_tw), otherwise 128, 192 or 256 as chosen by pm_autotune (or set by pm_tuning_set).
 static std::atomic<int> g_tw[PM_MAX_DEVICES][2][SQ_U8_RMAX + 1];
 
 static int tuned_tw(int dev, bool aligned, int r)
@@ -162,7 +162,7 @@ extern "C" int pm_autotune(void *stream)
     for (int r = 1; r <= (int)SQ_U8_RMAX && rc == PM_OK; r++) {
       float best = 1e30f;
       int best_tw = 0;
-      for (int tw : {128, 256}) {
+      for (int tw : {128, 192, 256}) {
         rc = one(tw, r);  // warm-up
         if (rc != PM_OK) break;
         if (hipEventRecord(e0, s) != hipSuccess) { rc = PM_E_HIP; break; }
@@ -204,8 +204,8 @@ extern "C" int pm_tuning_set(int aligned, int r, int tile_width)
 {
   clear_error();
   if (r < 1 || r > (int)SQ_U8_RMAX) PM_FAIL(PM_E_ARG, "tuning: radius %d outside [1, %d]", r, (int)SQ_U8_RMAX);
-  if (tile_width != 0 && tile_width != 128 && tile_width != 256)
-    PM_FAIL(PM_E_ARG, "tuning: tile width must be 128, 256 or 0 (built-in rule)");
+  if (tile_width != 0 && tile_width != 128 && tile_width != 192 && tile_width != 256)
+    PM_FAIL(PM_E_ARG, "tuning: tile width must be 128, 192, 256 or 0 (built-in rule)");
   int dev = 0;
   PM_HIP(hipGetDevice(&dev));
   if (dev < 0 || dev >= PM_MAX_DEVICES) PM_FAIL(PM_E_HIP, "device %d outside the tuning table", dev);

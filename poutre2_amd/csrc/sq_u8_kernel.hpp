@@ -46,7 +46,7 @@ __global__ __launch_bounds__(FTHREADS) void sq_u8_kernel(const uint8_t *__restri
 
 typedef void (*sq_kernel_t)(const uint8_t *, uint8_t *, int, int, int, int, int, int);
 
-// Kernel of radius r (1..32) and tile width tw (128 or 256), specialised on both, for
+// Kernel of radius r (1..32) and tile width tw (128, 192 or 256), specialised on both, for
 // 16-byte aligned rows (ALIGNED) or rows of any alignment (sq_u8_k*.hip).
 template <bool DIL, bool ALIGNED> sq_kernel_t sq_kernel_for(int r, int tw);
 template <> sq_kernel_t sq_kernel_for<true, true>(int r, int tw);

@@ -20,7 +20,7 @@ def test_tuning_set_get_and_widths_are_bit_exact():
     img = np.random.default_rng(3).integers(0, 256, size=(300, 517), dtype=np.uint8)
     aimg = np.random.default_rng(4).integers(0, 256, size=(300, 512), dtype=np.uint8)
     try:
-        for w in (128, 256):
+        for w in (128, 192, 256):
             for r in (1, 5, 9, 20):
                 pm._call("pm_tuning_set", 1, r, w)
                 pm._call("pm_tuning_set", 0, r, w)
@@ -40,7 +40,7 @@ def test_autotune_table_and_cache(tmp_path):
     cache = tmp_path / "tune.json"
     t1 = pm.autotune(cache=str(cache))
     assert set(t1) == {"aligned", "any"}
-    assert all(len(v) == 32 and set(v) <= {128, 256} for v in t1.values())
+    assert all(len(v) == 32 and set(v) <= {128, 192, 256} for v in t1.values())
     assert cache.exists()
     for r in range(1, 33):  # reset, then reload from the cache
         pm._call("pm_tuning_set", 1, r, 0)
