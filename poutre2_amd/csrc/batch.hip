@@ -3,7 +3,7 @@
 // pm_batch_dilate_open computes, for every image i of a batch,
 //   dil[i]  = Dilate(in[i], SESquare2D, r_i)                      (ero_dil_static.cpp:268-305)
 //   open[i] = Dilate(Erode(in[i], SESquare2D, r_i), SESquare2D, r_i)
-// The images are grouped by radius.  Per group up to r = SQ_U8_OPEN_RMAX, one persistent
+// The images are grouped by radius.  Per group up to r = BATCH_FUSED_RMAX (15), one persistent
 // launch walks a work list spanning all its images: the dilation tiles, then the fused
 // opening tiles (sq_open.hpp: erosion and dilation in LDS, nothing in between goes through
 // memory).  Larger radii take two launches: items for the dilation and for the erosion into
@@ -33,7 +33,7 @@ struct BatchLayout {
   static constexpr int NG = SQ_TILE_RMAX;  // one group per radius 1..32
   std::vector<BatchImage> grp[NG];
   int64_t tiles[NG] = {};
-  int64_t otiles[NG] = {};  // fused-opening tiles (groups r <= SQ_U8_OPEN_RMAX)
+  int64_t otiles[NG] = {};  // fused-opening tiles (groups r <= BATCH_FUSED_RMAX)
   int64_t pixels = 0;
   int64_t tmp_bytes = 0;
   size_t first[NG] = {};
@@ -67,7 +67,7 @@ static int batch_layout(int n, const pm_image *ins, const int *radii, pm_image *
     b.tile0 = (int)L.tiles[g];
     b.otile0 = 0;
     L.tiles[g] += (int64_t)b.tiles_x * ((H + FT - 1) / FT);
-    if (radii[i] <= SQ_U8_OPEN_RMAX) {
+    if (radii[i] <= BATCH_FUSED_RMAX) {
       // fused opening: tiles of open_fo(r) rows, same columns as the dilation tiles
       const int fo = open_fo(radii[i]);
       b.otile0 = (int)L.otiles[g];
@@ -96,7 +96,7 @@ static BatchImage *batch_bind(BatchLayout &L, uint8_t *base)
   int64_t off = 0;
   for (int g = 0; g < BatchLayout::NG; g++)
     for (auto &b : L.grp[g]) {
-      if (b.r <= SQ_U8_OPEN_RMAX) continue;  // fused opening: no eroded image
+      if (b.r <= BATCH_FUSED_RMAX) continue;  // fused opening: no eroded image
       b.tmp = base + off;
       off += ((int64_t)b.H * b.W + 255) & ~(int64_t)255;
     }
@@ -109,13 +109,13 @@ static BatchImage *batch_bind(BatchLayout &L, uint8_t *base)
 }
 
 // enqueue every radius group on s (kernel launches only, so that the sequence can be
-// captured into a graph): one fused launch for r <= SQ_U8_OPEN_RMAX, two phases above
+// captured into a graph): one fused launch for r <= BATCH_FUSED_RMAX, two phases above
 static int batch_enqueue(const BatchLayout &L, const BatchImage *dimg, hipStream_t s)
 {
   int dev = 0, cus = 0;
   PM_HIP(hipGetDevice(&dev));
   PM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  for (int g = 0; g < BatchLayout::NG && g < SQ_U8_OPEN_RMAX; g++) {
+  for (int g = 0; g < BatchLayout::NG && g < BATCH_FUSED_RMAX; g++) {
     if (L.grp[g].empty()) continue;
     const int r = g + 1;
     const batch_fused_t k = batch_fused_for(r);
@@ -131,7 +131,7 @@ static int batch_enqueue(const BatchLayout &L, const BatchImage *dimg, hipStream
     PM_HIP(hipGetLastError());
   }
   for (int phase = 0; phase < 2; phase++)
-    for (int g = SQ_U8_OPEN_RMAX; g < BatchLayout::NG; g++) {
+    for (int g = BATCH_FUSED_RMAX; g < BatchLayout::NG; g++) {
       if (L.grp[g].empty()) continue;
       const int r = g + 1;
       const batch_kernel_t k = batch_kernel(r, phase);

@@ -1,4 +1,4 @@
-// batch_open.hip -- one-launch batched dilation + fused opening for radii 1..SQ_U8_OPEN_RMAX
+// batch_open.hip -- one-launch batched dilation + fused opening for radii 1..BATCH_FUSED_RMAX
 // (batch_tile.hpp).
 #include <utility>
 
@@ -12,6 +12,6 @@ template <int... RS> static batch_fused_t pick(int r, std::integer_sequence<int,
   return tab[r - 1];
 }
 
-batch_fused_t batch_fused_for(int r) { return pick(r, std::make_integer_sequence<int, (int)SQ_U8_OPEN_RMAX>{}); }
+batch_fused_t batch_fused_for(int r) { return pick(r, std::make_integer_sequence<int, BATCH_FUSED_RMAX>{}); }
 
 }  // namespace pm

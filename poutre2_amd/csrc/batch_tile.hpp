@@ -10,7 +10,7 @@ namespace pm {
 
 // One image of a batch.  tile0: index of its first dilation tile (128 x TW) in its radius
 // group's work list; otile0: index of its first fused-opening tile (open_fo(r) x OPEN_TW,
-// radius groups up to SQ_U8_OPEN_RMAX; 0 otherwise); tmp: its eroded image (two-phase groups).
+// radius groups up to BATCH_FUSED_RMAX; 0 otherwise); tmp: its eroded image (two-phase groups).
 struct BatchImage {
   const uint8_t *in;
   uint8_t *dil, *open, *tmp;
@@ -61,7 +61,13 @@ __global__ __launch_bounds__(FTHREADS) void batch_sq_kernel(const BatchImage *__
   }
 }
 
-// Radius groups up to SQ_U8_OPEN_RMAX: one launch.  Items [0, ndil) are the dilation tiles,
+// Radius groups up to this radius run the one-launch form (batch_fused_kernel).  The fused
+// opening tile is used up to r = 15 here although single images switch to the two-launch
+// composition above r = 7 (sq_open.hpp): in a batch the composition's second launch per
+// radius group costs more than the fused tile's larger halo.
+constexpr int BATCH_FUSED_RMAX = 15;
+
+// Radius groups up to BATCH_FUSED_RMAX: one launch.  Items [0, ndil) are the dilation tiles,
 // items [ndil, nitems) the fused-opening tiles (sq_open.hpp) of the same images, so no
 // eroded image goes through memory and no second launch is needed.
 template <int R>
@@ -69,7 +75,7 @@ __global__ __launch_bounds__(FTHREADS) void batch_fused_kernel(const BatchImage 
                                                                int nitems)
 {
   constexpr int K = sq_k_of(R), TW = sq_tw(K, R), FO = open_fo(R);
-  static_assert(R <= SQ_U8_OPEN_RMAX && TW == OPEN_TW, "fused batch geometry");
+  static_assert(R <= BATCH_FUSED_RMAX && TW == OPEN_TW, "fused batch geometry");
   extern __shared__ __align__(16) uint8_t smem[];
   for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
     if (item < ndil) {
@@ -91,7 +97,7 @@ __global__ __launch_bounds__(FTHREADS) void batch_fused_kernel(const BatchImage 
 
 typedef void (*batch_kernel_t)(const BatchImage *, int, int);
 typedef void (*batch_fused_t)(const BatchImage *, int, int, int);
-// fused kernel of radius r in [1, SQ_U8_OPEN_RMAX] (batch_open.hip)
+// fused kernel of radius r in [1, BATCH_FUSED_RMAX] (batch_open.hip)
 batch_fused_t batch_fused_for(int r);
 
 // kernel of radius r in [LO, LO + 7] and phase 0 / 1 (batch_r*.hip)

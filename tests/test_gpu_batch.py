@@ -29,7 +29,7 @@ def _square(img, r, dil):
 
 
 @pytest.mark.parametrize("seed,n,hmax,rmax", [(1, 12, 300, 15), (2, 5, 40, 32), (3, 1, 129, 7), (4, 30, 70, 3),
-                                            (5, 8, 700, 7)])
+                                            (5, 8, 700, 7), (6, 6, 700, 15)])
 def test_batch_dilate_open_matches_oracle(seed, n, hmax, rmax):
     imgs, radii = _batch(n, seed, hmax, rmax)
     dils, opens = pm.batch_dilate_open([torch.from_numpy(a).to(DEV) for a in imgs], radii)
