@@ -108,32 +108,35 @@ static BatchImage *batch_bind(BatchLayout &L, uint8_t *base)
   return (BatchImage *)(base + L.tmp_bytes);
 }
 
-// enqueue every radius group on s (kernel launches only, so that the sequence can be
-// captured into a graph): one fused launch for r <= BATCH_FUSED_RMAX, two phases above
-static int batch_enqueue(const BatchLayout &L, const BatchImage *dimg, hipStream_t s)
+// enqueue every radius group (kernel launches only, so that the sequence can be captured
+// into a graph): one fused launch for r <= BATCH_FUSED_RMAX, two phases above.  Group j goes
+// to stream ss[j % ns]; the groups are independent, so on several streams their launches
+// overlap (the tail of one fills with the work of another).
+static int batch_enqueue(const BatchLayout &L, const BatchImage *dimg, const hipStream_t *ss, int ns)
 {
   int dev = 0, cus = 0;
   PM_HIP(hipGetDevice(&dev));
   PM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  for (int g = 0; g < BatchLayout::NG && g < BATCH_FUSED_RMAX; g++) {
+  int j = 0;
+  for (int g = 0; g < BatchLayout::NG; g++) {
     if (L.grp[g].empty()) continue;
+    const hipStream_t s = ss[j++ % ns];
     const int r = g + 1;
-    const batch_fused_t k = batch_fused_for(r);
-    const size_t l1 = sq_tile_lds_bytes(r, sq_tw(sq_k_of(r), r)), l2 = sq_open_lds_bytes(r);
-    const size_t lds = l1 > l2 ? l1 : l2;
-    int occ = 0;
-    PM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, FTHREADS, lds));
-    const int64_t items = L.tiles[g] + L.otiles[g];
-    int64_t G = (int64_t)(occ > 0 ? occ : 1) * cus;
-    if (G > items) G = items;
-    hipLaunchKernelGGL(k, dim3((unsigned)G), dim3(FTHREADS), lds, s, dimg + L.first[g], (int)L.grp[g].size(),
-                       (int)L.tiles[g], (int)items);
-    PM_HIP(hipGetLastError());
-  }
-  for (int phase = 0; phase < 2; phase++)
-    for (int g = BATCH_FUSED_RMAX; g < BatchLayout::NG; g++) {
-      if (L.grp[g].empty()) continue;
-      const int r = g + 1;
+    if (r <= BATCH_FUSED_RMAX) {
+      const batch_fused_t k = batch_fused_for(r);
+      const size_t l1 = sq_tile_lds_bytes(r, sq_tw(sq_k_of(r), r)), l2 = sq_open_lds_bytes(r);
+      const size_t lds = l1 > l2 ? l1 : l2;
+      int occ = 0;
+      PM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, FTHREADS, lds));
+      const int64_t items = L.tiles[g] + L.otiles[g];
+      int64_t G = (int64_t)(occ > 0 ? occ : 1) * cus;
+      if (G > items) G = items;
+      hipLaunchKernelGGL(k, dim3((unsigned)G), dim3(FTHREADS), lds, s, dimg + L.first[g], (int)L.grp[g].size(),
+                         (int)L.tiles[g], (int)items);
+      PM_HIP(hipGetLastError());
+      continue;
+    }
+    for (int phase = 0; phase < 2; phase++) {
       const batch_kernel_t k = batch_kernel(r, phase);
       const size_t lds = sq_tile_lds_bytes(r, sq_tw(sq_k_of(r), r));
       int occ = 0;
@@ -145,6 +148,7 @@ static int batch_enqueue(const BatchLayout &L, const BatchImage *dimg, hipStream
                          (int)items);
       PM_HIP(hipGetLastError());
     }
+  }
   return PM_OK;
 }
 
@@ -164,7 +168,7 @@ extern "C" int pm_batch_dilate_open(int n, const pm_image *ins, const int *radii
   PM_TRY(sc.alloc(batch_scratch_bytes(L), s));
   BatchImage *dimg = batch_bind(L, (uint8_t *)sc.ptr);
   PM_HIP(hipMemcpyAsync(dimg, L.all.data(), L.all.size() * sizeof(BatchImage), hipMemcpyHostToDevice, s));
-  return batch_enqueue(L, dimg, s);
+  return batch_enqueue(L, dimg, &s, 1);
 }
 
 // A batch captured once into a graph: its scratch and descriptor table are allocated at
@@ -211,19 +215,36 @@ extern "C" int pm_batch_plan_create(int n, const pm_image *ins, const int *radii
       PM_FAIL(PM_E_HIP, "batch plan: descriptor upload failed: %s", hipGetErrorString(e));
     }
   }
-  // capture the launches on a private stream (default priority)
-  hipStream_t cs = nullptr;
-  e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  // capture the launches on private streams (default priority): the capture stream forks
+  // into NB branches (one per group, round robin) that join back before the capture ends,
+  // so the graph's radius groups are independent nodes that may run concurrently
+  constexpr int NB = 4;
+  hipStream_t st[NB] = {};
+  hipEvent_t fork = nullptr, join[NB] = {};
+  for (int i = 0; i < NB && e == hipSuccess; i++) e = hipStreamCreateWithFlags(&st[i], hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
+  for (int i = 1; i < NB && e == hipSuccess; i++) e = hipEventCreateWithFlags(&join[i], hipEventDisableTiming);
   int rc = PM_OK;
+  if (e == hipSuccess) e = hipStreamBeginCapture(st[0], hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    rc = batch_enqueue(L, dimg, cs);
+    hipError_t ec = hipEventRecord(fork, st[0]);
+    for (int i = 1; i < NB && ec == hipSuccess; i++) ec = hipStreamWaitEvent(st[i], fork, 0);
+    if (ec == hipSuccess) rc = batch_enqueue(L, dimg, st, NB);
+    for (int i = 1; i < NB && ec == hipSuccess; i++) {
+      ec = hipEventRecord(join[i], st[i]);
+      if (ec == hipSuccess) ec = hipStreamWaitEvent(st[0], join[i], 0);
+    }
     hipGraph_t g = nullptr;
-    const hipError_t e2 = hipStreamEndCapture(cs, &g);
+    const hipError_t e2 = hipStreamEndCapture(st[0], &g);
     p->graph = g;
-    if (rc == PM_OK && e2 != hipSuccess) e = e2;
+    if (ec != hipSuccess) e = ec;
+    else if (rc == PM_OK && e2 != hipSuccess) e = e2;
   }
-  if (cs) (void)hipStreamDestroy(cs);
+  if (fork) (void)hipEventDestroy(fork);
+  for (int i = 0; i < NB; i++) {
+    if (join[i]) (void)hipEventDestroy(join[i]);
+    if (st[i]) (void)hipStreamDestroy(st[i]);
+  }
   if (rc == PM_OK && e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
   if (rc != PM_OK || e != hipSuccess) {
     plan_free(p);
