@@ -58,6 +58,7 @@ def lib():
             "or_arith_ssub_image": [ci, vp, vp, vp, c64],
             "or_arith_sup": [ci, vp, vp, vp, c64],
             "or_arith_inf": [ci, vp, vp, vp, c64],
+            "or_label": [ci, ci, vp, vp, ci, i64p, ci, i64p],
         }
         for name in ("or_h_maxima", "or_h_minima", "or_h_convex", "or_h_concave",
                      "or_dynamic_pseudo_opening", "or_dynamic_pseudo_closing"):
@@ -257,3 +258,14 @@ def dynamic_pseudo_opening(img, h, se):
 
 def dynamic_pseudo_closing(img, h, se):
     return _extrema("dynamic_pseudo_closing", img, h, se)
+
+
+def label(img, se, flat=False):
+    """label_binary (flat=False) / label_flat_zones (flat=True): (int64 labels, count);
+    label_t.hpp t_label_helper, see oracle.h."""
+    img, t = _t(img)
+    out = np.empty(img.shape, dtype=np.int64)
+    cnt = ctypes.c_int64(0)
+    _check(lib().or_label(t, int(flat), _p(img), _p(out), img.ndim, _shape(img), SE[se] if isinstance(se, str) else se,
+                          ctypes.byref(cnt)), "label")
+    return out, cnt.value

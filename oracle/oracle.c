@@ -783,3 +783,97 @@ int or_dynamic_pseudo_closing(int t, const void *in, double h, void *out, int ra
 {
   return dynamic_pseudo(t, 0, in, h, out, rank, shape, se);
 }
+
+/* ------------------------------------------------------------------------- */
+/* Connected-component labelling (label_t.hpp t_label_helper, see oracle.h)   */
+/* ------------------------------------------------------------------------- */
+static int64_t lab_find(int64_t *par, int64_t x)
+{
+  int64_t r = x;
+  while (par[r] != r) r = par[r];
+  while (par[x] != r) { int64_t n = par[x]; par[x] = r; x = n; }
+  return r;
+}
+/* disjoint_min_sets::union_sets: the smaller root becomes the parent */
+static int64_t lab_union(int64_t *par, int64_t a, int64_t b)
+{
+  a = lab_find(par, a);
+  b = lab_find(par, b);
+  if (a < b) { par[b] = a; return a; }
+  if (b < a) { par[a] = b; return b; }
+  return a;
+}
+/* SE neighbours that precede the centre in raster order (split_coordinates_upper_lower) */
+static int lab_upper(int se, int rank, int64_t *up)
+{
+  int64_t offs[27 * 3];
+  int serank = 0, n = or_se_offsets(se, offs, &serank), m = 0;
+  if (n < 0) return OR_E_SE;
+  if (serank != rank) return OR_E_RANK;
+  for (int k = 0; k < n; k++) {
+    int sign = 0;
+    for (int a = 0; a < rank && !sign; a++) sign = offs[k * rank + a] < 0 ? -1 : offs[k * rank + a] > 0 ? 1 : 0;
+    if (sign < 0) { memcpy(up + m * rank, offs + k * rank, sizeof(int64_t) * (size_t)rank); m++; }
+  }
+  return m;
+}
+
+#define OR_LABEL_DEFINE(S, T, ACCEPT, POS)                                                                    \
+  static int label_##S(int flat, const T *in, int64_t *out, int rank, const int64_t *shape, int se,           \
+                       int64_t *count)                                                                        \
+  {                                                                                                           \
+    if (rank < 1 || rank > 3) return OR_E_RANK;                                                               \
+    int64_t up[13 * 3], p[3], q;                                                                              \
+    const int nu = lab_upper(se, rank, up);                                                                   \
+    if (nu < 0) return nu;                                                                                    \
+    const int64_t N = numel(rank, shape);                                                                     \
+    int64_t *par = (int64_t *)malloc(sizeof(int64_t) * (size_t)(N > 0 ? N : 1));                              \
+    if (!par) return OR_E_NOMEM;                                                                              \
+    int64_t nlab = 0;                                                                                         \
+    for (int64_t i = 0; i < N; i++) out[i] = 0;                                                               \
+    for (int64_t i = 0; i < N; i++) {                                                                         \
+      const T v = in[i];                                                                                      \
+      if (!flat && !(ACCEPT(v))) continue;                                                                    \
+      unravel(i, rank, shape, p);                                                                             \
+      int64_t label = -1;                                                                                     \
+      for (int k = 0; k < nu; k++) {                                                                          \
+        if (!shifted(p, up + k * rank, rank, shape, &q)) continue;                                            \
+        const T w = in[q];                                                                                    \
+        const int rel = flat ? (w == v) : (POS(w) && (ACCEPT(w)));                                            \
+        if (!rel) continue;                                                                                   \
+        if (label == -1 || label == out[q]) label = out[q];                                                   \
+        else label = lab_union(par, label, out[q]);                                                           \
+      }                                                                                                       \
+      if (label == -1) { par[nlab] = nlab; label = nlab++; }                                                  \
+      out[i] = label;                                                                                         \
+    }                                                                                                         \
+    /* disjoint_min_sets::min_label: roots numbered 1.. in creation order */                                  \
+    int64_t *num = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nlab > 0 ? nlab : 1));                       \
+    if (!num) { free(par); return OR_E_NOMEM; }                                                               \
+    int64_t cnt = 0;                                                                                          \
+    for (int64_t l = 0; l < nlab; l++) { if (par[l] == l) cnt++; num[l] = cnt; }                              \
+    for (int64_t i = 0; i < N; i++)                                                                           \
+      if (flat || (ACCEPT(in[i]))) out[i] = num[lab_find(par, out[i])];                                       \
+    *count = cnt;                                                                                             \
+    free(num);                                                                                                \
+    free(par);                                                                                                \
+    return OR_OK;                                                                                             \
+  }
+
+#define LAB_NZ(v) ((v) != 0)
+#define LAB_NZF(v) (fabsf(v) > 1E-10F)
+#define LAB_NZD(v) (fabs(v) > 1E-10)
+#define LAB_POS(v) ((v) > 0)
+OR_LABEL_DEFINE(u8, uint8_t, LAB_NZ, LAB_POS)
+OR_LABEL_DEFINE(u16, uint16_t, LAB_NZ, LAB_POS)
+OR_LABEL_DEFINE(i32, int32_t, LAB_NZ, LAB_POS)
+OR_LABEL_DEFINE(i64, int64_t, LAB_NZ, LAB_POS)
+OR_LABEL_DEFINE(f32, float, LAB_NZF, LAB_POS)
+OR_LABEL_DEFINE(f64, double, LAB_NZD, LAB_POS)
+
+int or_label(int t, int flat, const void *in, int64_t *out, int rank, const int64_t *shape, int se, int64_t *count)
+{
+#define C(S) label_##S(flat, (const T *)in, out, rank, shape, se, count)
+  OR_DISPATCH(t, C)
+#undef C
+}

@@ -111,6 +111,19 @@ int or_h_concave(int t, const void *in, double h, void *out, int rank, const int
 int or_dynamic_pseudo_opening(int t, const void *in, double h, void *out, int rank, const int64_t *shape, int se);
 int or_dynamic_pseudo_closing(int t, const void *in, double h, void *out, int rank, const int64_t *shape, int se);
 
+/* Connected-component labelling, include/poutre/label/details/label_t.hpp t_label_helper:
+ * one raster pass over the accepted pixels with a union-find of provisional labels (new
+ * labels 0, 1, 2... in raster order, union keeps the smaller root), linking a pixel to its
+ * raster-earlier ("upper") SE neighbours that satisfy the relation, then roots renumbered
+ * 1..count in creation order; background 0.
+ *  flat = 0: label_binary (src/label/label_binary.cpp): accept v != 0 (|v| > 1e-10 for
+ *            float), relation: neighbour > 0 (label_t.hpp not_background / sup_zero);
+ *  flat = 1: label_flat_zones (src/label/label_flat_zones.cpp): accept all, relation ==.
+ * Positive float neighbours at or below the 1e-10 background threshold make the reference
+ * read an unassigned provisional label; here they are not linked (not pinned by the
+ * reference).  out is int64 (the reference's GINT64 output image). */
+int or_label(int t, int flat, const void *in, int64_t *out, int rank, const int64_t *shape, int se, int64_t *count);
+
 #ifdef __cplusplus
 }
 #endif

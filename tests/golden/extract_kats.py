@@ -16,6 +16,9 @@ Sources read (relative to the reference root):
   test/geodesy/leveling.cpp                       (low_/high_leveling, leveling)
   test/geodesy/extrema.cpp                        (h_minima/h_maxima/h_concave/h_convex/dynamic_pseudo_*)
   test/pixel_processing/transpose.cpp             (t_transpose)
+  test/label/label_binary.cpp                     (label::label_binary)
+  test/label/label_flat_zones.cpp                 (label::label_flat_zones)
+  test/label/label_extrema.cpp                    (label::label_minima / label_maxima)
 
 Usage: python tests/golden/extract_kats.py [/root/reference]
 """
@@ -32,6 +35,9 @@ FILES = [
     "test/geodesy/leveling.cpp",
     "test/geodesy/extrema.cpp",
     "test/pixel_processing/transpose.cpp",
+    "test/label/label_binary.cpp",
+    "test/label/label_flat_zones.cpp",
+    "test/label/label_extrema.cpp",
 ]
 
 OPS = [
@@ -39,6 +45,7 @@ OPS = [
     "Reconstruction", "low_leveling", "high_leveling", "leveling",
     "h_minima", "h_maxima", "h_concave", "h_convex",
     "dynamic_pseudo_closing", "dynamic_pseudo_opening", "t_transpose",
+    "label_binary", "label_flat_zones", "label_minima", "label_maxima",
 ]
 
 _STR = r'(?:R"\((?P<raw>.*?)\)"|(?P<cooked>"(?:[^"\\]|\\.|\\\n)*"))'
@@ -47,7 +54,7 @@ TOKEN_RE = re.compile(
     r'|(?P<ifs>ImageFromString\(\s*' + _STR.replace("raw", "raw1").replace("cooked", "cooked1") + r')'
     r'|(?P<exp>expected\s*=\s*' + _STR.replace("raw", "raw2").replace("cooked", "cooked2") + r')'
     r'|(?P<const>static_cast<poutre::p\w+>\((?P<cval>-?\d+)\))'
-    r'|(?P<call>(?:poutre::)?(?:llm::details::|geo::|details::)?(?P<op>' + "|".join(OPS) + r')\((?P<args>[^;]*)\);)',
+    r'|(?P<call>(?:poutre::)?(?:llm::details::|geo::|details::|label::)?(?P<op>' + "|".join(OPS) + r')\((?P<args>[^;]*)\);)',
     re.S,
 )
 

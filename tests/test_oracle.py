@@ -36,6 +36,12 @@ def _run_kat(c):
         return getattr(oracle, op)(ins[0], c["h"], c["se"])
     if op == "t_transpose":
         return oracle.transpose(ins[0])
+    if op in ("label_binary", "label_flat_zones"):
+        return oracle.label(ins[0], c["se"], flat=(op == "label_flat_zones"))[0]
+    if op == "label_minima":  # src/label/label_extrema.cpp: label_binary(h_concave(img, 1))
+        return oracle.label(oracle.h_concave(ins[0], 1, c["se"]), c["se"])[0]
+    if op == "label_maxima":  # label_binary(h_convex(img, 1))
+        return oracle.label(oracle.h_convex(ins[0], 1, c["se"]), c["se"])[0]
     raise AssertionError(op)
 
 
@@ -49,7 +55,74 @@ def test_oracle_matches_reference_kat(case):
 
 def test_kat_count():
     # 20 static + 20 runtime + 6 line + 4 reconstruction + 4 leveling + 6 extrema + 1 transpose
-    assert len(kats.cases()) == 61
+    # + 4 label_binary + 2 label_flat_zones + 2 label_minima/maxima
+    assert len(kats.cases()) == 69
+
+
+# ---------------------------------------------------------------- labelling pins
+def _label_brute(img, se, flat):
+    """Connected components of the link graph by breadth-first search, numbered by the
+    raster position of each component's first pixel -- an independent statement of what the
+    one-pass union-find of the reference computes (links: a pixel to its raster-earlier SE
+    neighbours; binary: both pixels != 0 and the earlier one > 0; flat zones: equal values)."""
+    offs = [tuple(o) for o in oracle.se_offsets(se)]
+    upper = [o for o in offs if next((x for x in o if x != 0), 0) < 0]
+    shp = img.shape
+    flat_img = img.reshape(-1)
+    N = flat_img.size
+
+    def acc(v):
+        return True if flat else (abs(v) > 1e-10 if img.dtype.kind == "f" else v != 0)
+
+    def link(v, w):  # v at the later pixel, w at the earlier one
+        return (w == v) if flat else (acc(v) and acc(w) and w > 0)
+    adj = [[] for _ in range(N)]
+    for i in range(N):
+        p = np.unravel_index(i, shp)
+        for o in upper:
+            q = tuple(a + b for a, b in zip(p, o))
+            if all(0 <= c < d for c, d in zip(q, shp)):
+                j = int(np.ravel_multi_index(q, shp))
+                if link(flat_img[i], flat_img[j]):
+                    adj[i].append(j)
+                    adj[j].append(i)
+    out = np.zeros(N, dtype=np.int64)
+    n = 0
+    for i in range(N):
+        if out[i] or not acc(flat_img[i]):
+            continue
+        n += 1
+        out[i] = n
+        stack = [i]
+        while stack:
+            a = stack.pop()
+            for b in adj[a]:
+                if not out[b]:
+                    out[b] = n
+                    stack.append(b)
+    return out.reshape(shp), n
+
+
+@pytest.mark.parametrize("se,shape", [("SESquare2D", (9, 13)), ("SECross2D", (12, 7)), ("SESegmentX2D", (5, 9)),
+                                      ("SESegmentY2D", (8, 6)), ("SESegmentX1D", (40,)), ("SECross3D", (4, 5, 6)),
+                                      ("SESquare3D", (3, 6, 5)), ("SESegmentZ3D", (5, 3, 4))])
+@pytest.mark.parametrize("dtype", [np.uint8, np.int32, np.float64])
+def test_label_matches_components(se, shape, dtype):
+    rng = np.random.default_rng(11)
+    for trial in range(4):
+        if dtype == np.int32:
+            img = rng.integers(-1, 2, size=shape).astype(dtype)  # negatives exercise the one-sided link
+        else:
+            img = (rng.random(shape) < 0.45).astype(dtype)
+        got, n = oracle.label(img, se)
+        exp, m = _label_brute(img, se, False)
+        np.testing.assert_array_equal(got, exp)
+        assert n == m
+        zones = rng.integers(0, 3, size=shape).astype(dtype)
+        got, n = oracle.label(zones, se, flat=True)
+        exp, m = _label_brute(zones, se, True)
+        np.testing.assert_array_equal(got, exp)
+        assert n == m
 
 
 # ---------------------------------------------------------------- brute-force pins
