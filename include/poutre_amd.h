@@ -163,6 +163,20 @@ int pm_maxplus_fwd(const float *x, int64_t B, int64_t H, int64_t W, const float 
 int pm_maxplus_bwd(const float *x, const float *gy, int64_t B, int64_t H, int64_t W, const float *w, int K,
                    int dilate, float *gx, float *gw, void *stream);
 
+/* ---- connected-component labelling (include/poutre/label/).  out is an int64 (PM_PTYPE_I64)
+ * image of the input's shape; background pixels get 0 and the components 1..count, numbered
+ * by the raster position of their first pixel (label_t.hpp t_label_helper).  Input PTypes:
+ * uint8, uint16, int32, int64, float32, float64; SEs of the input's rank.  *count (may be
+ * NULL) receives the number of labels; reading it synchronises the stream. */
+/* label_binary.hpp:35 -- foreground = value != 0 (|v| > 1e-10 for floats), linked to a
+ * raster-earlier neighbour whose value is > 0 */
+int pm_label_binary(const pm_image *in, int se, pm_image *out, int64_t *count, void *stream);
+/* label_flat_zones.hpp:35 -- every pixel labelled, linked to equal-valued neighbours */
+int pm_label_flat_zones(const pm_image *in, int se, pm_image *out, int64_t *count, void *stream);
+/* label_extrema.hpp:34,39 -- label_binary of h_concave(in, 1) / h_convex(in, 1) */
+int pm_label_minima(const pm_image *in, int se, pm_image *out, void *stream);
+int pm_label_maxima(const pm_image *in, int se, pm_image *out, void *stream);
+
 /* ---- batched launches (no reference counterpart: the reference processes one image per call).
  * pm_batch_dilate_open: for each i < n, dil_outs[i] = Dilate(ins[i], SESquare2D, radii[i]) and
  * open_outs[i] = Dilate(Erode(ins[i], SESquare2D, radii[i]), SESquare2D, radii[i]), all images

@@ -232,4 +232,13 @@ void dynamic_pseudo_closing(const IInterface &i_img, const ScalarTypeVariant &pv
                             IInterface &o_img);
 
 }  // namespace geo
+
+namespace label {
+// include/poutre/label/label_binary.hpp:35, label_flat_zones.hpp:35, label_extrema.hpp:34-39;
+// o_img must be a GINT64 image of i_img's shape
+std::size_t label_binary(const IInterface &i_img, se::Common_NL_SE nl_static, IInterface &o_img);
+std::size_t label_flat_zones(const IInterface &i_img, se::Common_NL_SE nl_static, IInterface &o_img);
+void label_minima(const IInterface &i_img, se::Common_NL_SE nl_static, IInterface &o_img);
+void label_maxima(const IInterface &i_img, se::Common_NL_SE nl_static, IInterface &o_img);
+}  // namespace label
 }  // namespace poutre

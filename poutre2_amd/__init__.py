@@ -296,6 +296,49 @@ def h_concave(img, value, se, out=None):
     return _ext("pm_h_concave", img, value, se, out)
 
 
+# ---------------------------------------------------------------- labelling
+def _label_out(img, out):
+    t = _torch()
+    return t.empty(img.shape, dtype=t.int64, device=img.device) if out is None else out
+
+
+def label_binary(img, se, out=None):
+    """label::label_binary (label_binary.hpp:35): int64 labels of the connected components of
+    the foreground, numbered by the raster position of their first pixel.  Returns
+    (out, count); pypoutre returns the count and fills the caller's image."""
+    out = _label_out(img, out)
+    di, do = _desc(img), _desc(out)
+    n = ctypes.c_int64(0)
+    _call("pm_label_binary", ctypes.byref(di), int(se), ctypes.byref(do), ctypes.byref(n), _stream(img))
+    return out, n.value
+
+
+def label_flat_zones(img, se, out=None):
+    """label::label_flat_zones (label_flat_zones.hpp:35): int64 labels of the flat zones.
+    Returns (out, count)."""
+    out = _label_out(img, out)
+    di, do = _desc(img), _desc(out)
+    n = ctypes.c_int64(0)
+    _call("pm_label_flat_zones", ctypes.byref(di), int(se), ctypes.byref(do), ctypes.byref(n), _stream(img))
+    return out, n.value
+
+
+def label_minima(img, se, out=None):
+    """label::label_minima (label_extrema.hpp:34): int64 labels of the regional minima."""
+    out = _label_out(img, out)
+    di, do = _desc(img), _desc(out)
+    _call("pm_label_minima", ctypes.byref(di), int(se), ctypes.byref(do), _stream(img))
+    return out
+
+
+def label_maxima(img, se, out=None):
+    """label::label_maxima (label_extrema.hpp:39): int64 labels of the regional maxima."""
+    out = _label_out(img, out)
+    di, do = _desc(img), _desc(out)
+    _call("pm_label_maxima", ctypes.byref(di), int(se), ctypes.byref(do), _stream(img))
+    return out
+
+
 def dynamic_pseudo_opening(img, value, se, out=None):
     """geo::dynamic_pseudo_opening (extrema.hpp:57)."""
     return _ext("pm_dynamic_pseudo_opening", img, value, se, out)

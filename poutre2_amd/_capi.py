@@ -64,6 +64,10 @@ SIGNATURES = {
     "pm_h_concave": [_P, _D, _I, _P, _VP],
     "pm_dynamic_pseudo_opening": [_P, _D, _I, _P, _VP],
     "pm_dynamic_pseudo_closing": [_P, _D, _I, _P, _VP],
+    "pm_label_binary": [_P, _I, _P, ctypes.POINTER(_I64), _VP],
+    "pm_label_flat_zones": [_P, _I, _P, ctypes.POINTER(_I64), _VP],
+    "pm_label_minima": [_P, _I, _P, _VP],
+    "pm_label_maxima": [_P, _I, _P, _VP],
     "pm_arith_sup": [_P, _P, _P, _VP],
     "pm_arith_inf": [_P, _P, _P, _VP],
     "pm_arith_ssub_image": [_P, _P, _P, _VP],

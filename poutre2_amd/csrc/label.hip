@@ -1,0 +1,310 @@
+// label.hip -- connected-component labelling: label_binary, label_flat_zones, label_minima,
+// label_maxima (include/poutre/label/label_binary.hpp:35, label_flat_zones.hpp:35,
+// label_extrema.hpp:34-39; algorithm of label_t.hpp t_label_helper).
+//
+// The reference makes one raster pass with a union-find over provisional labels and
+// renumbers the roots; its result is fixed by the link graph alone: a pixel is linked to
+// each raster-earlier ("upper") neighbour of the SE that satisfies the relation (binary:
+// both pixels are foreground and the earlier one is > 0; flat zones: equal values), and each
+// component is numbered 1.. by the raster position of its first pixel (the first pixel of a
+// component always opens a new provisional label, which stays the component's root because
+// the union keeps the smaller root).  On the GPU:
+//  1. lab_init:     parent P[i] = i for accepted pixels, -1 for background;
+//  2. lab_merge:    every link (i, j) is merged with a lock-free union whose roots only
+//                   decrease (atomicMin on the larger root, retried from the value it
+//                   returns), so each root ends as the smallest index of its component;
+//  3. lab_compress: P[i] = root;
+//  4. ranks of the roots in raster order by a block count, a scan of the block counts and a
+//     per-block scan (lab_count / lab_scan_blocks / lab_rank);
+//  5. lab_output:   out[i] = rank of P[i] (int64, the reference's GINT64 output), 0 for the
+//                   background.
+// Positive float values at or below the reference's 1e-10 background threshold are not
+// linked (the reference reads an unassigned provisional label for them).
+#include <cmath>
+
+#include "erodil.hpp"
+#include "pm_common.hpp"
+
+namespace pm {
+
+constexpr int LAB_BLOCK = 1024;  // elements per block of the rank scan (256 threads x 4)
+
+struct LabUpper {
+  int n;
+  int dz[13], dy[13], dx[13];
+};
+
+template <typename T> __device__ __forceinline__ bool lab_fg(T v) { return v != T(0); }
+template <> __device__ __forceinline__ bool lab_fg<float>(float v) { return fabsf(v) > 1E-10F; }
+template <> __device__ __forceinline__ bool lab_fg<double>(double v) { return fabs(v) > 1E-10; }
+
+template <typename T, bool FLAT> __device__ __forceinline__ bool lab_accept(T v) { return FLAT || lab_fg<T>(v); }
+// v at the pixel, w at its raster-earlier neighbour
+template <typename T, bool FLAT> __device__ __forceinline__ bool lab_link(T v, T w)
+{
+  if (FLAT) return w == v;
+  return lab_fg<T>(v) && lab_fg<T>(w) && w > T(0);
+}
+
+__device__ __forceinline__ int lab_find(const int *P, int x)
+{
+  int y;
+  while ((y = P[x]) != x) x = y;
+  return x;
+}
+
+// Merge the sets of a and b.  Invariant: P[x] <= x and values only decrease, so every find
+// terminates and every retry starts from a strictly smaller index.
+__device__ __forceinline__ void lab_unite(int *P, int a, int b)
+{
+  for (;;) {
+    a = lab_find(P, a);
+    b = lab_find(P, b);
+    if (a == b) return;
+    if (a < b) {
+      const int t = a;
+      a = b;
+      b = t;
+    }
+    const int old = atomicMin(&P[a], b);  // link the larger root under the smaller one
+    if (old == a) return;
+    a = old;  // a had been linked meanwhile: keep its new parent connected to b
+  }
+}
+
+template <typename T, bool FLAT> __global__ void lab_init(const T *__restrict__ in, int *__restrict__ P, int64_t N)
+{
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < N) P[i] = lab_accept<T, FLAT>(in[i]) ? (int)i : -1;
+}
+
+template <typename T, bool FLAT>
+__global__ void lab_merge(const T *__restrict__ in, int *P, int D, int H, int W, LabUpper u)
+{
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t N = (int64_t)D * H * W;
+  if (i >= N) return;
+  const T v = in[i];
+  if (!lab_accept<T, FLAT>(v)) return;
+  const int x = (int)(i % W);
+  const int64_t r = i / W;
+  const int y = (int)(r % H), z = (int)(r / H);
+  for (int k = 0; k < u.n; k++) {
+    const int zz = z + u.dz[k], yy = y + u.dy[k], xx = x + u.dx[k];
+    if ((unsigned)zz >= (unsigned)D || (unsigned)yy >= (unsigned)H || (unsigned)xx >= (unsigned)W) continue;
+    const int64_t j = ((int64_t)zz * H + yy) * W + xx;
+    if (lab_link<T, FLAT>(v, in[j])) lab_unite(P, (int)i, (int)j);
+  }
+}
+
+__global__ void lab_compress(int *P, int64_t N)
+{
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < N && P[i] >= 0) P[i] = lab_find(P, (int)i);
+}
+
+// roots (P[i] == i) per block of LAB_BLOCK elements
+__global__ __launch_bounds__(256) void lab_count(const int *__restrict__ P, int *__restrict__ cnt, int64_t N)
+{
+  __shared__ int s[256];
+  const int64_t base = (int64_t)blockIdx.x * LAB_BLOCK + 4 * threadIdx.x;
+  int c = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) c += (base + k < N && P[base + k] == (int)(base + k)) ? 1 : 0;
+  s[threadIdx.x] = c;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) s[threadIdx.x] += s[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) cnt[blockIdx.x] = s[0];
+}
+
+// inclusive scan of 256 values in LDS (Hillis-Steele)
+__device__ __forceinline__ int lab_scan256(int *s, int v)
+{
+  const int t = threadIdx.x;
+  s[t] = v;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {
+    const int a = t >= o ? s[t - o] : 0;
+    __syncthreads();
+    s[t] += a;
+    __syncthreads();
+  }
+  return s[t];
+}
+
+// exclusive offsets of the block counts (one workgroup, chunks of 1024 with a running
+// carry) and the total number of labels
+__global__ __launch_bounds__(256) void lab_scan_blocks(int *cnt, int nb, int64_t *total)
+{
+  __shared__ int s[256];
+  int carry = 0;
+  for (int c0 = 0; c0 < nb; c0 += 1024) {
+    int v[4], sum = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int b = c0 + 4 * threadIdx.x + k;
+      v[k] = b < nb ? cnt[b] : 0;
+      sum += v[k];
+    }
+    const int incl = lab_scan256(s, sum);
+    int run = carry + incl - sum;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int b = c0 + 4 * threadIdx.x + k;
+      if (b < nb) cnt[b] = run;
+      run += v[k];
+    }
+    carry += s[255];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+// rank (1-based, raster order) of every root into R
+__global__ __launch_bounds__(256) void lab_rank(const int *__restrict__ P, const int *__restrict__ off,
+                                                int *__restrict__ R, int64_t N)
+{
+  __shared__ int s[256];
+  const int64_t base = (int64_t)blockIdx.x * LAB_BLOCK + 4 * threadIdx.x;
+  bool f[4];
+  int c = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    f[k] = base + k < N && P[base + k] == (int)(base + k);
+    c += f[k] ? 1 : 0;
+  }
+  int run = off[blockIdx.x] + lab_scan256(s, c) - c;
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    if (f[k]) R[base + k] = ++run;
+}
+
+__global__ void lab_output(const int *__restrict__ P, const int *__restrict__ R, int64_t *__restrict__ out, int64_t N)
+{
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < N) out[i] = P[i] >= 0 ? (int64_t)R[P[i]] : 0;
+}
+
+// upper (raster-earlier) neighbours of a Common_NL_SE (neighbor_list_static_se_t.hpp
+// split_coordinates_upper_lower)
+static int lab_upper(int se, LabUpper &u)
+{
+  NLArgs a;
+  PM_TRY(se_nl(se, a));
+  u.n = 0;
+  for (int k = 0; k < a.n; k++) {
+    const int sign = a.dz[k] ? a.dz[k] : a.dy[k] ? a.dy[k] : a.dx[k];
+    if (sign < 0) {
+      u.dz[u.n] = a.dz[k];
+      u.dy[u.n] = a.dy[k];
+      u.dx[u.n] = a.dx[k];
+      u.n++;
+    }
+  }
+  return PM_OK;
+}
+
+static int lab_check(const pm_image *in, const pm_image *out, int se, const char *what)
+{
+  clear_error();
+  if (!in || !out) PM_FAIL(PM_E_ARG, "%s: null image descriptor", what);
+  if (in->rank == 0) PM_FAIL(PM_E_RANK, "%s Unsupported number of dims:0", what);
+  if (in->rank < 1 || in->rank > 3) PM_FAIL(PM_E_RANK, "%s Unsupported number of dims", what);
+  if (out->rank != in->rank) PM_FAIL(PM_E_SIZE, "%s incompatible size", what);
+  for (int i = 0; i < in->rank; i++) {
+    if (in->shape[i] < 0) PM_FAIL(PM_E_SIZE, "%s: negative extent", what);
+    if (out->shape[i] != in->shape[i]) PM_FAIL(PM_E_SIZE, "%s incompatible size", what);
+  }
+  if (ptype_size(in->ptype) == 0) PM_FAIL(PM_E_PTYPE, "%s unsupported PTYPE", what);
+  if (out->ptype != PM_PTYPE_I64) PM_FAIL(PM_E_PTYPE, "%s: the output image must be GINT64", what);
+  const int64_t n = numel(*in);
+  if (n > 0 && (!in->data || !out->data)) PM_FAIL(PM_E_ARG, "%s: null data pointer", what);
+  if (n > 0 && in->data == out->data) PM_FAIL(PM_E_ALIAS, "%s output must be != than input images", what);
+  if (n >= INT32_MAX) PM_FAIL(PM_E_SIZE, "%s: images of 2^31 pixels or more are not supported", what);
+  PM_TRY(check_se_rank(se, in->rank));
+  return PM_OK;
+}
+
+template <bool FLAT>
+static int lab_run(const pm_image *in, int se, pm_image *out, int64_t *count, hipStream_t s, const char *what)
+{
+  PM_TRY(lab_check(in, out, se, what));
+  const int64_t N = numel(*in);
+  if (N == 0) {
+    if (count) *count = 0;
+    return PM_OK;
+  }
+  LabUpper u;
+  PM_TRY(lab_upper(se, u));
+  const Dims d = dims3(*in);
+  const int64_t nb = (N + LAB_BLOCK - 1) / LAB_BLOCK;
+  Scratch sc;
+  const size_t pb = ((size_t)N * sizeof(int) + 255) & ~(size_t)255, cb = ((size_t)nb * sizeof(int) + 255) & ~(size_t)255;
+  PM_TRY(sc.alloc(2 * pb + cb + 256, s));
+  int *P = (int *)sc.ptr;
+  int *R = (int *)((char *)sc.ptr + pb);
+  int *cnt = (int *)((char *)sc.ptr + 2 * pb);
+  int64_t *total = (int64_t *)((char *)sc.ptr + 2 * pb + cb);
+  const unsigned g = (unsigned)((N + 255) / 256);
+  auto f = [&](auto tag) -> int {
+    using T = typename decltype(tag)::type;
+    const T *src = (const T *)in->data;
+    hipLaunchKernelGGL((lab_init<T, FLAT>), dim3(g), dim3(256), 0, s, src, P, N);
+    hipLaunchKernelGGL((lab_merge<T, FLAT>), dim3(g), dim3(256), 0, s, src, P, (int)d.D, (int)d.H, (int)d.W, u);
+    PM_HIP(hipGetLastError());
+    return PM_OK;
+  };
+  PM_TRY(dispatch_ptype(in->ptype, f));
+  hipLaunchKernelGGL(lab_compress, dim3(g), dim3(256), 0, s, P, N);
+  hipLaunchKernelGGL(lab_count, dim3((unsigned)nb), dim3(256), 0, s, P, cnt, N);
+  hipLaunchKernelGGL(lab_scan_blocks, dim3(1), dim3(256), 0, s, cnt, (int)nb, total);
+  hipLaunchKernelGGL(lab_rank, dim3((unsigned)nb), dim3(256), 0, s, P, cnt, R, N);
+  hipLaunchKernelGGL(lab_output, dim3(g), dim3(256), 0, s, P, R, (int64_t *)out->data, N);
+  PM_HIP(hipGetLastError());
+  if (count) {
+    PM_HIP(hipMemcpyAsync(count, total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    PM_HIP(hipStreamSynchronize(s));
+  }
+  return PM_OK;
+}
+
+}  // namespace pm
+
+using namespace pm;
+
+extern "C" {
+int pm_label_binary(const pm_image *in, int se, pm_image *out, int64_t *count, void *stream)
+{
+  return lab_run<false>(in, se, out, count, as_stream(stream), "label_binary");
+}
+
+int pm_label_flat_zones(const pm_image *in, int se, pm_image *out, int64_t *count, void *stream)
+{
+  return lab_run<true>(in, se, out, count, as_stream(stream), "label_flat_zones");
+}
+
+int pm_h_convex(const pm_image *in, double h, int se, pm_image *out, void *stream);
+int pm_h_concave(const pm_image *in, double h, int se, pm_image *out, void *stream);
+
+// label_extrema.cpp: label_binary of h_concave(img, 1) (minima) / h_convex(img, 1) (maxima)
+static int lab_extrema(bool minima, const pm_image *in, int se, pm_image *out, void *stream)
+{
+  const char *what = minima ? "label_minima" : "label_maxima";
+  PM_TRY(lab_check(in, out, se, what));
+  hipStream_t s = as_stream(stream);
+  const int64_t n = numel(*in);
+  if (n == 0) return PM_OK;
+  Scratch tmp;
+  PM_TRY(tmp.alloc((size_t)n * ptype_size(in->ptype), s));
+  pm_image t = *in;
+  t.data = tmp.ptr;
+  PM_TRY(minima ? pm_h_concave(in, 1.0, se, &t, stream) : pm_h_convex(in, 1.0, se, &t, stream));
+  return lab_run<false>(&t, se, out, nullptr, s, what);
+}
+
+int pm_label_minima(const pm_image *in, int se, pm_image *out, void *stream) { return lab_extrema(true, in, se, out, stream); }
+int pm_label_maxima(const pm_image *in, int se, pm_image *out, void *stream) { return lab_extrema(false, in, se, out, stream); }
+}

@@ -511,4 +511,45 @@ PM_EXTREMA(dynamic_pseudo_closing)
 #undef PM_EXTREMA
 
 }  // namespace geo
+
+// ------------------------------------------------------------------ labelling
+namespace label {
+
+namespace {
+// label_binary.cpp:21-35: the output must be a GINT64 image (downcast check), sizes
+// compatible, images different
+template <typename F> void labelled(const IInterface &i, IInterface &o, const std::string &what, F &&f)
+{
+  AssertSizesCompatible(i, o, what + " incompatible size");
+  AssertImagesAreDifferent(i, o, what + " output must be != than input images");
+  if (o.GetPType() != PType::PType_GrayINT64) fail(what + "ImageDispatch o_img downcast fail");
+  Staged si(i, true), so(o, false);
+  pm_check(f(si.desc(), so.desc()));
+  so.download();
+}
+}  // namespace
+
+std::size_t label_binary(const IInterface &i, se::Common_NL_SE s, IInterface &o)
+{
+  int64_t n = 0;
+  labelled(i, o, "label_binary", [&](pm_image *a, pm_image *b) { return pm_label_binary(a, (int)s, b, &n, nullptr); });
+  return (std::size_t)n;
+}
+std::size_t label_flat_zones(const IInterface &i, se::Common_NL_SE s, IInterface &o)
+{
+  int64_t n = 0;
+  labelled(i, o, "label_flat_zones",
+           [&](pm_image *a, pm_image *b) { return pm_label_flat_zones(a, (int)s, b, &n, nullptr); });
+  return (std::size_t)n;
+}
+void label_minima(const IInterface &i, se::Common_NL_SE s, IInterface &o)
+{
+  labelled(i, o, "label_minima", [&](pm_image *a, pm_image *b) { return pm_label_minima(a, (int)s, b, nullptr); });
+}
+void label_maxima(const IInterface &i, se::Common_NL_SE s, IInterface &o)
+{
+  labelled(i, o, "label_maxima", [&](pm_image *a, pm_image *b) { return pm_label_maxima(a, (int)s, b, nullptr); });
+}
+
+}  // namespace label
 }  // namespace poutre

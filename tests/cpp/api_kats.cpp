@@ -80,6 +80,10 @@ static int run_case(const std::vector<std::string> &f, bool dev)
   else if (op == "h_concave") geo::h_concave(*a, h, se, *out);
   else if (op == "dynamic_pseudo_opening") geo::dynamic_pseudo_opening(*a, h, se, *out);
   else if (op == "dynamic_pseudo_closing") geo::dynamic_pseudo_closing(*a, h, se, *out);
+  else if (op == "label_binary") label::label_binary(*a, se, *out);
+  else if (op == "label_flat_zones") label::label_flat_zones(*a, se, *out);
+  else if (op == "label_minima") label::label_minima(*a, se, *out);
+  else if (op == "label_maxima") label::label_maxima(*a, se, *out);
   else return 2;
   const std::string got = ImageToString(*out);
   const std::string exp = ImageToString(*ImageFromString(f[7]));

@@ -55,6 +55,10 @@ def _gpu_kat(c):
         return getattr(pm, op)(ins[0], ins[1], PM_SE[se])
     if op in ("h_minima", "h_maxima", "h_concave", "h_convex", "dynamic_pseudo_closing", "dynamic_pseudo_opening"):
         return getattr(pm, op)(ins[0], c["h"], PM_SE[se])
+    if op in ("label_binary", "label_flat_zones"):
+        return getattr(pm, op)(ins[0], PM_SE[se])[0]
+    if op in ("label_minima", "label_maxima"):
+        return getattr(pm, op)(ins[0], PM_SE[se])
     pytest.skip(f"{op} is not a device operation")
 
 
@@ -420,3 +424,60 @@ def test_square_u8_unaligned_views(w, off_in, off_out, r):
         np.testing.assert_array_equal(h(oview), oracle.static_iter(img, "SESquare2D", r, dil))
         rest = obuf.cpu().numpy()
         assert (rest[:off_out] == 77).all() and (rest[off_out + H * w:] == 77).all()
+
+
+# ----------------------------------------------------------------------------- labelling
+@pytest.mark.parametrize("se,shape", [("SESquare2D", (300, 517)), ("SECross2D", (129, 257)), ("SESegmentX2D", (64, 90)),
+                                      ("SESegmentY2D", (90, 64)), ("SESegmentX1D", (5000,)), ("SECross3D", (20, 33, 41)),
+                                      ("SESquare3D", (17, 30, 29)), ("SESegmentZ3D", (9, 10, 11))])
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.int32, np.int64, np.float32, np.float64])
+def test_label_binary_and_flat_zones(se, shape, dtype):
+    rng = np.random.default_rng(200)
+    if np.dtype(dtype).kind == "i":
+        img = rng.integers(-1, 2, size=shape).astype(dtype)  # negatives: foreground not linkable from below
+    else:
+        img = (rng.random(shape) < 0.5).astype(dtype)
+    out, n = pm.label_binary(g(img), PM_SE[se])
+    exp, m = oracle.label(img, se)
+    np.testing.assert_array_equal(h(out), exp)
+    assert n == m
+    zones = rng.integers(0, 4, size=shape).astype(dtype)
+    out, n = pm.label_flat_zones(g(zones), PM_SE[se])
+    exp, m = oracle.label(zones, se, flat=True)
+    np.testing.assert_array_equal(h(out), exp)
+    assert n == m
+
+
+@pytest.mark.parametrize("shape", [(1024, 1024), (1, 1), (3, 2049), (700, 3)])
+def test_label_binary_large_blobs(shape):
+    """long-range components (thresholded blurred noise: a few big, winding blobs) and
+    degenerate shapes"""
+    from poutre2_amd import synth
+    if min(shape) >= 64:
+        img = (synth.blurred_noise_mask(shape[0], shape[1], 9) > 127).astype(np.uint8)
+    else:
+        img = (np.random.default_rng(9).random(shape) < 0.6).astype(np.uint8)
+    for se in ("SESquare2D", "SECross2D"):
+        out, n = pm.label_binary(g(img), PM_SE[se])
+        exp, m = oracle.label(img, se)
+        np.testing.assert_array_equal(h(out), exp)
+        assert n == m
+
+
+@pytest.mark.parametrize("op", ["label_minima", "label_maxima"])
+def test_label_extrema(op):
+    img = rand((120, 97), np.int32, 210, 0, 6)
+    got = h(getattr(pm, op)(g(img), PM_SE["SESquare2D"]))
+    f = oracle.h_concave if op == "label_minima" else oracle.h_convex
+    exp, _ = oracle.label(f(img, 1, "SESquare2D"), "SESquare2D")
+    np.testing.assert_array_equal(got, exp)
+
+
+def test_label_errors():
+    a = g(rand((8, 9), np.uint8, 1))
+    with pytest.raises(pm.PoutreError):
+        pm.label_binary(a, PM_SE["SESquare2D"], out=g(np.zeros((8, 9), np.int32)))  # output must be int64
+    with pytest.raises(pm.PoutreError):
+        pm.label_binary(a, PM_SE["SESquare2D"], out=g(np.zeros((8, 8), np.int64)))
+    with pytest.raises(pm.PoutreError):
+        pm.label_binary(a, PM_SE["SECross3D"])
