@@ -9,10 +9,12 @@
 // component is numbered 1.. by the raster position of its first pixel (the first pixel of a
 // component always opens a new provisional label, which stays the component's root because
 // the union keeps the smaller root).  On the GPU:
-//  1. lab_init:     parent P[i] = i for accepted pixels, -1 for background;
-//  2. lab_merge:    every link (i, j) is merged with a lock-free union whose roots only
-//                   decrease (atomicMin on the larger root, retried from the value it
-//                   returns), so each root ends as the smallest index of its component;
+//  1. lab_tile:     per 32 x 64 tile (8 x 8 x 32 in 3D), the links inside the tile are merged
+//                   in LDS and every pixel gets the global index of its tile-local root;
+//  2. lab_border:   the links that cross tile borders are merged in global memory.  Both use
+//                   a lock-free union whose roots only decrease (atomicMin on the larger
+//                   root, retried from the value it returns), so each root ends as the
+//                   smallest index of its component;
 //  3. lab_compress: P[i] = root;
 //  4. ranks of the roots in raster order by a block count, a scan of the block counts and a
 //     per-block scan (lab_count / lab_scan_blocks / lab_rank);
@@ -72,24 +74,97 @@ __device__ __forceinline__ void lab_unite(int *P, int a, int b)
   }
 }
 
-template <typename T, bool FLAT> __global__ void lab_init(const T *__restrict__ in, int *__restrict__ P, int64_t N)
+// ---- tiled form: union-find inside a TD x TH x TW tile in LDS, then only the links that
+// cross tile borders go through global memory.
+__device__ __forceinline__ int lab_find_lds(const int *L, int x)
 {
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i < N) P[i] = lab_accept<T, FLAT>(in[i]) ? (int)i : -1;
+  int y;
+  while ((y = L[x]) != x) x = y;
+  return x;
+}
+__device__ __forceinline__ void lab_unite_lds(int *L, int a, int b)
+{
+  for (;;) {
+    a = lab_find_lds(L, a);
+    b = lab_find_lds(L, b);
+    if (a == b) return;
+    if (a < b) {
+      const int t = a;
+      a = b;
+      b = t;
+    }
+    const int old = atomicMin(&L[a], b);
+    if (old == a) return;
+    a = old;
+  }
 }
 
-template <typename T, bool FLAT>
-__global__ void lab_merge(const T *__restrict__ in, int *P, int D, int H, int W, LabUpper u)
+// Tile (blockIdx.x, .y, .z): local labels in LDS (local index = (lz*TH + ly)*TW + lx, which
+// orders the tile's pixels as the raster does, so a local root is the component's first
+// pixel in the tile), then P[i] = global index of the local root (-1 for background).
+template <typename T, bool FLAT, int TD, int TH, int TW>
+__global__ __launch_bounds__(256) void lab_tile(const T *__restrict__ in, int *__restrict__ P, int D, int H, int W,
+                                                LabUpper u)
+{
+  constexpr int NT = TD * TH * TW;
+  __shared__ int L[NT];
+  const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH, z0 = blockIdx.z * TD;
+  for (int e = threadIdx.x; e < NT; e += 256) {
+    const int lx = e % TW, ly = (e / TW) % TH, lz = e / (TW * TH);
+    const int x = x0 + lx, y = y0 + ly, z = z0 + lz;
+    bool a = false;
+    if (x < W && y < H && z < D) a = lab_accept<T, FLAT>(in[((int64_t)z * H + y) * W + x]);
+    L[e] = a ? e : -1;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < NT; e += 256) {
+    if (L[e] < 0) continue;  // background or outside the image (a pixel's own entry is only
+                             // lowered by unions, never to -1)
+    const int lx = e % TW, ly = (e / TW) % TH, lz = e / (TW * TH);
+    const int x = x0 + lx, y = y0 + ly, z = z0 + lz;
+    const T v = in[((int64_t)z * H + y) * W + x];
+    for (int k = 0; k < u.n; k++) {
+      const int nx = lx + u.dx[k], ny = ly + u.dy[k], nz = lz + u.dz[k];
+      if ((unsigned)nx >= (unsigned)TW || (unsigned)ny >= (unsigned)TH || (unsigned)nz >= (unsigned)TD) continue;
+      if (x0 + nx >= W || y0 + ny >= H || z0 + nz >= D) continue;
+      const T w = in[((int64_t)(z0 + nz) * H + (y0 + ny)) * W + (x0 + nx)];
+      if (lab_link<T, FLAT>(v, w)) lab_unite_lds(L, e, (nz * TH + ny) * TW + nx);
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < NT; e += 256) {
+    const int lx = e % TW, ly = (e / TW) % TH, lz = e / (TW * TH);
+    const int x = x0 + lx, y = y0 + ly, z = z0 + lz;
+    if (x >= W || y >= H || z >= D) continue;
+    int g = -1;
+    if (L[e] >= 0) {
+      const int r = lab_find_lds(L, e);
+      const int rx = r % TW, ry = (r / TW) % TH, rz = r / (TW * TH);
+      g = (int)(((int64_t)(z0 + rz) * H + (y0 + ry)) * W + (x0 + rx));
+    }
+    P[((int64_t)z * H + y) * W + x] = g;
+  }
+}
+
+// the links whose two pixels lie in different tiles, merged in global memory
+template <typename T, bool FLAT, int TD, int TH, int TW>
+__global__ void lab_border(const T *__restrict__ in, int *P, int D, int H, int W, LabUpper u)
 {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int64_t N = (int64_t)D * H * W;
   if (i >= N) return;
-  const T v = in[i];
-  if (!lab_accept<T, FLAT>(v)) return;
   const int x = (int)(i % W);
   const int64_t r = i / W;
   const int y = (int)(r % H), z = (int)(r / H);
+  const int lx = x % TW, ly = y % TH, lz = z % TD;
+  // only pixels on a tile face can have an upper neighbour in another tile (upper offsets
+  // have dz <= 0; with dz = -1 they may step +1 in y or x, with dy = -1 +1 in x)
+  if (lx != 0 && lx != TW - 1 && ly != 0 && ly != TH - 1 && !(TD > 1 && lz == 0)) return;
+  if (P[i] < 0) return;
+  const T v = in[i];
   for (int k = 0; k < u.n; k++) {
+    const int nx = lx + u.dx[k], ny = ly + u.dy[k], nz = lz + u.dz[k];
+    if ((unsigned)nx < (unsigned)TW && (unsigned)ny < (unsigned)TH && (unsigned)nz < (unsigned)TD) continue;
     const int zz = z + u.dz[k], yy = y + u.dy[k], xx = x + u.dx[k];
     if ((unsigned)zz >= (unsigned)D || (unsigned)yy >= (unsigned)H || (unsigned)xx >= (unsigned)W) continue;
     const int64_t j = ((int64_t)zz * H + yy) * W + xx;
@@ -252,8 +327,18 @@ static int lab_run(const pm_image *in, int se, pm_image *out, int64_t *count, hi
   auto f = [&](auto tag) -> int {
     using T = typename decltype(tag)::type;
     const T *src = (const T *)in->data;
-    hipLaunchKernelGGL((lab_init<T, FLAT>), dim3(g), dim3(256), 0, s, src, P, N);
-    hipLaunchKernelGGL((lab_merge<T, FLAT>), dim3(g), dim3(256), 0, s, src, P, (int)d.D, (int)d.H, (int)d.W, u);
+    if (d.D == 1) {  // 1D / 2D: tiles of 32 x 64
+      constexpr int TH = 32, TW = 64;
+      const dim3 tg((unsigned)((d.W + TW - 1) / TW), (unsigned)((d.H + TH - 1) / TH), 1);
+      hipLaunchKernelGGL((lab_tile<T, FLAT, 1, TH, TW>), tg, dim3(256), 0, s, src, P, 1, (int)d.H, (int)d.W, u);
+      hipLaunchKernelGGL((lab_border<T, FLAT, 1, TH, TW>), dim3(g), dim3(256), 0, s, src, P, 1, (int)d.H, (int)d.W, u);
+    } else {  // 3D: tiles of 8 x 8 x 32
+      constexpr int TD = 8, TH = 8, TW = 32;
+      const dim3 tg((unsigned)((d.W + TW - 1) / TW), (unsigned)((d.H + TH - 1) / TH), (unsigned)((d.D + TD - 1) / TD));
+      hipLaunchKernelGGL((lab_tile<T, FLAT, TD, TH, TW>), tg, dim3(256), 0, s, src, P, (int)d.D, (int)d.H, (int)d.W, u);
+      hipLaunchKernelGGL((lab_border<T, FLAT, TD, TH, TW>), dim3(g), dim3(256), 0, s, src, P, (int)d.D, (int)d.H,
+                         (int)d.W, u);
+    }
     PM_HIP(hipGetLastError());
     return PM_OK;
   };
