@@ -9,8 +9,9 @@
 // component is numbered 1.. by the raster position of its first pixel (the first pixel of a
 // component always opens a new provisional label, which stays the component's root because
 // the union keeps the smaller root).  On the GPU:
-//  1. lab_tile:     per 32 x 64 tile (8 x 8 x 32 in 3D), the links inside the tile are merged
-//                   in LDS and every pixel gets the global index of its tile-local root;
+//  1. lab_tile2d / lab_tile: per 32 x 64 tile (8 x 8 x 32 in 3D), the links inside the tile
+//                   are merged in LDS (in 2D the links along a row by a ballot over the
+//                   row's runs) and every pixel gets the global index of its tile-local root;
 //  2. lab_border:   the links that cross tile borders are merged in global memory.  Both use
 //                   a lock-free union whose roots only decrease (atomicMin on the larger
 //                   root, retried from the value it returns), so each root ends as the
@@ -48,6 +49,9 @@ template <typename T, bool FLAT> __device__ __forceinline__ bool lab_link(T v, T
   return lab_fg<T>(v) && lab_fg<T>(w) && w > T(0);
 }
 
+// Root of x in global memory.  No path compression here: the parents in global memory are
+// changed by atomics only while unions run (plain stores from other XCDs' caches could
+// overwrite them); lab_compress flattens the trees once the unions are done.
 __device__ __forceinline__ int lab_find(const int *P, int x)
 {
   int y;
@@ -76,11 +80,20 @@ __device__ __forceinline__ void lab_unite(int *P, int a, int b)
 
 // ---- tiled form: union-find inside a TD x TH x TW tile in LDS, then only the links that
 // cross tile borders go through global memory.
-__device__ __forceinline__ int lab_find_lds(const int *L, int x)
+// Root of x in LDS with path halving (x's parent moved up to its grandparent as the walk
+// goes).  A halving store may race with a union lowering L[x]; unions only link roots and,
+// finding x already linked, go on with x's parent, so x keeps an ancestor (a smaller index)
+// as parent either way and no link is lost.
+__device__ __forceinline__ int lab_find_lds(int *L, int x)
 {
-  int y;
-  while ((y = L[x]) != x) x = y;
-  return x;
+  for (;;) {
+    const int p = L[x];
+    if (p == x) return x;
+    const int gp = L[p];
+    if (gp == p) return p;
+    L[x] = gp;
+    x = gp;
+  }
 }
 __device__ __forceinline__ void lab_unite_lds(int *L, int a, int b)
 {
@@ -143,6 +156,60 @@ __global__ __launch_bounds__(256) void lab_tile(const T *__restrict__ in, int *_
       g = (int)(((int64_t)(z0 + rz) * H + (y0 + ry)) * W + (x0 + rx));
     }
     P[((int64_t)z * H + y) * W + x] = g;
+  }
+}
+
+// 2D tiles of TH rows x 64 columns, one wave per row at a time (lane = column).  The link to
+// the left pixel of the same row is resolved without unions: a ballot marks the lanes that
+// start a run (not linked to their left neighbour) and every pixel points at the start of
+// its run.  Only the links to the row above (dy = -1) then go through the LDS union-find.
+template <typename T, bool FLAT, int TH>
+__global__ __launch_bounds__(256) void lab_tile2d(const T *__restrict__ in, int *__restrict__ P, int H, int W,
+                                                  LabUpper u, int xlink)
+{
+  constexpr int TW = 64, NT = TH * TW, RPW = TH / 4;  // rows per wave
+  __shared__ int L[NT];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH, x = x0 + lane;
+  T v[RPW];
+  bool acc[RPW];
+#pragma unroll
+  for (int k = 0; k < RPW; k++) {
+    const int ly = wv + 4 * k, y = y0 + ly;
+    const bool in_img = x < W && y < H;
+    v[k] = in_img ? in[(int64_t)y * W + x] : T(0);
+    acc[k] = in_img && lab_accept<T, FLAT>(v[k]);
+    bool lk = false;
+    if (xlink && acc[k] && lane > 0) lk = lab_link<T, FLAT>(v[k], in[(int64_t)y * W + x - 1]);
+    const uint64_t starts = __ballot(!lk);
+    const uint64_t m = lane == 63 ? starts : (starts & ((2ull << lane) - 1));
+    const int st = 63 - __clzll((long long)m);
+    L[ly * TW + lane] = acc[k] ? ly * TW + st : -1;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < RPW; k++) {
+    const int ly = wv + 4 * k, y = y0 + ly;
+    if (!acc[k] || ly == 0) continue;
+    for (int j = 0; j < u.n; j++) {
+      if (u.dz[j] != 0 || u.dy[j] != -1) continue;  // (dz != 0 only for 3D SEs, out of a D = 1 image)
+      const int nx = lane + u.dx[j];
+      if ((unsigned)nx >= (unsigned)TW || x0 + nx >= W) continue;
+      if (lab_link<T, FLAT>(v[k], in[(int64_t)(y - 1) * W + x0 + nx]))
+        lab_unite_lds(L, ly * TW + lane, (ly - 1) * TW + nx);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < RPW; k++) {
+    const int ly = wv + 4 * k, y = y0 + ly;
+    if (x >= W || y >= H) continue;
+    int g = -1;
+    if (acc[k]) {
+      const int r = lab_find_lds(L, ly * TW + lane);
+      g = (int)((int64_t)(y0 + r / TW) * W + x0 + (r % TW));
+    }
+    P[(int64_t)y * W + x] = g;
   }
 }
 
@@ -330,7 +397,9 @@ static int lab_run(const pm_image *in, int se, pm_image *out, int64_t *count, hi
     if (d.D == 1) {  // 1D / 2D: tiles of 32 x 64
       constexpr int TH = 32, TW = 64;
       const dim3 tg((unsigned)((d.W + TW - 1) / TW), (unsigned)((d.H + TH - 1) / TH), 1);
-      hipLaunchKernelGGL((lab_tile<T, FLAT, 1, TH, TW>), tg, dim3(256), 0, s, src, P, 1, (int)d.H, (int)d.W, u);
+      int xlink = 0;  // is the left neighbour (dy = 0, dx = -1) an upper offset of the SE
+      for (int j = 0; j < u.n; j++) xlink |= (u.dz[j] == 0 && u.dy[j] == 0 && u.dx[j] == -1);
+      hipLaunchKernelGGL((lab_tile2d<T, FLAT, TH>), tg, dim3(256), 0, s, src, P, (int)d.H, (int)d.W, u, xlink);
       hipLaunchKernelGGL((lab_border<T, FLAT, 1, TH, TW>), dim3(g), dim3(256), 0, s, src, P, 1, (int)d.H, (int)d.W, u);
     } else {  // 3D: tiles of 8 x 8 x 32
       constexpr int TD = 8, TH = 8, TW = 32;
