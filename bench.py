@@ -214,6 +214,39 @@ def bench_small_1k(pm, torch, dev, reps=50):
     return out
 
 
+def bench_label_4k(pm, torch, dev, reps=5, cpu=True):
+    """Connected-component labelling of a 4096^2 uint8 image (blurred-noise mask, seed 4,
+    thresholded at 128: a few thousand large winding blobs): label_binary by SESquare2D and
+    SECross2D, label_flat_zones of the mask quantised to 8 levels.  Bytes: 1 B read + 8 B
+    (int64 label) written per pixel.  cpu: the oracle's one-pass union-find on one core."""
+    import time
+    import numpy as np
+    from poutre2_amd import synth
+    H = W = 4096
+    npx = H * W
+    mask = synth.blurred_noise_mask(H, W, 4)
+    binary = (mask > 127).astype(np.uint8)
+    zones = (mask // 32).astype(np.uint8)
+    b = torch.from_numpy(binary).to(dev)
+    z = torch.from_numpy(zones).to(dev)
+    out = torch.empty((H, W), dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    res = {}
+    for name, img, fn, se in (("binary_square", b, pm.label_binary, pm.static_nl_se.square2d),
+                              ("binary_cross", b, pm.label_binary, pm.static_nl_se.cross2d),
+                              ("flat_zones_square", z, pm.label_flat_zones, pm.static_nl_se.square2d)):
+        _, n = fn(img, se, out=out)
+        t = timed(lambda: fn(img, se, out=out), reps, stream) / reps
+        res[name] = {"ms": round(t, 3), "labels": n, "gpx_s": round(npx / (t * 1e-3) / 1e9, 2),
+                     "pct_hbm": round(100 * 9 * npx / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 2)}
+    if cpu:
+        import oracle
+        t0 = time.perf_counter()
+        oracle.label(binary, "SESquare2D")
+        res["binary_square"]["cpu_oracle_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+    return res
+
+
 def bench_batched(pm, torch, dev, reps=3):
     """512 uint8 images, H, W ~ U{64..2048}, square side ~ U{3,5,..,31}: dilation + opening per
     image in one launch vs the per-image path; 256 h-maxima reconstructions (sizes
@@ -440,7 +473,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--sections", default="dilate,ops,recon,3d,types,small,batch,autograd")
+    ap.add_argument("--sections", default="dilate,ops,recon,3d,types,small,batch,autograd,label")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc passes")
     ap.add_argument("--pmc-child", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--pmc-tw", type=int, default=0, help=argparse.SUPPRESS)
@@ -484,6 +517,9 @@ def main():
     if "batch" in sections:
         log("bench: batched 512 dilate+open, 256 reconstructions")
         extra["batched"] = bench_batched(pm, torch, dev)
+    if "label" in sections:
+        log("bench: 4096^2 connected-component labelling")
+        extra["label_4k"] = bench_label_4k(pm, torch, dev, cpu=not args.no_cpu)
     if "autograd" in sections:
         log("bench: autograd [64,32,256,256] fp32")
         extra["autograd"] = bench_autograd(pm, torch, dev)
