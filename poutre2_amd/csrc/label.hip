@@ -49,9 +49,10 @@ template <typename T, bool FLAT> __device__ __forceinline__ bool lab_link(T v, T
   return lab_fg<T>(v) && lab_fg<T>(w) && w > T(0);
 }
 
-// Root of x in global memory.  No path compression here: the parents in global memory are
-// changed by atomics only while unions run (plain stores from other XCDs' caches could
-// overwrite them); lab_compress flattens the trees once the unions are done.
+// Root of x in global memory, without path compression: while unions run, the parents in
+// global memory change by atomics only (plain stores going through another XCD's cache
+// could overwrite a link), and halving by atomics measured slower (4096^2 binary 0.65 ->
+// 0.74 ms); lab_compress flattens the trees once the unions are done.
 __device__ __forceinline__ int lab_find(const int *P, int x)
 {
   int y;
@@ -213,29 +214,48 @@ __global__ __launch_bounds__(256) void lab_tile2d(const T *__restrict__ in, int 
   }
 }
 
-// the links whose two pixels lie in different tiles, merged in global memory
+// The links whose two pixels lie in different tiles, merged in global memory.  After lab_tile
+// a pixel's parent is its tile-local root, and it stays so (unions only relink roots), so a
+// link can be told by the pair (P[i], P[j]) of tile roots: when the previous lane merged the
+// same pair -- consecutive pixels along a tile's top edge mostly do -- the union is skipped.
 template <typename T, bool FLAT, int TD, int TH, int TW>
-__global__ void lab_border(const T *__restrict__ in, int *P, int D, int H, int W, LabUpper u)
+__global__ __launch_bounds__(256) void lab_border(const T *__restrict__ in, int *P, int D, int H, int W, LabUpper u)
 {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int64_t N = (int64_t)D * H * W;
-  if (i >= N) return;
-  const int x = (int)(i % W);
-  const int64_t r = i / W;
-  const int y = (int)(r % H), z = (int)(r / H);
-  const int lx = x % TW, ly = y % TH, lz = z % TD;
-  // only pixels on a tile face can have an upper neighbour in another tile (upper offsets
-  // have dz <= 0; with dz = -1 they may step +1 in y or x, with dy = -1 +1 in x)
-  if (lx != 0 && lx != TW - 1 && ly != 0 && ly != TH - 1 && !(TD > 1 && lz == 0)) return;
-  if (P[i] < 0) return;
-  const T v = in[i];
+  int x = 0, y = 0, z = 0;
+  bool cand = false;
+  if (i < N) {
+    x = (int)(i % W);
+    const int64_t r = i / W;
+    y = (int)(r % H);
+    z = (int)(r / H);
+    const int lx = x % TW, ly = y % TH, lz = z % TD;
+    // only pixels on a tile face can have an upper neighbour in another tile (upper offsets
+    // have dz <= 0; with dz = -1 they may step +1 in y or x, with dy = -1 +1 in x)
+    cand = lx == 0 || lx == TW - 1 || ly == 0 || ly == TH - 1 || (TD > 1 && lz == 0);
+  }
+  if (!__any(cand)) return;  // the whole wave: no shuffle below loses a participant
+  const int ra = cand ? P[i] : -1;
+  cand = cand && ra >= 0;
+  const T v = cand ? in[i] : T(0);
+  const int lane = threadIdx.x & 63;
   for (int k = 0; k < u.n; k++) {
-    const int nx = lx + u.dx[k], ny = ly + u.dy[k], nz = lz + u.dz[k];
-    if ((unsigned)nx < (unsigned)TW && (unsigned)ny < (unsigned)TH && (unsigned)nz < (unsigned)TD) continue;
-    const int zz = z + u.dz[k], yy = y + u.dy[k], xx = x + u.dx[k];
-    if ((unsigned)zz >= (unsigned)D || (unsigned)yy >= (unsigned)H || (unsigned)xx >= (unsigned)W) continue;
-    const int64_t j = ((int64_t)zz * H + yy) * W + xx;
-    if (lab_link<T, FLAT>(v, in[j])) lab_unite(P, (int)i, (int)j);
+    bool lk = false;
+    int rb = -1;
+    int64_t j = 0;
+    if (cand) {
+      const int zz = z + u.dz[k], yy = y + u.dy[k], xx = x + u.dx[k];
+      const bool other_tile = (zz / TD != z / TD) || (yy / TH != y / TH) || (xx / TW != x / TW) || zz < 0 || yy < 0 || xx < 0;
+      if (other_tile && (unsigned)zz < (unsigned)D && (unsigned)yy < (unsigned)H && (unsigned)xx < (unsigned)W) {
+        j = ((int64_t)zz * H + yy) * W + xx;
+        lk = lab_link<T, FLAT>(v, in[j]);
+        if (lk) rb = P[j];
+      }
+    }
+    const int pra = __shfl_up(lk ? ra : -2, 1), prb = __shfl_up(rb, 1);
+    const bool dup = lane != 0 && pra == ra && prb == rb;
+    if (lk && !dup) lab_unite(P, (int)i, (int)j);
   }
 }
 
