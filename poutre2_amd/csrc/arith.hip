@@ -150,9 +150,9 @@ template <int K> static int compose(const pm_image *in, int se, int iter, pm_ima
       const T *src = (const T *)in->data;
       T *dst = (T *)out->data;
       if constexpr (std::is_same<T, uint8_t>::value) {
-        // uint8 gradient by a square: one fused launch (no scratch image, no subtraction
-        // pass).  Above r = 15 its doubled registers leave one workgroup per CU and the
-        // composition is faster (8192^2, r = 31: 0.143 vs 0.133 ms; r = 7: 0.053 vs 0.105 ms).
+        // uint8 gradient by a square, r <= 32: one fused launch (no scratch image, no
+        // subtraction pass); 8192^2: r = 7 0.053 vs 0.105 ms for the composition, r = 31
+        // 0.109 vs 0.137 ms (per-tile thread indices keep two workgroups per CU at r >= 8).
         if (K == C_GRAD && se == PM_SE_SQUARE_2D && rank == 2 && iter <= SQ_U8_GRAD_RMAX)
           return sq_u8_gradient(src, dst, 1, d.H, d.W, iter, s);
         // opening / closing by a square: one fused launch, the eroded (dilated) image stays

@@ -1,4 +1,4 @@
-// sq_grad.hip -- fused morphological gradient of uint8 images by a (2r+1)^2 square, r <= 15:
+// sq_grad.hip -- fused morphological gradient of uint8 images by a (2r+1)^2 square, r <= 32:
 // out = Dilate(in, SESquare2D, r) - Erode(in, SESquare2D, r) in one launch (the composition
 // of ero_dil_static.cpp:229-305 with arith_saturated_sub_image; max >= min, so the
 // subtraction never saturates).  One read of the input, one write of the output; the tile

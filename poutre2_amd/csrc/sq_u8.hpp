@@ -14,7 +14,7 @@ template <bool DIL> int sq_u8(const uint8_t *in, uint8_t *out, int64_t B, int64_
 
 // Dilate - Erode by the (2r+1)^2 square on B images of H x W in one launch,
 // 1 <= r <= SQ_U8_GRAD_RMAX
-constexpr int64_t SQ_U8_GRAD_RMAX = 15;
+constexpr int64_t SQ_U8_GRAD_RMAX = 32;
 int sq_u8_gradient(const uint8_t *in, uint8_t *out, int64_t B, int64_t H, int64_t W, int64_t r, hipStream_t s);
 
 }  // namespace pm
