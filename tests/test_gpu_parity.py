@@ -91,7 +91,8 @@ def test_static_3d(dtype, se, it):
 
 @pytest.mark.parametrize("se", ["SECross3D", "SESquare3D", "SESegmentZ3D"])
 @pytest.mark.parametrize("shape,it", [((11, 13, 48), 1), ((11, 13, 48), 2), ((7, 9, 32), 3), ((20, 5, 16), 8),
-                                      ((20, 5, 16), 9), ((3, 17, 64), 4), ((1, 6, 16), 2)])
+                                      ((20, 5, 16), 9), ((3, 17, 64), 4), ((1, 6, 16), 2), ((4, 2, 32), 2),
+                                      ((2, 3, 16), 5)])
 def test_static_3d_u8_volume_kernels(se, shape, it):
     """uint8 volumes with W % 16 == 0 take the z-streaming kernels (z radius <= 8; 9 falls back)."""
     img = rand(shape, np.uint8, 5 + it)
