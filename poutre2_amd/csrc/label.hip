@@ -11,16 +11,18 @@
 // the union keeps the smaller root).  On the GPU:
 //  1. lab_tile2d / lab_tile: per 32 x 64 tile (8 x 8 x 32 in 3D), the links inside the tile
 //                   are merged in LDS (in 2D the links along a row by a ballot over the
-//                   row's runs) and every pixel gets the global index of its tile-local root;
-//  2. lab_border:   the links that cross tile borders are merged in global memory.  Both use
-//                   a lock-free union whose roots only decrease (atomicMin on the larger
-//                   root, retried from the value it returns), so each root ends as the
-//                   smallest index of its component;
-//  3. lab_compress: P[i] = root;
-//  4. ranks of the roots in raster order by a block count, a scan of the block counts and a
-//     per-block scan (lab_count / lab_scan_blocks / lab_rank);
-//  5. lab_output:   out[i] = rank of P[i] (int64, the reference's GINT64 output), 0 for the
-//                   background.
+//                   row's runs) and every pixel gets the global index of its tile-local root
+//                   (2D: each tile also lists its roots);
+//  2. the links that cross tile borders are merged in global memory with a lock-free union
+//     whose roots only decrease (atomicMin on the larger root, retried from the value it
+//     returns), so each root ends as the smallest index of its component.  2D:
+//     lab_border_level, level by level (aligned groups of 2 x 2, 4 x 4, ... tiles become
+//     one), each level followed by lab_compress_roots, which points every listed tile root
+//     at its root; 3D: lab_border in one pass, then lab_compress (P[i] = root);
+//  3. lab_count_rank: per block of LAB_RB pixels, the number of roots and each root's rank
+//     among them; lab_scan_counts: block offsets and the number of components;
+//  4. lab_output:   out[i] = 1 + rank of the root of i (int64, the reference's GINT64
+//                   output), 0 for the background.
 // Positive float values at or below the reference's 1e-10 background threshold are not
 // linked (the reference reads an unassigned provisional label for them).
 #include <cmath>
@@ -30,7 +32,11 @@
 
 namespace pm {
 
-constexpr int LAB_BLOCK = 1024;  // elements per block of the rank scan (256 threads x 4)
+constexpr int LAB_RB = 4096;  // pixels per block of the root ranking (256 threads x 16)
+constexpr int LAB_U2 = 4;     // upper offsets of a 2D neighbourhood: (-1, -1), (-1, 0), (-1, 1), (0, -1)
+
+__device__ __forceinline__ int lab_bitlen(int v) { return v ? 32 - __clz(v) : 0; }
+inline int lab_bitlen_host(int v) { return v ? 32 - __builtin_clz((unsigned)v) : 0; }
 
 struct LabUpper {
   int n;
@@ -52,7 +58,7 @@ template <typename T, bool FLAT> __device__ __forceinline__ bool lab_link(T v, T
 // Root of x in global memory, without path compression: while unions run, the parents in
 // global memory change by atomics only (plain stores going through another XCD's cache
 // could overwrite a link), and halving by atomics measured slower (4096^2 binary 0.65 ->
-// 0.74 ms); lab_compress flattens the trees once the unions are done.
+// 0.74 ms); lab_compress / lab_compress_roots flatten the trees between union passes.
 __device__ __forceinline__ int lab_find(const int *P, int x)
 {
   int y;
@@ -160,44 +166,128 @@ __global__ __launch_bounds__(256) void lab_tile(const T *__restrict__ in, int *_
   }
 }
 
-// 2D tiles of TH rows x 64 columns, one wave per row at a time (lane = column).  The link to
-// the left pixel of the same row is resolved without unions: a ballot marks the lanes that
-// start a run (not linked to their left neighbour) and every pixel points at the start of
-// its run.  Only the links to the row above (dy = -1) then go through the LDS union-find.
+// value of lane - 1 (lane 0: zero), by DPP wave_shr:1
+template <typename T> __device__ __forceinline__ T lab_from_left(T v)
+{
+  if constexpr (sizeof(T) == 8) {
+    uint32_t b[2];
+    __builtin_memcpy(b, &v, 8);
+    b[0] = (uint32_t)__builtin_amdgcn_mov_dpp((int)b[0], 0x138, 0xf, 0xf, true);
+    b[1] = (uint32_t)__builtin_amdgcn_mov_dpp((int)b[1], 0x138, 0xf, 0xf, true);
+    __builtin_memcpy(&v, b, 8);
+    return v;
+  } else {
+    uint32_t b = 0;
+    __builtin_memcpy(&b, &v, sizeof(T));
+    b = (uint32_t)__builtin_amdgcn_mov_dpp((int)b, 0x138, 0xf, 0xf, true);
+    T r;
+    __builtin_memcpy(&r, &b, sizeof(T));
+    return r;
+  }
+}
+
+// start column of the run that holds column c, given the row's mask of run starts
+__device__ __forceinline__ int lab_run_start(uint64_t starts, int c)
+{
+  const uint64_t m = c == 63 ? starts : (starts & ((2ull << c) - 1));
+  return 63 - __clzll((long long)m);
+}
+
+// 2D tiles of TH rows x 64 columns, one wave per row at a time (lane = column).
+//  1. The link to the left pixel of the same row is resolved without unions: a ballot marks
+//     the lanes that start a run (not linked to their left neighbour) and every pixel points
+//     at the start of its run (row masks of run starts and the pixel values kept in LDS).
+//  2. Links to the row above (dy = -1, dx in upm: bit dx + 1) join the pixel's run with the
+//     run above it: one LDS union per pair of runs, met first by a lane in raster order (tests
+//     on ballot masks skip the others), the owed unions of all the wave's rows run in one loop.
+//  3. Every run start is pointed at its root (no union runs meanwhile), so a pixel's root is
+//     one LDS read away: P[pixel] = global index of the root (-1 for the background).
+// The tile's roots (global indices, in no particular order) go to RL[tile * cap ...] (cap: at
+// least the tile's pixel count) and their number to TC[tile], tile = blockIdx.y * gridDim.x +
+// blockIdx.x.
 template <typename T, bool FLAT, int TH>
-__global__ __launch_bounds__(256) void lab_tile2d(const T *__restrict__ in, int *__restrict__ P, int H, int W,
-                                                  LabUpper u, int xlink)
+__global__ __launch_bounds__(256) void lab_tile2d(const T *__restrict__ in, int *__restrict__ P, int H, int W, int upm,
+                                                  int xlink, int *__restrict__ RL, int *__restrict__ TC, int cap)
 {
   constexpr int TW = 64, NT = TH * TW, RPW = TH / 4;  // rows per wave
   __shared__ int L[NT];
+  __shared__ T V[NT];
+  __shared__ uint64_t M[TH];
+  __shared__ int nroot;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH, x = x0 + lane;
+  const int64_t tile = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+  if (threadIdx.x == 0) nroot = 0;
   T v[RPW];
   bool acc[RPW];
+  int st[RPW];
+  uint64_t sm[RPW];
+  // all rows' loads in flight at once: clamped addresses, no branch around the loads
+  const int xc = x < W ? x : W - 1;
+#pragma unroll
+  for (int k = 0; k < RPW; k++) {
+    const int y = y0 + wv + 4 * k;
+    v[k] = in[(int64_t)(y < H ? y : H - 1) * W + xc];
+  }
 #pragma unroll
   for (int k = 0; k < RPW; k++) {
     const int ly = wv + 4 * k, y = y0 + ly;
     const bool in_img = x < W && y < H;
-    v[k] = in_img ? in[(int64_t)y * W + x] : T(0);
+    if (!in_img) v[k] = T(0);
     acc[k] = in_img && lab_accept<T, FLAT>(v[k]);
-    bool lk = false;
-    if (xlink && acc[k] && lane > 0) lk = lab_link<T, FLAT>(v[k], in[(int64_t)y * W + x - 1]);
-    const uint64_t starts = __ballot(!lk);
-    const uint64_t m = lane == 63 ? starts : (starts & ((2ull << lane) - 1));
-    const int st = 63 - __clzll((long long)m);
-    L[ly * TW + lane] = acc[k] ? ly * TW + st : -1;
+    const T left = lab_from_left(v[k]);
+    const bool lk = xlink && acc[k] && lane > 0 && lab_link<T, FLAT>(v[k], left);
+    sm[k] = __ballot(!lk);
+    st[k] = lab_run_start(sm[k], lane);
+    L[ly * TW + lane] = acc[k] ? ly * TW + st[k] : -1;
+    V[ly * TW + lane] = v[k];
+    if (lane == 0) M[ly] = sm[k];
   }
   __syncthreads();
+  // the unions this lane owes: bit 3k + dx + 1 for row wv + 4k and offset dx
+  uint32_t pend = 0;
 #pragma unroll
   for (int k = 0; k < RPW; k++) {
-    const int ly = wv + 4 * k, y = y0 + ly;
-    if (!acc[k] || ly == 0) continue;
-    for (int j = 0; j < u.n; j++) {
-      if (u.dz[j] != 0 || u.dy[j] != -1) continue;  // (dz != 0 only for 3D SEs, out of a D = 1 image)
-      const int nx = lane + u.dx[j];
-      if ((unsigned)nx >= (unsigned)TW || x0 + nx >= W) continue;
-      if (lab_link<T, FLAT>(v[k], in[(int64_t)(y - 1) * W + x0 + nx]))
-        lab_unite_lds(L, ly * TW + lane, (ly - 1) * TW + nx);
+    const int ly = wv + 4 * k;  // the same for the whole wave
+    if (ly == 0) continue;
+    const uint64_t above = M[ly - 1];
+    const bool start = (sm[k] >> lane) & 1;
+    bool cov = false;  // an earlier offset of this lane reached the run above that holds column lane + dx
+#pragma unroll
+    for (int dx = -1; dx <= 1; dx++) {
+      const int nx = lane + dx;
+      const bool inx = (unsigned)nx < (unsigned)TW && x0 + nx < W;
+      const bool above_start = inx && ((above >> nx) & 1);
+      cov = cov && !above_start;
+      if (!((upm >> (dx + 1)) & 1)) continue;  // the same for every lane
+      // acc: the pixel is in the image, so the row above is too
+      const bool lk = acc[k] && inx && lab_link<T, FLAT>(v[k], V[(ly - 1) * TW + (inx ? nx : lane)]);
+      const uint64_t lkm = __ballot(lk);
+      // the same pair of runs was met earlier: by lane - 1 with this offset, by lane - 1 with
+      // offset dx + 1 (the same column, and lane - 1 links to it exactly when this lane does:
+      // same run), or by this lane with a smaller offset
+      const bool dup = (lane != 0 && ((lkm >> (lane - 1)) & 1) && !start && !above_start) ||
+                       (!start && ((upm >> (dx + 2)) & 1)) || cov;
+      if (lk && !dup) pend |= 1u << (3 * k + dx + 1);
+      cov = cov || lk;
+    }
+  }
+  // unions of the pixel with the pixel above (their parents are the two run starts)
+  while (pend) {
+    const int bit = __builtin_ctz(pend);
+    pend &= pend - 1;
+    const int k = bit / 3, dx = bit - 3 * k - 1, ly = wv + 4 * k;
+    lab_unite_lds(L, ly * TW + lane, (ly - 1) * TW + lane + dx);
+  }
+  __syncthreads();
+  // run starts -> roots: a walk without halving, so the only store to L[e] is this one
+#pragma unroll
+  for (int k = 0; k < RPW; k++) {
+    const int ly = wv + 4 * k, e = ly * TW + lane;
+    if (acc[k] && st[k] == lane) {
+      int r = e, q;
+      while ((q = L[r]) != r) r = q;
+      L[e] = r;
     }
   }
   __syncthreads();
@@ -207,11 +297,14 @@ __global__ __launch_bounds__(256) void lab_tile2d(const T *__restrict__ in, int 
     if (x >= W || y >= H) continue;
     int g = -1;
     if (acc[k]) {
-      const int r = lab_find_lds(L, ly * TW + lane);
+      const int r = L[ly * TW + st[k]];
       g = (int)((int64_t)(y0 + r / TW) * W + x0 + (r % TW));
+      if (r == ly * TW + lane) RL[tile * cap + atomicAdd(&nroot, 1)] = g;
     }
     P[(int64_t)y * W + x] = g;
   }
+  __syncthreads();
+  if (threadIdx.x == 0) TC[tile] = nroot;
 }
 
 // The links whose two pixels lie in different tiles, merged in global memory.  After lab_tile
@@ -259,95 +352,213 @@ __global__ __launch_bounds__(256) void lab_border(const T *__restrict__ in, int 
   }
 }
 
+// 2D links between tiles, one level at a time.  Tiles (tx, ty) and (tx', ty') fall into one
+// aligned group of 2^l x 2^l tiles from level l = bitlen((tx ^ tx') | (ty ^ ty')) on, so a
+// link between neighbouring pixels is merged at that level.  Level l's links start on the
+// rows y = b * TH (links whose y part, bitlen(ty ^ ty'), is l) and on the columns x = b * TW
+// (dx = -1) and b * TW - 1 (dx = +1) (links whose x part is l and y part below l), b an odd
+// multiple of 2^(l-1); items: nby rows of W pixels, then nbx column pairs of 2 x H pixels.
+// Between levels lab_compress_roots points every tile root at its root, so a level's finds
+// walk a few links (pixel, tile root, group roots merged at this level).  u: the SE's upper
+// offsets in the plane (dz = 0), at most LAB_U2.
+template <typename T, bool FLAT, int TH, int TW>
+__global__ __launch_bounds__(256) void lab_border_level(const T *__restrict__ in, int *P, int H, int W, LabUpper u,
+                                                        int l, int nby, int nbx)
+{
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t nrow = (int64_t)nby * W, ncol = (int64_t)nbx * 2 * H;
+  const int h = 1 << (l - 1);
+  int x = 0, y = 0;
+  bool rowitem = false, cand = false;
+  if (t < nrow) {
+    const int k = (int)(t / W);
+    x = (int)(t - (int64_t)k * W);
+    y = h * (2 * k + 1) * TH;
+    rowitem = cand = true;
+  } else if (t < nrow + ncol) {
+    const int64_t c = t - nrow;
+    const int k = (int)(c / (2 * (int64_t)H));
+    const int rem = (int)(c - (int64_t)k * 2 * H);
+    const int side = rem >= H ? 1 : 0;
+    y = rem - side * H;
+    x = h * (2 * k + 1) * TW - side;
+    cand = true;
+  }
+  if (!__any(cand)) return;  // the whole wave: no shuffle below loses a participant
+  // every load of the item at once (lanes without an item read pixel 0): the pixel, then the
+  // value and parent of each neighbour whose link belongs to this level
+  const int64_t i = (int64_t)y * W + x;
+  const int ra = P[i];
+  const T v = in[i];
+  int64_t jn[LAB_U2];
+  bool mine[LAB_U2];
+#pragma unroll
+  for (int k = 0; k < LAB_U2; k++) {
+    mine[k] = false;
+    jn[k] = i;
+    if (k < u.n) {
+      const int yy = y + u.dy[k], xx = x + u.dx[k];
+      if ((unsigned)yy < (unsigned)H && (unsigned)xx < (unsigned)W) {
+        const int lx = lab_bitlen((x / TW) ^ (xx / TW)), ly = lab_bitlen((y / TH) ^ (yy / TH));
+        mine[k] = rowitem ? (ly == l && lx <= l) : (lx == l && ly < l);
+        if (mine[k]) jn[k] = (int64_t)yy * W + xx;
+      }
+    }
+  }
+  T w[LAB_U2];
+  int pj[LAB_U2];
+#pragma unroll
+  for (int k = 0; k < LAB_U2; k++) {
+    w[k] = in[jn[k]];
+    pj[k] = P[jn[k]];
+  }
+  cand = cand && ra >= 0;
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < LAB_U2; k++) {
+    if (k >= u.n) break;  // the same for every lane
+    const bool lk = cand && mine[k] && lab_link<T, FLAT>(v, w[k]);
+    const int rb = lk ? pj[k] : -1;
+    const int pra = __shfl_up(lk ? ra : -2, 1), prb = __shfl_up(rb, 1);
+    const bool dup = lane != 0 && pra == ra && prb == rb;
+    if (lk && !dup) lab_unite(P, ra, rb);
+  }
+}
+
+// Every tile root listed by lab_tile2d points at its root (one wave per tile).  No union runs
+// during this kernel, so its plain stores cannot overwrite a link, and every parent a find
+// reads, old or already rewritten, is an ancestor.
+__global__ __launch_bounds__(256) void lab_compress_roots(int *P, const int *__restrict__ RL, const int *__restrict__ TC,
+                                                          int ntiles, int nt)
+{
+  const int tile = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (tile >= ntiles) return;
+  const int n = TC[tile];
+  const int *rl = RL + (int64_t)tile * nt;
+  for (int k = lane; k < n; k += 64) {
+    const int r = rl[k];
+    P[r] = lab_find(P, r);
+  }
+}
+
 __global__ void lab_compress(int *P, int64_t N)
 {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i < N && P[i] >= 0) P[i] = lab_find(P, (int)i);
 }
 
-// roots (P[i] == i) per block of LAB_BLOCK elements
-__global__ __launch_bounds__(256) void lab_count(const int *__restrict__ P, int *__restrict__ cnt, int64_t N)
+// inclusive prefix sum over the lanes of a wave
+__device__ __forceinline__ int lab_wave_scan(int v)
 {
-  __shared__ int s[256];
-  const int64_t base = (int64_t)blockIdx.x * LAB_BLOCK + 4 * threadIdx.x;
-  int c = 0;
+  const int lane = threadIdx.x & 63;
 #pragma unroll
-  for (int k = 0; k < 4; k++) c += (base + k < N && P[base + k] == (int)(base + k)) ? 1 : 0;
-  s[threadIdx.x] = c;
-  __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
-    if (threadIdx.x < h) s[threadIdx.x] += s[threadIdx.x + h];
-    __syncthreads();
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(v, o);
+    if (lane >= o) v += t;
   }
-  if (threadIdx.x == 0) cnt[blockIdx.x] = s[0];
+  return v;
 }
 
-// inclusive scan of 256 values in LDS (Hillis-Steele)
-__device__ __forceinline__ int lab_scan256(int *s, int v)
+// exclusive prefix of v over the 256 threads of the block, and the block's total (ws: 4
+// ints of LDS; every thread calls it)
+__device__ __forceinline__ int lab_block_excl(int v, int *ws, int &total)
 {
-  const int t = threadIdx.x;
-  s[t] = v;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int inc = lab_wave_scan(v);
+  if (lane == 63) ws[w] = inc;
   __syncthreads();
-  for (int o = 1; o < 256; o <<= 1) {
-    const int a = t >= o ? s[t - o] : 0;
-    __syncthreads();
-    s[t] += a;
-    __syncthreads();
+  int before = 0;
+  total = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    before += k < w ? ws[k] : 0;
+    total += ws[k];
   }
-  return s[t];
+  __syncthreads();  // ws is free again
+  return before + inc - v;
 }
 
-// exclusive offsets of the block counts (one workgroup, chunks of 1024 with a running
-// carry) and the total number of labels
-__global__ __launch_bounds__(256) void lab_scan_blocks(int *cnt, int nb, int64_t *total)
+// Roots (P[i] == i) of block b = pixels [b * LAB_RB, (b+1) * LAB_RB): their number into
+// cnt[b] and each root's rank among them, in raster order, into Q[root].  Thread t owns the
+// 16 consecutive pixels from b * LAB_RB + 16 t.
+__global__ __launch_bounds__(256) void lab_count_rank(const int *__restrict__ P, int *__restrict__ Q, int *__restrict__ cnt,
+                                                      int64_t N)
 {
-  __shared__ int s[256];
-  int carry = 0;
-  for (int c0 = 0; c0 < nb; c0 += 1024) {
-    int v[4], sum = 0;
+  __shared__ int ws[4];
+  const int64_t base = (int64_t)blockIdx.x * LAB_RB + 16 * threadIdx.x;
+  int p[16];
+  if (base + 16 <= N) {
+    const int4 *q = reinterpret_cast<const int4 *>(P + base);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      const int b = c0 + 4 * threadIdx.x + k;
-      v[k] = b < nb ? cnt[b] : 0;
+      const int4 w = q[k];
+      p[4 * k] = w.x;
+      p[4 * k + 1] = w.y;
+      p[4 * k + 2] = w.z;
+      p[4 * k + 3] = w.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 16; k++) p[k] = base + k < N ? P[base + k] : -1;
+  }
+  unsigned f = 0;
+#pragma unroll
+  for (int k = 0; k < 16; k++) f |= (p[k] == (int)(base + k) ? 1u : 0u) << k;
+  int total;
+  int run = lab_block_excl(__popc(f), ws, total);
+#pragma unroll
+  for (int k = 0; k < 16; k++)
+    if ((f >> k) & 1u) Q[base + k] = run++;
+  if (threadIdx.x == 0) cnt[blockIdx.x] = total;
+}
+
+// cnt[b] := number of roots in blocks before b (one workgroup, 4096 counts per step with a
+// running carry); *total := number of components
+__global__ __launch_bounds__(256) void lab_scan_counts(int *cnt, int nb, int64_t *total)
+{
+  __shared__ int ws[4];
+  int carry = 0;
+  for (int c0 = 0; c0 < nb; c0 += 256 * 16) {
+    const int b0 = c0 + 16 * threadIdx.x;
+    int v[16], sum = 0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      v[k] = b0 + k < nb ? cnt[b0 + k] : 0;
       sum += v[k];
     }
-    const int incl = lab_scan256(s, sum);
-    int run = carry + incl - sum;
+    int step;
+    int run = carry + lab_block_excl(sum, ws, step);
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const int b = c0 + 4 * threadIdx.x + k;
-      if (b < nb) cnt[b] = run;
+    for (int k = 0; k < 16; k++) {
+      if (b0 + k < nb) cnt[b0 + k] = run;
       run += v[k];
     }
-    carry += s[255];
-    __syncthreads();
+    carry += step;
   }
   if (threadIdx.x == 0) *total = carry;
 }
 
-// rank (1-based, raster order) of every root into R
-__global__ __launch_bounds__(256) void lab_rank(const int *__restrict__ P, const int *__restrict__ off,
-                                                int *__restrict__ R, int64_t N)
+// out[i] = 1 + rank of the root of i in raster order (off: lab_scan_counts' block offsets,
+// Q: lab_count_rank's ranks within a block), 0 for the background.  Each thread writes 4
+// pixels 256 apart, so every store instruction covers 512 contiguous bytes.
+__global__ __launch_bounds__(256) void lab_output(const int *__restrict__ P, const int *__restrict__ Q,
+                                                  const int *__restrict__ off, int64_t *__restrict__ out, int64_t N)
 {
-  __shared__ int s[256];
-  const int64_t base = (int64_t)blockIdx.x * LAB_BLOCK + 4 * threadIdx.x;
-  bool f[4];
-  int c = 0;
+  const int64_t b = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+  int p[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) p[k] = b + 256 * k < N ? P[b + 256 * k] : -1;
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    f[k] = base + k < N && P[base + k] == (int)(base + k);
-    c += f[k] ? 1 : 0;
+    const int64_t i = b + 256 * k;
+    if (i >= N) break;
+    int64_t o = 0;
+    if (p[k] >= 0) {
+      const int r = lab_find(P, p[k]);
+      o = (int64_t)off[r / LAB_RB] + Q[r] + 1;
+    }
+    out[i] = o;
   }
-  int run = off[blockIdx.x] + lab_scan256(s, c) - c;
-#pragma unroll
-  for (int k = 0; k < 4; k++)
-    if (f[k]) R[base + k] = ++run;
-}
-
-__global__ void lab_output(const int *__restrict__ P, const int *__restrict__ R, int64_t *__restrict__ out, int64_t N)
-{
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i < N) out[i] = P[i] >= 0 ? (int64_t)R[P[i]] : 0;
 }
 
 // upper (raster-earlier) neighbours of a Common_NL_SE (neighbor_list_static_se_t.hpp
@@ -402,41 +613,73 @@ static int lab_run(const pm_image *in, int se, pm_image *out, int64_t *count, hi
   LabUpper u;
   PM_TRY(lab_upper(se, u));
   const Dims d = dims3(*in);
-  const int64_t nb = (N + LAB_BLOCK - 1) / LAB_BLOCK;
+  const int64_t nb = (N + LAB_RB - 1) / LAB_RB;
+  // 2D tiles of 32 x 64 pixels; each lists its roots in `cap` slots (at most its pixel count)
+  constexpr int TH2 = 32, TW2 = 64;
+  const bool flat2d = d.D == 1;
+  const int64_t tx = (d.W + TW2 - 1) / TW2, ty = (d.H + TH2 - 1) / TH2, nt2 = tx * ty;
+  const int cap = (int)((d.H < TH2 ? d.H : TH2) * (d.W < TW2 ? d.W : TW2));
+  const int64_t rn = flat2d && nt2 * cap > N ? nt2 * cap : N;  // R: root lists, then ranks
+  auto al = [](int64_t n) { return ((size_t)n * sizeof(int) + 255) & ~(size_t)255; };
+  const size_t pb = al(N), rb = al(rn), tb = flat2d ? al(nt2) : 0, cb = al(nb);
   Scratch sc;
-  const size_t pb = ((size_t)N * sizeof(int) + 255) & ~(size_t)255, cb = ((size_t)nb * sizeof(int) + 255) & ~(size_t)255;
-  PM_TRY(sc.alloc(2 * pb + cb + 256, s));
-  int *P = (int *)sc.ptr;
-  int *R = (int *)((char *)sc.ptr + pb);
-  int *cnt = (int *)((char *)sc.ptr + 2 * pb);
-  int64_t *total = (int64_t *)((char *)sc.ptr + 2 * pb + cb);
+  PM_TRY(sc.alloc(pb + rb + tb + cb + 256, s));
+  char *base = (char *)sc.ptr;
+  int *P = (int *)base;
+  int *R = (int *)(base + pb);
+  int *TC = (int *)(base + pb + rb);
+  int *cnt = (int *)(base + pb + rb + tb);
+  int64_t *total = (int64_t *)(base + pb + rb + tb + cb);
   const unsigned g = (unsigned)((N + 255) / 256);
   auto f = [&](auto tag) -> int {
     using T = typename decltype(tag)::type;
     const T *src = (const T *)in->data;
-    if (d.D == 1) {  // 1D / 2D: tiles of 32 x 64
-      constexpr int TH = 32, TW = 64;
-      const dim3 tg((unsigned)((d.W + TW - 1) / TW), (unsigned)((d.H + TH - 1) / TH), 1);
+    if (flat2d) {  // 1D / 2D
+      const dim3 tg((unsigned)tx, (unsigned)ty, 1);
       int xlink = 0;  // is the left neighbour (dy = 0, dx = -1) an upper offset of the SE
       for (int j = 0; j < u.n; j++) xlink |= (u.dz[j] == 0 && u.dy[j] == 0 && u.dx[j] == -1);
-      hipLaunchKernelGGL((lab_tile2d<T, FLAT, TH>), tg, dim3(256), 0, s, src, P, (int)d.H, (int)d.W, u, xlink);
-      hipLaunchKernelGGL((lab_border<T, FLAT, 1, TH, TW>), dim3(g), dim3(256), 0, s, src, P, 1, (int)d.H, (int)d.W, u);
+      int upm = 0;  // dx + 1 of each upper offset into the row above
+      for (int j = 0; j < u.n; j++)
+        if (u.dz[j] == 0 && u.dy[j] == -1) upm |= 1 << (u.dx[j] + 1);
+      hipLaunchKernelGGL((lab_tile2d<T, FLAT, TH2>), tg, dim3(256), 0, s, src, P, (int)d.H, (int)d.W, upm, xlink, R,
+                         TC, cap);
+      LabUpper u2;  // the offsets in the plane
+      u2.n = 0;
+      for (int j = 0; j < u.n; j++)
+        if (u.dz[j] == 0) {
+          if (u2.n == LAB_U2) PM_FAIL(PM_E_ARG, "%s: more than %d upper offsets in the plane", what, LAB_U2);
+          u2.dz[u2.n] = 0;
+          u2.dy[u2.n] = u.dy[j];
+          u2.dx[u2.n] = u.dx[j];
+          u2.n++;
+        }
+      const int levels = lab_bitlen_host((int)((tx - 1) | (ty - 1)));
+      for (int l = 1; l <= levels; l++) {
+        const int64_t h = (int64_t)1 << (l - 1);
+        const int64_t nby = ((ty - 1) / h + 1) / 2, nbx = ((tx - 1) / h + 1) / 2;
+        const int64_t items = nby * d.W + nbx * 2 * d.H;
+        if (items == 0) continue;
+        hipLaunchKernelGGL((lab_border_level<T, FLAT, TH2, TW2>), dim3((unsigned)((items + 255) / 256)), dim3(256), 0, s,
+                           src, P, (int)d.H, (int)d.W, u2, l, (int)nby, (int)nbx);
+        hipLaunchKernelGGL(lab_compress_roots, dim3((unsigned)((nt2 + 3) / 4)), dim3(256), 0, s, P, R, TC, (int)nt2,
+                           cap);
+      }
     } else {  // 3D: tiles of 8 x 8 x 32
       constexpr int TD = 8, TH = 8, TW = 32;
       const dim3 tg((unsigned)((d.W + TW - 1) / TW), (unsigned)((d.H + TH - 1) / TH), (unsigned)((d.D + TD - 1) / TD));
       hipLaunchKernelGGL((lab_tile<T, FLAT, TD, TH, TW>), tg, dim3(256), 0, s, src, P, (int)d.D, (int)d.H, (int)d.W, u);
       hipLaunchKernelGGL((lab_border<T, FLAT, TD, TH, TW>), dim3(g), dim3(256), 0, s, src, P, (int)d.D, (int)d.H,
                          (int)d.W, u);
+      hipLaunchKernelGGL(lab_compress, dim3(g), dim3(256), 0, s, P, N);
     }
     PM_HIP(hipGetLastError());
     return PM_OK;
   };
   PM_TRY(dispatch_ptype(in->ptype, f));
-  hipLaunchKernelGGL(lab_compress, dim3(g), dim3(256), 0, s, P, N);
-  hipLaunchKernelGGL(lab_count, dim3((unsigned)nb), dim3(256), 0, s, P, cnt, N);
-  hipLaunchKernelGGL(lab_scan_blocks, dim3(1), dim3(256), 0, s, cnt, (int)nb, total);
-  hipLaunchKernelGGL(lab_rank, dim3((unsigned)nb), dim3(256), 0, s, P, cnt, R, N);
-  hipLaunchKernelGGL(lab_output, dim3(g), dim3(256), 0, s, P, R, (int64_t *)out->data, N);
+  hipLaunchKernelGGL(lab_count_rank, dim3((unsigned)nb), dim3(256), 0, s, P, R, cnt, N);
+  hipLaunchKernelGGL(lab_scan_counts, dim3(1), dim3(256), 0, s, cnt, (int)nb, total);
+  hipLaunchKernelGGL(lab_output, dim3((unsigned)((N + 1023) / 1024)), dim3(256), 0, s, P, R, cnt,
+                     (int64_t *)out->data, N);
   PM_HIP(hipGetLastError());
   if (count) {
     PM_HIP(hipMemcpyAsync(count, total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
