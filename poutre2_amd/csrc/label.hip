@@ -16,7 +16,7 @@
 //  2. the links that cross tile borders are merged in global memory with a lock-free union
 //     whose roots only decrease (atomicMin on the larger root, retried from the value it
 //     returns), so each root ends as the smallest index of its component.  2D:
-//     lab_border_level, level by level (aligned groups of 2 x 2, 4 x 4, ... tiles become
+//     lab_border_level, level by level (aligned groups of 4 x 4, 16 x 16, ... tiles become
 //     one), each level followed by lab_compress_roots, which points every listed tile root
 //     at its root; 3D: lab_border in one pass, then lab_compress (P[i] = root);
 //  3. lab_count_rank: per block of LAB_RB pixels, the number of roots and each root's rank
@@ -35,7 +35,12 @@ namespace pm {
 constexpr int LAB_RB = 4096;  // pixels per block of the root ranking (256 threads x 16)
 constexpr int LAB_U2 = 4;     // upper offsets of a 2D neighbourhood: (-1, -1), (-1, 0), (-1, 1), (0, -1)
 
-__device__ __forceinline__ int lab_bitlen(int v) { return v ? 32 - __clz(v) : 0; }
+constexpr int LAB_LS = 2;     // tile-coordinate bits merged per border level (groups of 4 x 4)
+
+// level at which two tiles whose coordinates differ by the bits of v (x or y) become one group
+__device__ __forceinline__ int lab_level(int v) { return ((v ? 32 - __clz(v) : 0) + LAB_LS - 1) / LAB_LS; }
+// the k-th multiple m >= 1 that is not a multiple of 2^LAB_LS (boundary b = m << hb of a level)
+__device__ __forceinline__ int lab_kth_boundary(int k) { return k + k / ((1 << LAB_LS) - 1) + 1; }
 inline int lab_bitlen_host(int v) { return v ? 32 - __builtin_clz((unsigned)v) : 0; }
 
 struct LabUpper {
@@ -197,15 +202,16 @@ __device__ __forceinline__ int lab_run_start(uint64_t starts, int c)
 //  1. The link to the left pixel of the same row is resolved without unions: a ballot marks
 //     the lanes that start a run (not linked to their left neighbour) and every pixel points
 //     at the start of its run (row masks of run starts and the pixel values kept in LDS).
-//  2. Links to the row above (dy = -1, dx in upm: bit dx + 1) join the pixel's run with the
-//     run above it: one LDS union per pair of runs, met first by a lane in raster order (tests
-//     on ballot masks skip the others), the owed unions of all the wave's rows run in one loop.
+//  2. Links to the row above (dy = -1, dx in the mask UPM, or upm when UPM < 0: bit dx + 1)
+//     join the pixel's run with the run above it: one LDS union per pair of runs, met first
+//     by a lane in raster order (tests on ballot masks skip the others), the owed unions of
+//     all the wave's rows run in one loop.
 //  3. Every run start is pointed at its root (no union runs meanwhile), so a pixel's root is
 //     one LDS read away: P[pixel] = global index of the root (-1 for the background).
 // The tile's roots (global indices, in no particular order) go to RL[tile * cap ...] (cap: at
 // least the tile's pixel count) and their number to TC[tile], tile = blockIdx.y * gridDim.x +
 // blockIdx.x.
-template <typename T, bool FLAT, int TH>
+template <typename T, bool FLAT, int TH, int UPM>
 __global__ __launch_bounds__(256) void lab_tile2d(const T *__restrict__ in, int *__restrict__ P, int H, int W, int upm,
                                                   int xlink, int *__restrict__ RL, int *__restrict__ TC, int cap)
 {
@@ -244,7 +250,9 @@ __global__ __launch_bounds__(256) void lab_tile2d(const T *__restrict__ in, int 
     if (lane == 0) M[ly] = sm[k];
   }
   __syncthreads();
-  // the unions this lane owes: bit 3k + dx + 1 for row wv + 4k and offset dx
+  // the unions this lane owes: bit 3k + dx + 1 for row wv + 4k and offset dx (bitwise
+  // logic, no branch: every lane reads V at a clamped column)
+  const int um = UPM >= 0 ? UPM : upm;
   uint32_t pend = 0;
 #pragma unroll
   for (int k = 0; k < RPW; k++) {
@@ -256,20 +264,21 @@ __global__ __launch_bounds__(256) void lab_tile2d(const T *__restrict__ in, int 
 #pragma unroll
     for (int dx = -1; dx <= 1; dx++) {
       const int nx = lane + dx;
-      const bool inx = (unsigned)nx < (unsigned)TW && x0 + nx < W;
-      const bool above_start = inx && ((above >> nx) & 1);
-      cov = cov && !above_start;
-      if (!((upm >> (dx + 1)) & 1)) continue;  // the same for every lane
+      const bool inx = ((unsigned)nx < (unsigned)TW) & (x0 + nx < W);
+      const int nxc = inx ? nx : lane;
+      const bool above_start = inx & (((above >> nxc) & 1) != 0);
+      cov = cov & !above_start;
+      if (!((um >> (dx + 1)) & 1)) continue;  // the same for every lane
       // acc: the pixel is in the image, so the row above is too
-      const bool lk = acc[k] && inx && lab_link<T, FLAT>(v[k], V[(ly - 1) * TW + (inx ? nx : lane)]);
+      const bool lk = acc[k] & inx & lab_link<T, FLAT>(v[k], V[(ly - 1) * TW + nxc]);
       const uint64_t lkm = __ballot(lk);
       // the same pair of runs was met earlier: by lane - 1 with this offset, by lane - 1 with
       // offset dx + 1 (the same column, and lane - 1 links to it exactly when this lane does:
       // same run), or by this lane with a smaller offset
-      const bool dup = (lane != 0 && ((lkm >> (lane - 1)) & 1) && !start && !above_start) ||
-                       (!start && ((upm >> (dx + 2)) & 1)) || cov;
-      if (lk && !dup) pend |= 1u << (3 * k + dx + 1);
-      cov = cov || lk;
+      const bool left_lk = (((lkm << 1) >> lane) & 1) != 0;  // lane - 1 links (0 for lane 0)
+      const bool dup = (left_lk & !start & !above_start) | (!start & (((um >> (dx + 2)) & 1) != 0)) | cov;
+      pend |= (uint32_t)(lk & !dup) << (3 * k + dx + 1);
+      cov = cov | lk;
     }
   }
   // unions of the pixel with the pixel above (their parents are the two run starts)
@@ -291,17 +300,29 @@ __global__ __launch_bounds__(256) void lab_tile2d(const T *__restrict__ in, int 
     }
   }
   __syncthreads();
+  // global index of each pixel's root (N < 2^31: int arithmetic); the wave's roots get their
+  // list slots from one LDS atomic
+  int g[RPW];
+  uint64_t rm[RPW];
+  int nr = 0;
 #pragma unroll
   for (int k = 0; k < RPW; k++) {
-    const int ly = wv + 4 * k, y = y0 + ly;
-    if (x >= W || y >= H) continue;
-    int g = -1;
-    if (acc[k]) {
-      const int r = L[ly * TW + st[k]];
-      g = (int)((int64_t)(y0 + r / TW) * W + x0 + (r % TW));
-      if (r == ly * TW + lane) RL[tile * cap + atomicAdd(&nroot, 1)] = g;
-    }
-    P[(int64_t)y * W + x] = g;
+    const int ly = wv + 4 * k;
+    const unsigned r = (unsigned)L[ly * TW + st[k]];
+    g[k] = acc[k] ? (y0 + (int)(r >> 6)) * W + x0 + (int)(r & 63) : -1;
+    rm[k] = __ballot(acc[k] && r == (unsigned)(ly * TW + lane));
+    nr += __popcll(rm[k]);
+  }
+  int base = 0;
+  if (lane == 0 && nr) base = atomicAdd(&nroot, nr);
+  base = __builtin_amdgcn_readfirstlane(base);
+  const uint64_t below = (1ull << lane) - 1;
+#pragma unroll
+  for (int k = 0; k < RPW; k++) {
+    if ((rm[k] >> lane) & 1) RL[tile * cap + base + __popcll(rm[k] & below)] = g[k];
+    base += __popcll(rm[k]);
+    const int y = y0 + wv + 4 * k;
+    if (x < W && y < H) P[(int64_t)y * W + x] = g[k];
   }
   __syncthreads();
   if (threadIdx.x == 0) TC[tile] = nroot;
@@ -352,28 +373,29 @@ __global__ __launch_bounds__(256) void lab_border(const T *__restrict__ in, int 
   }
 }
 
-// 2D links between tiles, one level at a time.  Tiles (tx, ty) and (tx', ty') fall into one
-// aligned group of 2^l x 2^l tiles from level l = bitlen((tx ^ tx') | (ty ^ ty')) on, so a
-// link between neighbouring pixels is merged at that level.  Level l's links start on the
-// rows y = b * TH (links whose y part, bitlen(ty ^ ty'), is l) and on the columns x = b * TW
-// (dx = -1) and b * TW - 1 (dx = +1) (links whose x part is l and y part below l), b an odd
-// multiple of 2^(l-1); items: nby rows of W pixels, then nbx column pairs of 2 x H pixels.
-// Between levels lab_compress_roots points every tile root at its root, so a level's finds
-// walk a few links (pixel, tile root, group roots merged at this level).  u: the SE's upper
-// offsets in the plane (dz = 0), at most LAB_U2.
+// 2D links between tiles, one level at a time.  Level l joins the aligned groups of
+// G^(l-1) x G^(l-1) tiles into groups of G^l x G^l (G = 2^LAB_LS), so tiles (tx, ty) and
+// (tx', ty') meet at level lab_level((tx ^ tx') | (ty ^ ty')) = ceil(bitlen / LAB_LS), and a link
+// between neighbouring pixels is merged at that level.  Level l's links start on the rows
+// y = b * TH (links whose y part is level l) and on the columns x = b * TW (dx = -1) and
+// b * TW - 1 (dx = +1) (links whose x part is level l and y part below l), b a boundary whose
+// lowest set bit lies in [(l-1) LAB_LS, l LAB_LS); items: nby rows of W pixels, then nbx column
+// pairs of 2 x H pixels.  Between levels lab_compress_roots points every tile root at its
+// root, so a level's finds walk a few links (pixel, tile root, group roots merged at this
+// level).  u: the SE's upper offsets in the plane (dz = 0), at most LAB_U2.
 template <typename T, bool FLAT, int TH, int TW>
 __global__ __launch_bounds__(256) void lab_border_level(const T *__restrict__ in, int *P, int H, int W, LabUpper u,
                                                         int l, int nby, int nbx)
 {
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int64_t nrow = (int64_t)nby * W, ncol = (int64_t)nbx * 2 * H;
-  const int h = 1 << (l - 1);
+  const int hb = (l - 1) * LAB_LS;
   int x = 0, y = 0;
   bool rowitem = false, cand = false;
   if (t < nrow) {
     const int k = (int)(t / W);
     x = (int)(t - (int64_t)k * W);
-    y = h * (2 * k + 1) * TH;
+    y = (lab_kth_boundary(k) << hb) * TH;
     rowitem = cand = true;
   } else if (t < nrow + ncol) {
     const int64_t c = t - nrow;
@@ -381,7 +403,7 @@ __global__ __launch_bounds__(256) void lab_border_level(const T *__restrict__ in
     const int rem = (int)(c - (int64_t)k * 2 * H);
     const int side = rem >= H ? 1 : 0;
     y = rem - side * H;
-    x = h * (2 * k + 1) * TW - side;
+    x = (lab_kth_boundary(k) << hb) * TW - side;
     cand = true;
   }
   if (!__any(cand)) return;  // the whole wave: no shuffle below loses a participant
@@ -399,7 +421,7 @@ __global__ __launch_bounds__(256) void lab_border_level(const T *__restrict__ in
     if (k < u.n) {
       const int yy = y + u.dy[k], xx = x + u.dx[k];
       if ((unsigned)yy < (unsigned)H && (unsigned)xx < (unsigned)W) {
-        const int lx = lab_bitlen((x / TW) ^ (xx / TW)), ly = lab_bitlen((y / TH) ^ (yy / TH));
+        const int lx = lab_level((x / TW) ^ (xx / TW)), ly = lab_level((y / TH) ^ (yy / TH));
         mine[k] = rowitem ? (ly == l && lx <= l) : (lx == l && ly < l);
         if (mine[k]) jn[k] = (int64_t)yy * W + xx;
       }
@@ -641,8 +663,12 @@ static int lab_run(const pm_image *in, int se, pm_image *out, int64_t *count, hi
       int upm = 0;  // dx + 1 of each upper offset into the row above
       for (int j = 0; j < u.n; j++)
         if (u.dz[j] == 0 && u.dy[j] == -1) upm |= 1 << (u.dx[j] + 1);
-      hipLaunchKernelGGL((lab_tile2d<T, FLAT, TH2>), tg, dim3(256), 0, s, src, P, (int)d.H, (int)d.W, upm, xlink, R,
-                         TC, cap);
+      // the masks of the static SEs get their own kernels (square: 7, cross / segment y: 2, none)
+      auto tk = upm == 7 ? lab_tile2d<T, FLAT, TH2, 7>
+                : upm == 2 ? lab_tile2d<T, FLAT, TH2, 2>
+                : upm == 0 ? lab_tile2d<T, FLAT, TH2, 0>
+                           : lab_tile2d<T, FLAT, TH2, -1>;
+      hipLaunchKernelGGL(tk, tg, dim3(256), 0, s, src, P, (int)d.H, (int)d.W, upm, xlink, R, TC, cap);
       LabUpper u2;  // the offsets in the plane
       u2.n = 0;
       for (int j = 0; j < u.n; j++)
@@ -653,10 +679,11 @@ static int lab_run(const pm_image *in, int se, pm_image *out, int64_t *count, hi
           u2.dx[u2.n] = u.dx[j];
           u2.n++;
         }
-      const int levels = lab_bitlen_host((int)((tx - 1) | (ty - 1)));
+      const int levels = (lab_bitlen_host((int)((tx - 1) | (ty - 1))) + LAB_LS - 1) / LAB_LS;
       for (int l = 1; l <= levels; l++) {
-        const int64_t h = (int64_t)1 << (l - 1);
-        const int64_t nby = ((ty - 1) / h + 1) / 2, nbx = ((tx - 1) / h + 1) / 2;
+        const int hb = (l - 1) * LAB_LS;  // boundaries b of level l: lowest set bit in [hb, hb + LAB_LS)
+        const int64_t nby = ((ty - 1) >> hb) - ((ty - 1) >> (hb + LAB_LS));
+        const int64_t nbx = ((tx - 1) >> hb) - ((tx - 1) >> (hb + LAB_LS));
         const int64_t items = nby * d.W + nbx * 2 * d.H;
         if (items == 0) continue;
         hipLaunchKernelGGL((lab_border_level<T, FLAT, TH2, TW2>), dim3((unsigned)((items + 255) / 256)), dim3(256), 0, s,
@@ -682,8 +709,13 @@ static int lab_run(const pm_image *in, int se, pm_image *out, int64_t *count, hi
                      (int64_t *)out->data, N);
   PM_HIP(hipGetLastError());
   if (count) {
-    PM_HIP(hipMemcpyAsync(count, total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    // through a pinned slot (allocated once per host thread): a copy into pageable memory is
+    // staged by the runtime and costs tens of microseconds
+    static thread_local int64_t *hcount = nullptr;
+    if (!hcount) PM_HIP(hipHostMalloc((void **)&hcount, sizeof(int64_t), hipHostMallocDefault));
+    PM_HIP(hipMemcpyAsync(hcount, total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     PM_HIP(hipStreamSynchronize(s));
+    *count = *hcount;
   }
   return PM_OK;
 }
