@@ -465,6 +465,47 @@ def test_label_binary_large_blobs(shape):
         assert n == m
 
 
+def _serpentine(H, W):
+    """one 1-pixel-wide path through every row pair: rows 0, 2, 4, ... are foreground and
+    joined alternately at the right and the left end"""
+    img = np.zeros((H, W), np.uint8)
+    img[0::2, :] = 1
+    for y in range(1, H, 2):
+        img[y, W - 1 if (y // 2) % 2 == 0 else 0] = 1
+    return img
+
+
+@pytest.mark.parametrize("pattern,shape", [("serpentine", (1001, 2100)), ("comb", (900, 4100)),
+                                           ("checker", (777, 1100)), ("blobs", (2048, 4100))])
+def test_label_chains_across_tiles(pattern, shape):
+    """components that cross many tiles in both directions (every border level and partial
+    tiles): one path snaking through the whole image, vertical bars joined by one row at
+    the bottom, a checkerboard (one diagonal component by SESquare2D, isolated pixels by
+    SECross2D), large blobs; binary and flat zones"""
+    from poutre2_amd import synth
+    H, W = shape
+    if pattern == "serpentine":
+        img = _serpentine(H, W)
+    elif pattern == "comb":
+        img = np.zeros(shape, np.uint8)
+        img[:, ::3] = 1
+        img[H - 1, :] = 1
+    elif pattern == "checker":
+        img = ((np.arange(H)[:, None] + np.arange(W)[None, :]) % 2).astype(np.uint8)
+    else:
+        img = (synth.blurred_noise_mask(H, W, 11) > 127).astype(np.uint8)
+    for se in ("SESquare2D", "SECross2D"):
+        out, n = pm.label_binary(g(img), PM_SE[se])
+        exp, m = oracle.label(img, se)
+        assert n == m, (se, n, m)
+        np.testing.assert_array_equal(h(out), exp)
+    zones = (img * 3 + (np.arange(W)[None, :] // 700).astype(np.uint8)).astype(np.uint8)
+    out, n = pm.label_flat_zones(g(zones), PM_SE["SESquare2D"])
+    exp, m = oracle.label(zones, "SESquare2D", flat=True)
+    assert n == m
+    np.testing.assert_array_equal(h(out), exp)
+
+
 @pytest.mark.parametrize("op", ["label_minima", "label_maxima"])
 def test_label_extrema(op):
     img = rand((120, 97), np.int32, 210, 0, 6)
