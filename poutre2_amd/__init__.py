@@ -13,7 +13,10 @@ Differences from pypoutre, all additive:
   * ``h_minima`` calls h_minima (the reference binds h_maxima under that name,
     python/geodesy/geodesy.cpp:19);
   * ``reconstruction`` is bound (the reference has no binding for it);
-  * ``opening`` / ``closing`` / ``gradient`` / ``erode_rect`` / ``dilate_rect`` are added.
+  * ``opening`` / ``closing`` / ``gradient`` / ``erode_rect`` / ``dilate_rect`` are added;
+  * images may also be any object exporting DLPack (``__dlpack__``), used in place (zero
+    copy); ``factory_image`` / ``from_string`` return device tensors and ``to_string``
+    accepts them (python/base/image_interface.cpp:13-31 binds no buffer interop at all).
 """
 import ctypes
 import enum
@@ -28,7 +31,8 @@ __all__ = ["static_nl_se", "static_compound_se", "RecType", "erode", "dilate", "
            "h_minima", "h_maxima", "h_concave", "h_convex", "dynamic_pseudo_closing", "dynamic_pseudo_opening",
            "high_leveling", "low_leveling", "leveling", "arith_sup", "arith_inf", "arith_ssub_image",
            "arith_ssub_const", "arith_sadd_const", "batch_dilate_open", "batch_reconstruction", "PoutreError",
-           "version"]
+           "version", "PType", "CompoundType", "factory_image", "from_string", "to_string", "as_sizes_compatible",
+           "as_types_compatible", "are_images_different"]
 
 
 class static_nl_se(enum.IntEnum):
@@ -82,7 +86,18 @@ def _ptype(t):
     return p
 
 
+def _img(x):
+    """x as a torch tensor: itself, or a zero-copy view of a DLPack producer."""
+    torch = _torch()
+    if isinstance(x, torch.Tensor):
+        return x
+    if hasattr(x, "__dlpack__"):
+        return torch.from_dlpack(x)
+    raise PoutreError(-1, f"images must be torch tensors or DLPack producers, not {type(x).__name__}")
+
+
 def _desc(t):
+    t = _img(t)
     if not t.is_cuda:
         raise PoutreError(-6, "images must be device tensors (HBM-resident)")
     if not t.is_contiguous():
@@ -128,11 +143,11 @@ def _rec_ptr(rec):
 
 
 def _stream(t):
-    return ctypes.c_void_p(_torch().cuda.current_stream(t.device).cuda_stream)
+    return ctypes.c_void_p(_torch().cuda.current_stream(_img(t).device).cuda_stream)
 
 
 def _out(img, out):
-    return _torch().empty_like(img) if out is None else out
+    return _torch().empty_like(_img(img)) if out is None else out
 
 
 def _call(name, *args):
@@ -141,6 +156,93 @@ def _call(name, *args):
 
 def version():
     return _capi.lib().pm_version().decode()
+
+
+# ---------------------------------------------------------------- image model (python/base)
+class PType(enum.IntEnum):
+    """poutre::PType bit values (include/poutre/base/types.hpp:517-527, python/base/types.cpp:10-17);
+    GrayUINT16 is an addition."""
+    GrayUINT8 = 1 << 2
+    GrayINT32 = 1 << 3
+    F32 = 1 << 4
+    GrayINT64 = 1 << 5
+    D64 = 1 << 6
+    GrayUINT16 = 1 << 7
+
+
+# poutre::CompoundType (types.hpp; python/base/types.cpp:19-23 names "Scalar", "3Planes", "4Planes")
+CompoundType = enum.IntEnum("CompoundType", [("Scalar", 1), ("3Planes", 2), ("4Planes", 4)])
+
+_PT_INFO = {PType.GrayUINT8: ("GUINT8", np.uint8), PType.GrayUINT16: ("GUINT16", np.uint16),
+            PType.GrayINT32: ("GINT32", np.int32), PType.GrayINT64: ("GINT64", np.int64),
+            PType.F32: ("F32", np.float32), PType.D64: ("D64", np.float64)}
+
+
+def _torch_dtype(np_dtype):
+    torch = _torch()
+    return {np.uint8: torch.uint8, np.uint16: getattr(torch, "uint16", None), np.int32: torch.int32,
+            np.int64: torch.int64, np.float32: torch.float32, np.float64: torch.float64}[np_dtype]
+
+
+def factory_image(dims, ctype, ptype, device="cuda"):
+    """poutre::Create(dims, ctype, ptype) (python/base/image_interface.cpp:27): a zero-filled
+    image of that shape and type, as a device tensor.  Only Scalar images exist here."""
+    if CompoundType(ctype) != CompoundType.Scalar:
+        raise PoutreError(-2, "Create: only scalar images are supported")
+    dims = tuple(int(d) for d in dims)
+    if len(dims) < 1 or len(dims) > 3:
+        raise PoutreError(-4, f"Unsupported number of dims {len(dims)}")
+    return _torch().zeros(dims, dtype=_torch_dtype(_PT_INFO[PType(ptype)][1]), device=device)
+
+
+def from_string(text, device="cuda"):
+    """poutre::ImageFromString (python/base/image_interface.cpp:29): "Scalar <PType> <rank>
+    <dims...> <values...>", values read as numbers and converted to the pixel type (trailing
+    values are ignored)."""
+    tok = text.split()
+    if len(tok) < 3 or tok[0] != "Scalar":
+        raise PoutreError(-1, "ImageFromString: only Scalar images are supported")
+    names = {v[0]: k for k, v in _PT_INFO.items()}
+    if tok[1] not in names:
+        raise PoutreError(-2, f"ImageFromString: unknown PType {tok[1]}")
+    pt, rank = names[tok[1]], int(tok[2])
+    if rank < 1 or rank > 3:
+        raise PoutreError(-4, f"Unsupported number of dims {rank}")
+    dims = [int(v) for v in tok[3:3 + rank]]
+    n = int(np.prod(dims))
+    vals = tok[3 + rank:3 + rank + n]
+    if len(dims) != rank or len(vals) < n:
+        raise PoutreError(-1, "ImageFromString: not enough values")
+    arr = np.array([float(v) for v in vals], dtype=np.float64).astype(_PT_INFO[pt][1]).reshape(dims)
+    return _torch().from_numpy(arr).to(device)
+
+
+def to_string(img):
+    """poutre::ImageToString (python/base/image_interface.cpp:30): the from_string format;
+    floats printed with 6 significant digits (the C++ stream default)."""
+    t = _img(img)
+    pt = PType(_ptype(t))
+    a = t.detach().cpu().numpy().reshape(-1)
+    if a.dtype.kind == "f":
+        vals = ["%g" % float(v) for v in a]
+    else:
+        vals = [str(int(v)) for v in a]
+    return " ".join(["Scalar", _PT_INFO[pt][0], str(t.dim())] + [str(d) for d in t.shape] + vals)
+
+
+def as_sizes_compatible(a, b):
+    """poutre::AsSizesCompatible: same rank and shape."""
+    return tuple(_img(a).shape) == tuple(_img(b).shape)
+
+
+def as_types_compatible(a, b):
+    """poutre::AsTypesCompatible: same pixel type (every image here is Scalar)."""
+    return _img(a).dtype == _img(b).dtype
+
+
+def are_images_different(a, b):
+    """poutre::AreImagesDifferent: the two images are different memory."""
+    return _img(a).data_ptr() != _img(b).data_ptr()
 
 
 # ---------------------------------------------------------------- low-level morphology
@@ -298,7 +400,7 @@ def h_concave(img, value, se, out=None):
 
 # ---------------------------------------------------------------- labelling
 def _label_out(img, out):
-    t = _torch()
+    t, img = _torch(), _img(img)
     return t.empty(img.shape, dtype=t.int64, device=img.device) if out is None else out
 
 

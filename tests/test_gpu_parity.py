@@ -523,3 +523,31 @@ def test_label_errors():
         pm.label_binary(a, PM_SE["SESquare2D"], out=g(np.zeros((8, 8), np.int64)))
     with pytest.raises(pm.PoutreError):
         pm.label_binary(a, PM_SE["SECross3D"])
+
+
+class _DLPackOnly:
+    """an image that only exports DLPack (as a CuPy or JAX device array would)"""
+
+    def __init__(self, t):
+        self._t = t
+
+    def __dlpack__(self, *args, **kw):
+        return self._t.__dlpack__(*args, **kw)
+
+    def __dlpack_device__(self):
+        return self._t.__dlpack_device__()
+
+
+def test_dlpack_images_zero_copy():
+    """DLPack producers as inputs and outputs (used in place) and a factory_image output"""
+    img = rand((300, 517), np.uint8, 301)
+    out = pm.factory_image(img.shape, pm.CompoundType.Scalar, pm.PType.GrayUINT8)
+    r = pm.dilate(_DLPackOnly(g(img)), PM_SE["SESquare2D"], 3, out=_DLPackOnly(out))
+    assert isinstance(r, _DLPackOnly)
+    np.testing.assert_array_equal(h(out), oracle.static_iter(img, "SESquare2D", 3, True))
+    fg = (img > 128).astype(np.uint8)
+    lab, n = pm.label_binary(_DLPackOnly(g(fg)), PM_SE["SESquare2D"])
+    exp, m = oracle.label(fg, "SESquare2D")
+    assert n == m
+    np.testing.assert_array_equal(h(lab), exp)
+    assert pm.to_string(pm.from_string("Scalar GUINT8 2 2 2 1 2 3 4")) == "Scalar GUINT8 2 2 2 1 2 3 4"
