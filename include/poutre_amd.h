@@ -143,12 +143,38 @@ int pm_dynamic_pseudo_opening(const pm_image *in, double h, int se, pm_image *ou
 int pm_dynamic_pseudo_closing(const pm_image *in, double h, int se, pm_image *out, void *stream);
 
 /* ---- elementwise helpers on the path (include/poutre/pixel_processing/arith.hpp:31-53):
- * ArithSupImage / ArithInfImage / ArithSaturatedSubImage / ArithSaturatedSub/AddConstant */
+ * ArithSupImage / ArithInfImage / ArithSaturatedSubImage / ArithSaturatedAddImage /
+ * ArithSaturatedSub/AddConstant (accumulated in a wider type, clamped at the type's minimum /
+ * maximum), ArithInvertImage (-x; integers modulo 2^bits) */
 int pm_arith_sup(const pm_image *a, const pm_image *b, pm_image *out, void *stream);
 int pm_arith_inf(const pm_image *a, const pm_image *b, pm_image *out, void *stream);
 int pm_arith_ssub_image(const pm_image *a, const pm_image *b, pm_image *out, void *stream);
+int pm_arith_sadd_image(const pm_image *a, const pm_image *b, pm_image *out, void *stream);
 int pm_arith_ssub_const(const pm_image *in, double c, pm_image *out, void *stream);
 int pm_arith_sadd_const(const pm_image *in, double c, pm_image *out, void *stream);
+int pm_arith_invert(const pm_image *in, pm_image *out, void *stream);
+
+/* ---- poutre::CompareImage, all eight overloads (include/poutre/pixel_processing/compare.hpp:31-84,
+ * src/pixel_processing/compare.cpp:601-): out = (in OP comp) ? vtrue : vfalse, pixel by pixel,
+ * the input pixel on the left.  Each operand is an image (image != NULL, same shape as in) or
+ * the scalar value.  comp has in's PType; vtrue, vfalse and out share a PType that may differ
+ * from in's.  Scalars are converted to those PTypes (integers: truncated toward zero and
+ * clamped to the type's range, NaN -> 0).  op: one CompOpType bit (types.hpp:443-453).  out
+ * may be one of the inputs. */
+enum pm_compare_op {
+  PM_CMP_EQUAL = 1 << 0,
+  PM_CMP_DIFF = 1 << 1,
+  PM_CMP_SUP = 1 << 2,
+  PM_CMP_SUP_EQUAL = 1 << 3,
+  PM_CMP_INF = 1 << 4,
+  PM_CMP_INF_EQUAL = 1 << 5
+};
+typedef struct pm_operand {
+  const pm_image *image; /* NULL: the operand is `value` */
+  double value;
+} pm_operand;
+int pm_compare(const pm_image *in, int op, const pm_operand *comp, const pm_operand *vtrue, const pm_operand *vfalse,
+               pm_image *out, void *stream);
 
 /* ---- differentiable float32 dilation / erosion (no reference counterpart; extends the flat
  * Dilate/Erode of include/poutre/low_level_morpho/ero_dil.hpp:32-47 to a structuring function).

@@ -56,6 +56,8 @@ def lib():
             "or_arith_ssub_const": [ci, vp, cd, vp, c64],
             "or_arith_sadd_const": [ci, vp, cd, vp, c64],
             "or_arith_ssub_image": [ci, vp, vp, vp, c64],
+            "or_arith_sadd_image": [ci, vp, vp, vp, c64],
+            "or_arith_invert": [ci, vp, vp, c64],
             "or_arith_sup": [ci, vp, vp, vp, c64],
             "or_arith_inf": [ci, vp, vp, vp, c64],
             "or_label": [ci, ci, vp, vp, ci, i64p, ci, i64p],
@@ -218,6 +220,37 @@ def _ew2(fn, a, b):
 
 def ssub_image(a, b):
     return _ew2(lib().or_arith_ssub_image, a, b)
+
+
+def sadd_image(a, b):
+    return _ew2(lib().or_arith_sadd_image, a, b)
+
+
+def invert(img):
+    img, t = _t(img)
+    out = np.empty_like(img)
+    _check(lib().or_arith_invert(t, _p(img), _p(out), img.size), "arith")
+    return out
+
+
+# CompareImage: CompOpType bit values (include/poutre/base/types.hpp:443-453) and the operators
+# of compare_op_t.hpp:40-165 (the input pixel on the left)
+CMP_OPS = {"Equal": 1, "Diff": 2, "Sup": 4, "SupEqual": 8, "Inf": 16, "InfEqual": 32}
+_CMP_FN = {1: np.equal, 2: np.not_equal, 4: np.greater, 8: np.greater_equal, 16: np.less, 32: np.less_equal}
+
+
+def compare(img, op, comp, vtrue, vfalse, out_dtype):
+    """CompareImage, every operand form (compare.hpp:31-84 -> compare.cpp; select of
+    compare_op_t.hpp:305-351): out = (img OP comp) ? vtrue : vfalse.  comp is an image of
+    img's type or a scalar of that type; vtrue / vfalse are images or scalars of the output
+    type.  Elementwise numpy (the comparisons are IEEE's, as C++'s)."""
+    img = np.asarray(img)
+    out_dtype = np.dtype(out_dtype)
+    c = comp if isinstance(comp, np.ndarray) else np.array(comp).astype(img.dtype)
+    tv = vtrue if isinstance(vtrue, np.ndarray) else np.array(vtrue).astype(out_dtype)
+    fv = vfalse if isinstance(vfalse, np.ndarray) else np.array(vfalse).astype(out_dtype)
+    cond = _CMP_FN[int(op)](img, c)
+    return np.where(cond, tv, fv).astype(out_dtype)
 
 
 def sup(a, b):

@@ -645,11 +645,28 @@ int or_leveling(int t, const void *ref, const void *marker, void *out, int rank,
     }                                                                                                         \
   }
 #define E_SSUB(S) ssub_##S(a[i], b[i])
+#define E_SADD(S) sadd_##S(a[i], b[i])
 #define E_SUP(S) OR_MAX(a[i], b[i])
 #define E_INF(S) OR_MIN(a[i], b[i])
 OR_EW(or_arith_ssub_image, E_SSUB)
+OR_EW(or_arith_sadd_image, E_SADD)
 OR_EW(or_arith_sup, E_SUP)
 OR_EW(or_arith_inf, E_INF)
+
+/* op_Invert (arith_op_t.hpp:40-57): -a in the pixel type; integers negate modulo 2^bits
+   (what -a converted back to the pixel type gives), floats flip the sign */
+int or_arith_invert(int t, const void *a_, void *o_, int64_t n)
+{
+  switch (t) {
+  case OR_T_U8: { const uint8_t *a = a_; uint8_t *o = o_; for (int64_t i = 0; i < n; i++) o[i] = (uint8_t)(0u - a[i]); return OR_OK; }
+  case OR_T_U16: { const uint16_t *a = a_; uint16_t *o = o_; for (int64_t i = 0; i < n; i++) o[i] = (uint16_t)(0u - a[i]); return OR_OK; }
+  case OR_T_I32: { const int32_t *a = a_; int32_t *o = o_; for (int64_t i = 0; i < n; i++) o[i] = (int32_t)(0u - (uint32_t)a[i]); return OR_OK; }
+  case OR_T_I64: { const int64_t *a = a_; int64_t *o = o_; for (int64_t i = 0; i < n; i++) o[i] = (int64_t)(0ull - (uint64_t)a[i]); return OR_OK; }
+  case OR_T_F32: { const float *a = a_; float *o = o_; for (int64_t i = 0; i < n; i++) o[i] = -a[i]; return OR_OK; }
+  case OR_T_F64: { const double *a = a_; double *o = o_; for (int64_t i = 0; i < n; i++) o[i] = -a[i]; return OR_OK; }
+  default: return OR_E_TYPE;
+  }
+}
 
 #define OR_EWC(name, FN)                                                                                      \
   int name(int t, const void *a_, double c, void *o_, int64_t n)                                              \

@@ -100,6 +100,10 @@ int or_leveling(int t, const void *ref, const void *marker, void *out, int rank,
 int or_arith_ssub_const(int t, const void *in, double c, void *out, int64_t n);
 int or_arith_sadd_const(int t, const void *in, double c, void *out, int64_t n);
 int or_arith_ssub_image(int t, const void *a, const void *b, void *out, int64_t n);
+/* ArithSaturatedAddImage (arith.hpp:44; op_Saturated_Add arith_op_t.hpp:146-190) and
+   ArithInvertImage (arith.hpp:33; op_Invert arith_op_t.hpp:40-57) */
+int or_arith_sadd_image(int t, const void *a, const void *b, void *out, int64_t n);
+int or_arith_invert(int t, const void *in, void *out, int64_t n);
 int or_arith_sup(int t, const void *a, const void *b, void *out, int64_t n);
 int or_arith_inf(int t, const void *a, const void *b, void *out, int64_t n);
 

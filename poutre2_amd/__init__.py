@@ -32,7 +32,9 @@ __all__ = ["static_nl_se", "static_compound_se", "RecType", "erode", "dilate", "
            "high_leveling", "low_leveling", "leveling", "arith_sup", "arith_inf", "arith_ssub_image",
            "arith_ssub_const", "arith_sadd_const", "batch_dilate_open", "batch_reconstruction", "PoutreError",
            "version", "PType", "CompoundType", "factory_image", "from_string", "to_string", "as_sizes_compatible",
-           "as_types_compatible", "are_images_different"]
+           "as_types_compatible", "are_images_different", "CompOpType", "compare", "arith_invert", "arith_sadd_image",
+           "arith_invert_image", "arith_inf_image", "arith_sup_image", "arith_saturated_add_image",
+           "arith_saturated_sub_image", "arith_saturated_add_constant", "arith_saturated_sub_constant"]
 
 
 class static_nl_se(enum.IntEnum):
@@ -459,6 +461,55 @@ def _ew2(name, a, b, out):
     return out
 
 
+def arith_sadd_image(a, b, out=None):
+    return _ew2("pm_arith_sadd_image", a, b, out)
+
+
+def arith_invert(img, out=None):
+    """ArithInvertImage (arith.hpp:33): -img, integers modulo 2^bits; out may be img."""
+    out = _out(img, out)
+    di, do = _desc(img), _desc(out)
+    _call("pm_arith_invert", ctypes.byref(di), ctypes.byref(do), _stream(img))
+    return out
+
+
+class CompOpType(enum.IntEnum):
+    """poutre::CompOpType (include/poutre/base/types.hpp:443-453; python/base/types.cpp:25-31)."""
+    Equal = 1 << 0
+    Diff = 1 << 1
+    Sup = 1 << 2
+    SupEqual = 1 << 3
+    Inf = 1 << 4
+    InfEqual = 1 << 5
+
+
+def _operand(x, keep):
+    op = _capi.PmOperand()
+    if isinstance(x, (int, float, np.integer, np.floating)) and not isinstance(x, bool):
+        op.value = float(x)
+        return op
+    d = _desc(x)
+    keep.append(d)
+    op.image = ctypes.pointer(d)
+    return op
+
+
+def compare(img, op, comp, vtrue, vfalse, out=None):
+    """poutre::CompareImage / pypoutre.compare, every overload (compare.hpp:31-84): out =
+    (img OP comp) ? vtrue : vfalse pixel by pixel; comp, vtrue and vfalse are each an image
+    or a number.  comp has img's type; vtrue / vfalse / out share a type (out's if given,
+    else the first image among vtrue / vfalse, else img's).  out may be one of the inputs."""
+    if out is None:
+        like = next((v for v in (vtrue, vfalse) if not isinstance(v, (int, float, np.integer, np.floating))), img)
+        out = _torch().empty(_img(img).shape, dtype=_img(like).dtype, device=_img(img).device)
+    keep = []
+    oc, ot, of = _operand(comp, keep), _operand(vtrue, keep), _operand(vfalse, keep)
+    di, do = _desc(img), _desc(out)
+    _call("pm_compare", ctypes.byref(di), int(op), ctypes.byref(oc), ctypes.byref(ot), ctypes.byref(of),
+          ctypes.byref(do), _stream(img))
+    return out
+
+
 def arith_sup(a, b, out=None):
     return _ew2("pm_arith_sup", a, b, out)
 
@@ -484,6 +535,16 @@ def arith_ssub_const(a, c, out=None):
 
 def arith_sadd_const(a, c, out=None):
     return _ewc("pm_arith_sadd_const", a, c, out)
+
+
+# the names of pypoutre's pixel-processing module (python/pixel_processing/arith.cpp:13-19)
+arith_invert_image = arith_invert
+arith_inf_image = arith_inf
+arith_sup_image = arith_sup
+arith_saturated_add_image = arith_sadd_image
+arith_saturated_sub_image = arith_ssub_image
+arith_saturated_add_constant = arith_sadd_const
+arith_saturated_sub_constant = arith_ssub_const
 
 
 # ---------------------------------------------------------------- batched launches

@@ -21,6 +21,11 @@ class PmImage(ctypes.Structure):
                 ("shape", ctypes.c_int64 * 3)]
 
 
+class PmOperand(ctypes.Structure):
+    """pm_operand of pm_compare: an image, or (image NULL) the scalar value"""
+    _fields_ = [("image", ctypes.POINTER(PmImage)), ("value", ctypes.c_double)]
+
+
 class PoutreError(RuntimeError):
     """Raised when an entry point returns a negative status (the reference throws
     std::runtime_error in the same situations, include/poutre/base/config.hpp:203-215)."""
@@ -73,6 +78,9 @@ SIGNATURES = {
     "pm_arith_ssub_image": [_P, _P, _P, _VP],
     "pm_arith_ssub_const": [_P, _D, _P, _VP],
     "pm_arith_sadd_const": [_P, _D, _P, _VP],
+    "pm_arith_sadd_image": [_P, _P, _P, _VP],
+    "pm_arith_invert": [_P, _P, _VP],
+    "pm_compare": [_P, _I, ctypes.POINTER(PmOperand), ctypes.POINTER(PmOperand), ctypes.POINTER(PmOperand), _P, _VP],
     "pm_maxplus_fwd": [_VP, _I64, _I64, _I64, _VP, _I, _I, _VP, _VP],
     "pm_maxplus_bwd": [_VP, _VP, _I64, _I64, _I64, _VP, _I, _I, _VP, _VP, _VP],
     "pm_batch_dilate_open": [_I, _P, ctypes.POINTER(_I), _P, _P, _VP],

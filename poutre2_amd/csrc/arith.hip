@@ -1,7 +1,8 @@
 // arith.hip -- elementwise helpers on the path (include/poutre/pixel_processing/arith.hpp:31-53,
-// arith_op_t.hpp:84-445): sup/inf of two images, saturated subtraction of images,
-// saturated addition/subtraction of a constant (clamped at numeric_limits<T>::min()/max(),
-// accumulated in a wider type), plus the opening / closing / gradient compositions.
+// arith_op_t.hpp:40-445, compare.hpp:31-84): sup/inf of two images, saturated addition and
+// subtraction of images and of a constant (clamped at numeric_limits<T>::min()/max(),
+// accumulated in a wider type), negation, the comparison-select of CompareImage, plus the
+// opening / closing / gradient compositions.
 #include <type_traits>
 
 #include "arith.hpp"
@@ -40,6 +41,7 @@ __global__ __launch_bounds__(256) void ew2_kernel(const T *__restrict__ a, const
     T r;
     if (OP == EW_SUP) r = x > y ? x : y;
     else if (OP == EW_INF) r = x < y ? x : y;
+    else if (OP == EW_SADD) r = sadd_sat<T>(x, y);
     else r = ssub_sat<T>(x, y);
     o[i] = r;
   }
@@ -50,6 +52,46 @@ __global__ __launch_bounds__(256) void ewc_kernel(const T *__restrict__ a, T c, 
 {
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
     o[i] = (OP == EWC_SSUB) ? ssub_sat<T>(a[i], c) : sadd_sat<T>(a[i], c);
+}
+
+// op_Invert (arith_op_t.hpp:40-57): -a in the pixel type (integers modulo 2^bits)
+template <typename T> __device__ __forceinline__ T negate(T a)
+{
+  if constexpr (std::is_integral<T>::value) {
+    using U = typename std::make_unsigned<T>::type;
+    return (T)(U)(U(0) - (U)a);
+  } else {
+    return -a;
+  }
+}
+template <typename T>
+__global__ __launch_bounds__(256) void inv_kernel(const T *__restrict__ a, T *__restrict__ o, int64_t n)
+{
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) o[i] = negate<T>(a[i]);
+}
+
+// CompareImage (compare_op_t.hpp:40-165, select :305-351): o = (in OP comp) ? vtrue : vfalse,
+// the input pixel on the left; an operand image pointer that is null selects its scalar.
+// op is one CompOpType bit; ">=" is "> or ==" (the same for every value, NaN included).  The
+// output may be one of the inputs (each pixel is read before it is written), so no restrict.
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void cmp_kernel(const TI *in, int op, const TI *ci, TI cv, const TO *ti, TO tv,
+                                                  const TO *fi, TO fv, TO *o, int64_t n)
+{
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const TI a = in[i], c = ci ? ci[i] : cv;
+    const bool eq = a == c, gt = a > c, lt = a < c;
+    bool p;
+    switch (op) {
+    case PM_CMP_EQUAL: p = eq; break;
+    case PM_CMP_DIFF: p = !eq; break;
+    case PM_CMP_SUP: p = gt; break;
+    case PM_CMP_SUP_EQUAL: p = gt || eq; break;
+    case PM_CMP_INF: p = lt; break;
+    default: p = lt || eq; break;  // PM_CMP_INF_EQUAL (op is checked on the host)
+    }
+    o[i] = p ? (ti ? ti[i] : tv) : (fi ? fi[i] : fv);
+  }
 }
 
 // marker of dynamic_pseudo_opening/closing (extrema.cpp:90-93,110-113) in one pass:
@@ -91,6 +133,7 @@ template <typename T, bool OPENING> int dpo_marker(const T *ext, const T *tmp, c
 }
 
 #define PM_INST(T)                                                                                           \
+  template int ew2<T, EW_SADD>(const T *, const T *, T *, int64_t, hipStream_t);                             \
   template int ew2<T, EW_SUP>(const T *, const T *, T *, int64_t, hipStream_t);                              \
   template int ew2<T, EW_INF>(const T *, const T *, T *, int64_t, hipStream_t);                              \
   template int ew2<T, EW_SSUB>(const T *, const T *, T *, int64_t, hipStream_t);                             \
@@ -109,6 +152,7 @@ PM_INST(double)
 template <int OP> static int arith2(const pm_image *a, const pm_image *b, pm_image *out, void *stream, const char *what)
 {
   clear_error();
+  PM_TRY(check_image(a, what));
   PM_TRY(check_pair(a, b, what));
   PM_TRY(check_pair(b, out, what));
   const int64_t n = numel(*a);
@@ -120,6 +164,7 @@ template <int OP> static int arith2(const pm_image *a, const pm_image *b, pm_ima
 template <int OP> static int arithc(const pm_image *a, double c, pm_image *out, void *stream, const char *what)
 {
   clear_error();
+  PM_TRY(check_image(a, what));
   PM_TRY(check_pair(a, out, what));
   const int64_t n = numel(*a);
   auto f = [a = a, c = c, o = out, n = n, s = as_stream(stream)](auto tag) -> int {
@@ -182,11 +227,89 @@ template <int K> static int compose(const pm_image *in, int se, int iter, pm_ima
   return dispatch_ptype(in->ptype, f);
 }
 
+// a scalar operand in the pixel type T: integers truncated toward zero and clamped to T's
+// range (NaN -> 0), floats converted
+template <typename T> static T to_pixel(double v)
+{
+  if constexpr (std::is_integral<T>::value) {
+    if (!(v == v)) return T(0);
+    if (v <= (double)std::numeric_limits<T>::lowest()) return std::numeric_limits<T>::lowest();
+    if (v >= (double)std::numeric_limits<T>::max()) return std::numeric_limits<T>::max();
+    return (T)v;
+  } else {
+    return (T)v;
+  }
+}
+
+static int cmp_operand(const pm_operand *o, const pm_image *like, int ptype, const char *name)
+{
+  if (!o) PM_FAIL(PM_E_ARG, "CompareImage: null %s operand", name);
+  if (!o->image) return PM_OK;
+  PM_TRY(check_image(o->image, "CompareImage"));
+  if (!same_shape(o->image, like)) PM_FAIL(PM_E_SIZE, "CompareImage images have not compatible sizes");
+  if (o->image->ptype != ptype) PM_FAIL(PM_E_PTYPE, "CompareImage: the %s image has not the expected PType", name);
+  return PM_OK;
+}
+
+static int compare_entry(const pm_image *in, int op, const pm_operand *comp, const pm_operand *vt, const pm_operand *vf,
+                         pm_image *out, hipStream_t s)
+{
+  clear_error();
+  PM_TRY(check_image(in, "CompareImage"));
+  PM_TRY(check_image(out, "CompareImage"));
+  if (!same_shape(in, out)) PM_FAIL(PM_E_SIZE, "CompareImage images have not compatible sizes");
+  if (op != PM_CMP_EQUAL && op != PM_CMP_DIFF && op != PM_CMP_SUP && op != PM_CMP_SUP_EQUAL && op != PM_CMP_INF &&
+      op != PM_CMP_INF_EQUAL)
+    PM_FAIL(PM_E_ARG, "CompareImage: unsupported CompOpType %d", op);
+  PM_TRY(cmp_operand(comp, in, in->ptype, "comparand"));
+  PM_TRY(cmp_operand(vt, in, out->ptype, "true"));
+  PM_TRY(cmp_operand(vf, in, out->ptype, "false"));
+  const int64_t n = numel(*in);
+  if (n == 0) return PM_OK;
+  return dispatch_ptype(in->ptype, [&](auto ti) -> int {
+    using TI = typename decltype(ti)::type;
+    return dispatch_ptype(out->ptype, [&](auto to) -> int {
+      using TO = typename decltype(to)::type;
+      const TI *cimg = comp->image ? (const TI *)comp->image->data : nullptr;
+      const TO *timg = vt->image ? (const TO *)vt->image->data : nullptr;
+      const TO *fimg = vf->image ? (const TO *)vf->image->data : nullptr;
+      hipLaunchKernelGGL((cmp_kernel<TI, TO>), dim3(grid1d(n, 256, 8192)), dim3(256), 0, s, (const TI *)in->data, op,
+                         cimg, to_pixel<TI>(comp->value), timg, to_pixel<TO>(vt->value), fimg, to_pixel<TO>(vf->value),
+                         (TO *)out->data, n);
+      PM_HIP(hipGetLastError());
+      return PM_OK;
+    });
+  });
+}
+
+static int invert_entry(const pm_image *in, pm_image *out, hipStream_t s)
+{
+  clear_error();
+  PM_TRY(check_image(in, "ArithInvertImage"));
+  PM_TRY(check_pair(in, out, "ArithInvertImage"));
+  const int64_t n = numel(*in);
+  if (n == 0) return PM_OK;
+  return dispatch_ptype(in->ptype, [&](auto tag) -> int {
+    using T = typename decltype(tag)::type;
+    hipLaunchKernelGGL(inv_kernel<T>, dim3(grid1d(n, 256, 8192)), dim3(256), 0, s, (const T *)in->data, (T *)out->data, n);
+    PM_HIP(hipGetLastError());
+    return PM_OK;
+  });
+}
+
 }  // namespace pm
 
 using namespace pm;
 
 extern "C" {
+int pm_arith_sadd_image(const pm_image *a, const pm_image *b, pm_image *out, void *stream)
+{ return arith2<EW_SADD>(a, b, out, stream, "ArithSaturatedAddImage"); }
+int pm_arith_invert(const pm_image *in, pm_image *out, void *stream) { return invert_entry(in, out, as_stream(stream)); }
+int pm_compare(const pm_image *in, int op, const pm_operand *comp, const pm_operand *vtrue, const pm_operand *vfalse,
+               pm_image *out, void *stream)
+{
+  return compare_entry(in, op, comp, vtrue, vfalse, out, as_stream(stream));
+}
 int pm_arith_sup(const pm_image *a, const pm_image *b, pm_image *out, void *stream)
 { return arith2<EW_SUP>(a, b, out, stream, "ArithSupImage"); }
 int pm_arith_inf(const pm_image *a, const pm_image *b, pm_image *out, void *stream)

@@ -4,7 +4,7 @@
 
 namespace pm {
 
-enum { EW_SUP = 0, EW_INF = 1, EW_SSUB = 2 };
+enum { EW_SUP = 0, EW_INF = 1, EW_SSUB = 2, EW_SADD = 3 };
 enum { EWC_SSUB = 0, EWC_SADD = 1 };
 
 template <typename T, int OP> int ew2(const T *a, const T *b, T *o, int64_t n, hipStream_t s);

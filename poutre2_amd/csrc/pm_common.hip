@@ -20,7 +20,7 @@ void set_error(const char *fmt, ...)
 
 void clear_error() { g_err[0] = 0; }
 
-static bool same_shape(const pm_image *a, const pm_image *b)
+bool same_shape(const pm_image *a, const pm_image *b)
 {
   if (a->rank != b->rank) return false;
   for (int i = 0; i < a->rank; i++)
@@ -28,7 +28,7 @@ static bool same_shape(const pm_image *a, const pm_image *b)
   return true;
 }
 
-static int check_image(const pm_image *im, const char *what)
+int check_image(const pm_image *im, const char *what)
 {
   if (!im) PM_FAIL(PM_E_ARG, "%s: null image descriptor", what);
   if (im->rank < 1 || im->rank > 3) PM_FAIL(PM_E_RANK, "%s: unsupported number of dims %d", what, im->rank);

@@ -115,6 +115,9 @@ inline Dims dims3(const pm_image &im)
 // AssertSizesCompatible / AssertAsTypesCompatible / AssertImagesAreDifferent
 // (include/poutre/base/image_interface.hpp:95-102), then rank 1..3.
 int check_pair(const pm_image *a, const pm_image *b, const char *what);
+bool same_shape(const pm_image *a, const pm_image *b);
+// one descriptor: rank 1..3, a known PType, extents >= 0, data for a non-empty image
+int check_image(const pm_image *im, const char *what);
 int check_unary(const pm_image *in, const pm_image *out, const char *what);
 int check_binary(const pm_image *a, const pm_image *b, const pm_image *out, const char *what);
 

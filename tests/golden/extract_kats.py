@@ -20,6 +20,9 @@ Sources read (relative to the reference root):
   test/label/label_flat_zones.cpp                 (label::label_flat_zones)
   test/label/label_extrema.cpp                    (label::label_minima / label_maxima)
 
+and, into ``compare_kats.json``, the whole-image cases of
+  test/pixel_processing/compare.cpp                (CompareImage, every operand form)
+
 Usage: python tests/golden/extract_kats.py [/root/reference]
 """
 import json
@@ -129,12 +132,60 @@ def extract(ref_root):
     return cases
 
 
+COMPARE_FILE = "test/pixel_processing/compare.cpp"
+_CMP_CALL = re.compile(r'(?:t_CompareImage_|ViewCompare_)(?P<form>[si]{3})\w*\(\s*(?P<args>[^;]*)\);', re.S)
+_INT_CONST = re.compile(r'\b(?P<name>\w+)\s*=\s*(?P<val>-?\d+)\s*;')
+
+
+def extract_compare(ref_root):
+    """Cases of compare.cpp on whole images: the input and image operands in the order the test
+    creates them (input, comparand, true, false), scalar operands as literals or the named
+    constants the test defines, the CompOpType and the expected image.  Cases on sub-views
+    (sections) are left out: they compare part of an image."""
+    src = open(os.path.join(ref_root, COMPARE_FILE)).read()
+    heads = [m for m in re.finditer(r'TEST_CASE\("(?P<name>[^"]*)"', src)]
+    cases = []
+    for k, hm in enumerate(heads):
+        body = src[hm.end():heads[k + 1].start() if k + 1 < len(heads) else len(src)]
+        line0 = src.count("\n", 0, hm.start()) + 1
+        call = _CMP_CALL.search(body)
+        if call is None:
+            continue
+        args = [a.strip() for a in re.sub(r"//[^\n]*", "", call.group("args")).replace("\n", " ").split(",")]
+        if args[0].startswith("s_") or args[-1].startswith("s_"):
+            continue  # sections
+        images = []
+        for m in re.finditer(r'ImageFromString\(\s*' + _STR, body, re.S):
+            raw = m.group("raw")
+            images.append(parse_image(raw if raw is not None else _cooked(m.group("cooked"))))
+        exp = re.search(r'expected\s*=\s*' + _STR, body, re.S)
+        if exp is None:
+            continue
+        consts = {m.group("name"): int(m.group("val")) for m in _INT_CONST.finditer(body[:call.start()])}
+        op = re.search(r"CompOpType::CompOp(\w+)", args[1]).group(1)
+        operands, nxt = [], 1
+        for kind, a in zip(call.group("form"), args[2:5]):
+            if kind == "i":
+                operands.append({"image": images[nxt]})
+                nxt += 1
+            else:
+                operands.append({"value": int(a) if re.fullmatch(r"-?\d+", a) else consts[a]})
+        expected = parse_image(exp.group("raw") if exp.group("raw") is not None else _cooked(exp.group("cooked")))
+        cases.append({"file": COMPARE_FILE, "name": hm.group("name"), "line": line0,
+                      "form": call.group("form"), "op": op, "input": images[0],
+                      "comp": operands[0], "true": operands[1], "false": operands[2], "expected": expected})
+    return cases
+
+
 def main():
     ref_root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     cases = extract(ref_root)
     out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "kats.json")
     with open(out, "w") as f:
         json.dump({"source": "ThomasRetornaz/poutre2 inline KATs", "cases": cases}, f, indent=1)
+    cmp = extract_compare(ref_root)
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "compare_kats.json"), "w") as f:
+        json.dump({"source": "ThomasRetornaz/poutre2 inline KATs (compare.cpp)", "cases": cmp}, f, indent=1)
     print(f"wrote {len(cases)} cases to {out}")
 
 

@@ -551,3 +551,62 @@ def test_dlpack_images_zero_copy():
     assert n == m
     np.testing.assert_array_equal(h(lab), exp)
     assert pm.to_string(pm.from_string("Scalar GUINT8 2 2 2 1 2 3 4")) == "Scalar GUINT8 2 2 2 1 2 3 4"
+
+
+# ----------------------------------------------------------------------------- CompareImage / arith
+from tests.test_oracle import COMPARE_KATS, compare_operands  # noqa: E402
+
+
+@pytest.mark.parametrize("c", COMPARE_KATS, ids=lambda c: f"compare.cpp:{c['line']}:{c['name']}")
+def test_compare_kats(c):
+    exp = kats.image(c["expected"])
+    ops = [g(o) if isinstance(o, np.ndarray) else o for o in compare_operands(c)]
+    out = g(np.zeros_like(exp))
+    got = pm.compare(g(kats.image(c["input"])), pm.CompOpType[c["op"]], *ops, out=out)
+    np.testing.assert_array_equal(h(got), exp)
+
+
+@pytest.mark.parametrize("tin,tout", [(np.uint8, np.uint8), (np.float32, np.uint8), (np.int64, np.float64),
+                                      (np.uint16, np.int32)])
+@pytest.mark.parametrize("form", ["sss", "iss", "sis", "ssi", "sii", "iii", "iis", "isi"])
+def test_compare_all_ops_forms_types(tin, tout, form):
+    shape = (37, 259)
+    a = rand(shape, tin, 400, 0, 6)
+    imgs = {"c": rand(shape, tin, 401, 0, 6), "t": rand(shape, tout, 402, 0, 100), "f": rand(shape, tout, 403, 0, 100)}
+    scal = {"c": 3, "t": 7, "f": 2}
+    opnds = [imgs[k] if kind == "i" else scal[k] for k, kind in zip("ctf", form)]
+    for op in pm.CompOpType:
+        out = g(np.zeros(shape, tout))
+        pm.compare(g(a), op, *[g(o) if isinstance(o, np.ndarray) else o for o in opnds], out=out)
+        np.testing.assert_array_equal(h(out), oracle.compare(a, int(op), *opnds, tout), err_msg=str(op))
+
+
+def test_compare_nan_inplace_and_errors():
+    x = np.array([[np.nan, 1.0, 2.0, -np.inf, np.inf, 0.0]], np.float32)
+    y = np.array([[1.0, np.nan, 2.0, -np.inf, 0.0, -0.0]], np.float32)
+    for op in pm.CompOpType:
+        got = pm.compare(g(x), op, g(y), 1.5, -1.0)
+        np.testing.assert_array_equal(h(got), oracle.compare(x, int(op), y, 1.5, -1.0, np.float32))
+    a = rand((50, 60), np.int32, 404)
+    t = g(a)
+    pm.compare(t, pm.CompOpType.Sup, 0, t, 0, out=t)  # out is the input and the true image
+    np.testing.assert_array_equal(h(t), np.where(a > 0, a, 0))
+    with pytest.raises(pm.PoutreError):
+        pm.compare(g(a), 3, 0, 1, 0)  # not a single CompOpType bit
+    with pytest.raises(pm.PoutreError):
+        pm.compare(g(a), pm.CompOpType.Equal, g(a[:, :10].copy()), 1, 0)
+    with pytest.raises(pm.PoutreError):
+        pm.compare(g(a), pm.CompOpType.Equal, g(a.astype(np.float32)), 1, 0)  # comparand of another type
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_sadd_image_and_invert(dtype):
+    a = rand((130, 257), dtype, 405)
+    b = rand((130, 257), dtype, 406)
+    if np.dtype(dtype).kind == "i":
+        a, b = np.abs(a // 2).astype(dtype), (b // 2).astype(dtype)  # sums stay above the type's minimum
+    np.testing.assert_array_equal(h(pm.arith_saturated_add_image(g(a), g(b))), oracle.sadd_image(a, b))
+    np.testing.assert_array_equal(h(pm.arith_invert_image(g(a))), oracle.invert(a))
+    t = g(a)
+    pm.arith_invert(t, out=t)
+    np.testing.assert_array_equal(h(t), oracle.invert(a))

@@ -304,3 +304,68 @@ def test_h_maxima_is_ssub_then_reconstruct():
     f = rng.integers(0, 256, size=(20, 20)).astype(np.uint8)
     exp = oracle.reconstruct_bf("dilate", np.maximum(f.astype(int) - 32, 0).astype(np.uint8), f, "SESquare2D")
     np.testing.assert_array_equal(oracle.h_maxima(f, 32, "SESquare2D"), exp)
+
+
+def test_saturated_add_image_and_invert():
+    """ArithSaturatedAddImage / ArithInvertImage: against Python integers (arbitrary precision,
+    no accumulator type), the involution -(-a) = a, and the reference's own values
+    (test/pixel_processing/arith.cpp:26-45: an int64 image of 1 negates to -1;
+    arith.cpp:657-678: uint8 250 + 150 saturates at 255)."""
+    rng = np.random.default_rng(12)
+    for dt in (np.uint8, np.uint16, np.int32, np.int64):
+        info = np.iinfo(dt)
+        a = rng.integers(info.min, info.max, size=500, dtype=dt, endpoint=True)
+        b = rng.integers(info.min, info.max, size=500, dtype=dt, endpoint=True)
+        if info.min < 0:  # the reference clamps at the maximum only: keep sums above the minimum
+            a, b = np.abs(a // 2), b // 2
+        s = oracle.sadd_image(a, b)
+        exp = [min(int(x) + int(y), int(info.max)) for x, y in zip(a, b)]
+        assert [int(v) for v in s] == exp
+        inv = oracle.invert(a)
+        m = 1 << info.bits  # -a taken back into the type's range modulo 2^bits
+        assert [int(v) for v in inv] == [(-int(x) - int(info.min)) % m + int(info.min) for x in a]
+        np.testing.assert_array_equal(oracle.invert(inv), a)
+    f = np.array([1.5, -0.0, 3e38, -2.0], np.float32)
+    np.testing.assert_array_equal(oracle.invert(f), -f)
+    assert np.signbit(oracle.invert(f)[1]) == False  # noqa: E712  (-(-0.0) = +0.0)
+    big = np.array([3e38, 1.0], np.float32)
+    assert oracle.sadd_image(big, big)[0] == np.finfo(np.float32).max
+    np.testing.assert_array_equal(oracle.invert(np.ones((5, 6), np.int64)), -np.ones((5, 6), np.int64))
+    np.testing.assert_array_equal(oracle.sadd_image(np.full((5, 6), 250, np.uint8), np.full((5, 6), 150, np.uint8)),
+                                  np.full((5, 6), 255, np.uint8))
+
+
+def _compare_kats():
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "compare_kats.json")
+    with open(path) as f:
+        return json.load(f)["cases"]
+
+
+COMPARE_KATS = _compare_kats()
+
+
+def compare_operands(c):
+    return [kats.image(o["image"]) if "image" in o else o["value"] for o in (c["comp"], c["true"], c["false"])]
+
+
+@pytest.mark.parametrize("c", COMPARE_KATS, ids=lambda c: f"compare.cpp:{c['line']}:{c['name']}")
+def test_oracle_compare_kats(c):
+    """every whole-image CompareImage KAT of the reference (8 operand forms x 6 operators)"""
+    exp = kats.image(c["expected"])
+    got = oracle.compare(kats.image(c["input"]), oracle.CMP_OPS[c["op"]], *compare_operands(c), exp.dtype)
+    np.testing.assert_array_equal(got, exp)
+
+
+def test_oracle_compare_ieee_and_types():
+    """NaN compares false except for Diff (Python's float comparisons decide each pixel), and
+    the output type is the true / false values' type, not the input's"""
+    import operator
+    x = np.array([np.nan, 1.0, 2.0, -np.inf, np.inf], np.float64)
+    y = np.array([1.0, np.nan, 2.0, -np.inf, 0.0], np.float64)
+    fns = {1: operator.eq, 2: operator.ne, 4: operator.gt, 8: operator.ge, 16: operator.lt, 32: operator.le}
+    for op, fn in fns.items():
+        got = oracle.compare(x, op, y, np.int32(7), np.int32(-1), np.int32)
+        assert got.dtype == np.int32
+        assert got.tolist() == [7 if fn(float(a), float(b)) else -1 for a, b in zip(x, y)]
