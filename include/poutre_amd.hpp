@@ -213,6 +213,44 @@ void Opening(const IInterface &i_img, se::Common_NL_SE nl_static, const int iter
 void Closing(const IInterface &i_img, se::Common_NL_SE nl_static, const int iter, IInterface &o_img);
 void Gradient(const IInterface &i_img, se::Common_NL_SE nl_static, const int iter, IInterface &o_img);
 
+// ---- pixel processing (include/poutre/pixel_processing/arith.hpp:31-53, compare.hpp:31-84).
+// Scalars of any ScalarTypeVariant alternative are converted to the images' pixel type (the
+// reference requires the alternative of that type).  CompareImage: o_img = (i_img OP comp) ?
+// true : false pixel by pixel; comp has i_img's PType, true / false have o_img's, which may
+// differ from i_img's.
+enum class CompOpType {
+  CompOpUndef = 0,
+  CompOpEqual = 1 << 0,
+  CompOpDiff = 1 << 1,
+  CompOpSup = 1 << 2,
+  CompOpSupEqual = 1 << 3,
+  CompOpInf = 1 << 4,
+  CompOpInfEqual = 1 << 5
+};
+void ArithInvertImage(const IInterface &i_img, IInterface &o_img);
+void ArithSupImage(const IInterface &i_img1, const IInterface &i_img2, IInterface &o_img);
+void ArithInfImage(const IInterface &i_img1, const IInterface &i_img2, IInterface &o_img);
+void ArithSaturatedAddImage(const IInterface &i_img1, const IInterface &i_img2, IInterface &o_img);
+void ArithSaturatedSubImage(const IInterface &i_img1, const IInterface &i_img2, IInterface &o_img);
+void ArithSaturatedAddConstant(const IInterface &i_img, const ScalarTypeVariant &pvalue, IInterface &o_img);
+void ArithSaturatedSubConstant(const IInterface &i_img, const ScalarTypeVariant &pvalue, IInterface &o_img);
+void CompareImage(const IInterface &i_img, CompOpType compOpType, const ScalarTypeVariant &i_comp,
+                  const ScalarTypeVariant &i_valtrue, const ScalarTypeVariant &i_valfalse, IInterface &o_img);
+void CompareImage(const IInterface &i_img, CompOpType compOpType, const IInterface &i_imgcomp,
+                  const IInterface &i_imgtrue, const IInterface &i_imgfalse, IInterface &o_img);
+void CompareImage(const IInterface &i_img, CompOpType compOpType, const ScalarTypeVariant &i_comp,
+                  const IInterface &i_imgtrue, const IInterface &i_imgfalse, IInterface &o_img);
+void CompareImage(const IInterface &i_img, CompOpType compOpType, const ScalarTypeVariant &i_comp,
+                  const IInterface &i_imgtrue, const ScalarTypeVariant &i_valfalse, IInterface &o_img);
+void CompareImage(const IInterface &i_img, CompOpType compOpType, const IInterface &i_imgcomp,
+                  const ScalarTypeVariant &i_valtrue, const IInterface &i_imgfalse, IInterface &o_img);
+void CompareImage(const IInterface &i_img, CompOpType compOpType, const IInterface &i_imgcomp,
+                  const ScalarTypeVariant &i_valtrue, const ScalarTypeVariant &i_valfalse, IInterface &o_img);
+void CompareImage(const IInterface &i_img, CompOpType compOpType, const ScalarTypeVariant &i_comp,
+                  const ScalarTypeVariant &i_valtrue, const IInterface &i_imgfalse, IInterface &o_img);
+void CompareImage(const IInterface &i_img, CompOpType compOpType, const IInterface &i_imgcomp,
+                  const IInterface &i_imgtrue, const ScalarTypeVariant &i_valfalse, IInterface &o_img);
+
 namespace geo {
 
 enum class reconstruction_type : std::uint8_t { erode, dilate };

@@ -394,6 +394,108 @@ double scalar(const ScalarTypeVariant &v)
 
 }  // namespace
 
+// ------------------------------------------------------------------ pixel processing
+// The reference's arith and compare functions check sizes and types only (arith.cpp,
+// compare.cpp): the output may be an input.
+namespace {
+template <typename F>
+void arith_binary(const IInterface &a, const IInterface &b, IInterface &out, const std::string &what, F &&f)
+{
+  AssertSizesCompatible(a, out, what + " images have not compatible sizes");
+  AssertSizesCompatible(b, out, what + " images have not compatible sizes");
+  AssertAsTypesCompatible(a, out, what + " images must have compatible types");
+  AssertAsTypesCompatible(b, out, what + " images must have compatible types");
+  Staged sa(a, true), sb(b, true), so(out, false);
+  pm_check(f(sa.desc(), sb.desc(), so.desc()));
+  so.download();
+}
+template <typename F> void arith_unary(const IInterface &a, IInterface &out, const std::string &what, F &&f)
+{
+  AssertSizesCompatible(a, out, what + " images have not compatible sizes");
+  AssertAsTypesCompatible(a, out, what + " images must have compatible types");
+  Staged sa(a, true), so(out, false);
+  pm_check(f(sa.desc(), so.desc()));
+  so.download();
+}
+
+// an operand of CompareImage: an image or a scalar
+struct CmpArg {
+  const IInterface *img = nullptr;
+  double value = 0;
+  CmpArg(const IInterface &i) : img(&i) {}
+  CmpArg(const ScalarTypeVariant &v) : value(scalar(v)) {}
+};
+
+void compare_any(const IInterface &in, CompOpType op, const CmpArg &c, const CmpArg &t, const CmpArg &f,
+                 IInterface &out)
+{
+  AssertSizesCompatible(in, out, "CompareImage images have not compatible sizes");
+  for (const CmpArg *x : {&c, &t, &f})
+    if (x->img) AssertSizesCompatible(*x->img, out, "CompareImage images have not compatible sizes");
+  Staged si(in, true), so(out, false);
+  std::unique_ptr<Staged> st[3];
+  pm_operand po[3];
+  const CmpArg *args[3] = {&c, &t, &f};
+  for (int k = 0; k < 3; k++) {
+    po[k].image = nullptr;
+    po[k].value = args[k]->value;
+    if (args[k]->img) {
+      st[k] = std::make_unique<Staged>(*args[k]->img, true);
+      po[k].image = st[k]->desc();
+    }
+  }
+  pm_check(pm_compare(si.desc(), (int)op, &po[0], &po[1], &po[2], so.desc(), nullptr));
+  so.download();
+}
+}  // namespace
+
+void ArithInvertImage(const IInterface &i, IInterface &o)
+{
+  arith_unary(i, o, "ArithInvertImage", [&](pm_image *a, pm_image *b) { return pm_arith_invert(a, b, nullptr); });
+}
+void ArithSupImage(const IInterface &a, const IInterface &b, IInterface &o)
+{
+  arith_binary(a, b, o, "ArithSupImage", [&](pm_image *x, pm_image *y, pm_image *z) { return pm_arith_sup(x, y, z, nullptr); });
+}
+void ArithInfImage(const IInterface &a, const IInterface &b, IInterface &o)
+{
+  arith_binary(a, b, o, "ArithInfImage", [&](pm_image *x, pm_image *y, pm_image *z) { return pm_arith_inf(x, y, z, nullptr); });
+}
+void ArithSaturatedAddImage(const IInterface &a, const IInterface &b, IInterface &o)
+{
+  arith_binary(a, b, o, "ArithSaturatedAddImage",
+               [&](pm_image *x, pm_image *y, pm_image *z) { return pm_arith_sadd_image(x, y, z, nullptr); });
+}
+void ArithSaturatedSubImage(const IInterface &a, const IInterface &b, IInterface &o)
+{
+  arith_binary(a, b, o, "ArithSaturatedSubImage",
+               [&](pm_image *x, pm_image *y, pm_image *z) { return pm_arith_ssub_image(x, y, z, nullptr); });
+}
+void ArithSaturatedAddConstant(const IInterface &i, const ScalarTypeVariant &v, IInterface &o)
+{
+  arith_unary(i, o, "ArithSaturatedAddConstant",
+              [&](pm_image *a, pm_image *b) { return pm_arith_sadd_const(a, scalar(v), b, nullptr); });
+}
+void ArithSaturatedSubConstant(const IInterface &i, const ScalarTypeVariant &v, IInterface &o)
+{
+  arith_unary(i, o, "ArithSaturatedSubConstant",
+              [&](pm_image *a, pm_image *b) { return pm_arith_ssub_const(a, scalar(v), b, nullptr); });
+}
+#define PM_CMP_OVERLOAD(TC, TT, TF)                                                                           \
+  void CompareImage(const IInterface &i, CompOpType op, TC c, TT t, TF f, IInterface &o)                       \
+  {                                                                                                           \
+    compare_any(i, op, CmpArg(c), CmpArg(t), CmpArg(f), o);                                                   \
+  }
+PM_CMP_OVERLOAD(const ScalarTypeVariant &, const ScalarTypeVariant &, const ScalarTypeVariant &)
+PM_CMP_OVERLOAD(const IInterface &, const IInterface &, const IInterface &)
+PM_CMP_OVERLOAD(const ScalarTypeVariant &, const IInterface &, const IInterface &)
+PM_CMP_OVERLOAD(const ScalarTypeVariant &, const IInterface &, const ScalarTypeVariant &)
+PM_CMP_OVERLOAD(const IInterface &, const ScalarTypeVariant &, const IInterface &)
+PM_CMP_OVERLOAD(const IInterface &, const ScalarTypeVariant &, const ScalarTypeVariant &)
+PM_CMP_OVERLOAD(const ScalarTypeVariant &, const ScalarTypeVariant &, const IInterface &)
+PM_CMP_OVERLOAD(const IInterface &, const IInterface &, const ScalarTypeVariant &)
+#undef PM_CMP_OVERLOAD
+
 // ------------------------------------------------------------------ low-level morphology
 void Erode(const IInterface &i, se::Common_NL_SE s, const int iter, IInterface &o)
 {

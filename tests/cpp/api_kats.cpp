@@ -4,6 +4,9 @@
 // Each case runs twice: on host images (staged through HBM) and on device images.
 //
 // usage: api_kats <cases.txt>   one case per line: op|se|rec|r|h|in1|in2|expected
+//   CompareImage cases: op = compare:<form>:<CompOpType bit>, form = three of s / i (comparand,
+//   true, false: scalar or image), se = the scalar operands "c,t,f", in2 = the image operands
+//   in that order, ';'-separated; the output has the expected image's PType.
 #include <cstdio>
 #include <fstream>
 #include <functional>
@@ -46,9 +49,44 @@ static std::unique_ptr<IInterface> on(bool dev, const std::string &s)
   return d;
 }
 
+static void run_compare(const std::vector<std::string> &f, bool dev, const IInterface &a, IInterface &out)
+{
+  const auto spec = split(f[0], ':');
+  const std::string &form = spec[1];
+  const auto cop = (CompOpType)std::stoi(spec[2]);
+  const auto vals = split(f[1], ',');
+  std::vector<std::unique_ptr<IInterface>> ims;
+  for (const auto &t : split(f[6], ';'))
+    if (!t.empty()) ims.push_back(on(dev, t));
+  std::size_t next = 0;
+  std::vector<ScalarTypeVariant> sv(3);
+  std::vector<const IInterface *> iv(3, nullptr);
+  for (int k = 0; k < 3; k++) {
+    if (form[k] == 'i') iv[k] = ims.at(next++).get();
+    else sv[k] = (pINT64)std::stoll(vals[k]);
+  }
+  if (form == "sss") CompareImage(a, cop, sv[0], sv[1], sv[2], out);
+  else if (form == "iii") CompareImage(a, cop, *iv[0], *iv[1], *iv[2], out);
+  else if (form == "sii") CompareImage(a, cop, sv[0], *iv[1], *iv[2], out);
+  else if (form == "sis") CompareImage(a, cop, sv[0], *iv[1], sv[2], out);
+  else if (form == "isi") CompareImage(a, cop, *iv[0], sv[1], *iv[2], out);
+  else if (form == "iss") CompareImage(a, cop, *iv[0], sv[1], sv[2], out);
+  else if (form == "ssi") CompareImage(a, cop, sv[0], sv[1], *iv[2], out);
+  else CompareImage(a, cop, *iv[0], *iv[1], sv[2], out);  // iis
+}
+
 static int run_case(const std::vector<std::string> &f, bool dev)
 {
   const std::string &op = f[0];
+  if (op.rfind("compare:", 0) == 0) {
+    auto a = on(dev, f[5]);
+    auto out = on(dev, f[7]);  // overwritten: only its geometry and PType matter
+    run_compare(f, dev, *a, *out);
+    const std::string got = ImageToString(*out), exp = ImageToString(*ImageFromString(f[7]));
+    if (got == exp) return 0;
+    std::printf("FAIL %s dev=%d\n  got: %s\n  exp: %s\n", op.c_str(), (int)dev, got.c_str(), exp.c_str());
+    return 1;
+  }
   const auto se = se_of(f[1]);
   auto a = on(dev, f[5]);
   std::unique_ptr<IInterface> b = f[6].empty() ? nullptr : on(dev, f[6]);
@@ -84,6 +122,13 @@ static int run_case(const std::vector<std::string> &f, bool dev)
   else if (op == "label_flat_zones") label::label_flat_zones(*a, se, *out);
   else if (op == "label_minima") label::label_minima(*a, se, *out);
   else if (op == "label_maxima") label::label_maxima(*a, se, *out);
+  else if (op == "ArithInvertImage") ArithInvertImage(*a, *out);
+  else if (op == "ArithSupImage") ArithSupImage(*a, *b, *out);
+  else if (op == "ArithInfImage") ArithInfImage(*a, *b, *out);
+  else if (op == "ArithSaturatedAddImage") ArithSaturatedAddImage(*a, *b, *out);
+  else if (op == "ArithSaturatedSubImage") ArithSaturatedSubImage(*a, *b, *out);
+  else if (op == "ArithSaturatedAddConstant") ArithSaturatedAddConstant(*a, h, *out);
+  else if (op == "ArithSaturatedSubConstant") ArithSaturatedSubConstant(*a, h, *out);
   else return 2;
   const std::string got = ImageToString(*out);
   const std::string exp = ImageToString(*ImageFromString(f[7]));

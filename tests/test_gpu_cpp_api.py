@@ -1,16 +1,24 @@
 """The reference's known-answer tests through the C++ API (include/poutre_amd.hpp), on host
-and device images, plus the reference's error behaviour and SE-model KATs."""
+and device images (the compare KATs of test/pixel_processing/compare.cpp and the arith
+functions against the oracle included), plus the reference's error behaviour and SE-model KATs."""
 import os
 import subprocess
 
+import numpy as np
 import pytest
 
+import oracle
 from tests import kats
+from tests.test_oracle import COMPARE_KATS
 
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "poutre2_amd", "api_kats")
+
+
+def _arr_str(pt, a):
+    return " ".join(["Scalar", pt, str(a.ndim)] + [str(x) for x in a.shape] + [repr(v) for v in a.reshape(-1).tolist()])
 
 
 def _img_str(d):
@@ -30,6 +38,27 @@ def test_cpp_api_kats(tmp_path):
         lines.append("|".join([op, c.get("se") or "", c.get("rec_type") or "", str(c.get("r") or ""),
                                str(c.get("h") or ""), _img_str(ins[0]), _img_str(ins[1]) if len(ins) > 1 else "",
                                _img_str(c["expected"])]))
+    # CompareImage: every whole-image KAT of the reference (test/pixel_processing/compare.cpp)
+    for c in COMPARE_KATS:
+        vals = ",".join(str(o.get("value", 0)) for o in (c["comp"], c["true"], c["false"]))
+        imgs = ";".join(_img_str(o["image"]) for o in (c["comp"], c["true"], c["false"]) if "image" in o)
+        lines.append("|".join([f"compare:{c['form']}:{oracle.CMP_OPS[c['op']]}", vals, "", "", "",
+                               _img_str(c["input"]), imgs, _img_str(c["expected"])]))
+    # the arith functions against the oracle on seeded images of every reference PType
+    rng = np.random.default_rng(21)
+    for pt, dt in (("GUINT8", np.uint8), ("GINT32", np.int32), ("GINT64", np.int64), ("F32", np.float32),
+                   ("D64", np.float64)):
+        a = rng.integers(0, 200, size=(7, 9)).astype(dt)
+        b = rng.integers(0, 200, size=(7, 9)).astype(dt)
+        if dt in (np.float32, np.float64):
+            a, b = a + dt(0.5), b - dt(0.25)
+        exp = {"ArithInvertImage": oracle.invert(a), "ArithSupImage": oracle.sup(a, b), "ArithInfImage": oracle.inf(a, b),
+               "ArithSaturatedAddImage": oracle.sadd_image(a, b), "ArithSaturatedSubImage": oracle.ssub_image(a, b),
+               "ArithSaturatedAddConstant": oracle.sadd_const(a, 77), "ArithSaturatedSubConstant": oracle.ssub_const(a, 77)}
+        for op, e in exp.items():
+            two = op.endswith("Image") and op != "ArithInvertImage"
+            lines.append("|".join([op, "", "", "", "77" if "Constant" in op else "", _arr_str(pt, a),
+                                   _arr_str(pt, b) if two else "", _arr_str(pt, e)]))
     f = tmp_path / "cases.txt"
     f.write_text("\n".join(lines) + "\n")
     r = subprocess.run([BIN, str(f)], capture_output=True, text=True, timeout=120)
