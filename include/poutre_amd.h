@@ -154,6 +154,12 @@ int pm_arith_ssub_const(const pm_image *in, double c, pm_image *out, void *strea
 int pm_arith_sadd_const(const pm_image *in, double c, pm_image *out, void *stream);
 int pm_arith_invert(const pm_image *in, pm_image *out, void *stream);
 
+/* ---- ConvertInto / CopyInto (include/poutre/pixel_processing/copy_convert.hpp:49-52): out =
+ * in converted pixel by pixel to out's PType (a C cast, as the reference's static_cast; float to
+ * integer is defined only for values the integer type holds), a copy when the PTypes agree.
+ * Same shape; out may be in only when the PTypes agree. */
+int pm_convert_into(const pm_image *in, pm_image *out, void *stream);
+
 /* ---- poutre::CompareImage, all eight overloads (include/poutre/pixel_processing/compare.hpp:31-84,
  * src/pixel_processing/compare.cpp:601-): out = (in OP comp) ? vtrue : vfalse, pixel by pixel,
  * the input pixel on the left.  Each operand is an image (image != NULL, same shape as in) or

@@ -126,6 +126,13 @@ std::unique_ptr<IInterface> Create(const std::vector<std::size_t> &dims, Compoun
 std::unique_ptr<IInterface> CreateDevice(const std::vector<std::size_t> &dims, PType ptype);
 std::unique_ptr<IInterface> CloneGeometry(const IInterface &i_img);
 void CopyInto(const IInterface &i_src, IInterface &o_dst);
+// copy_convert.hpp:31-52: Clone / Convert / ConvertGeometry give an image where i_img lives
+// (host or device); ConvertInto converts pixel by pixel (a C cast) into o_img's PType
+std::unique_ptr<IInterface> Clone(const IInterface &i_img);
+std::unique_ptr<IInterface> Convert(const IInterface &i_img, CompoundType ctype, PType ptype);
+std::unique_ptr<IInterface> ConvertGeometry(const IInterface &i_img, CompoundType ctype, PType ptype);
+std::unique_ptr<IInterface> ConvertGeometry(const IInterface &i_img, PType ptype);
+void ConvertInto(const IInterface &i_img, IInterface &o_img);
 
 // KAT fixture format "<Scalar> <GUINT8|GINT32|GINT64|F32|D64> <rank> <dims...> <values...>"
 std::unique_ptr<IInterface> ImageFromString(const std::string &str);

@@ -34,7 +34,8 @@ __all__ = ["static_nl_se", "static_compound_se", "RecType", "erode", "dilate", "
            "version", "PType", "CompoundType", "factory_image", "from_string", "to_string", "as_sizes_compatible",
            "as_types_compatible", "are_images_different", "CompOpType", "compare", "arith_invert", "arith_sadd_image",
            "arith_invert_image", "arith_inf_image", "arith_sup_image", "arith_saturated_add_image",
-           "arith_saturated_sub_image", "arith_saturated_add_constant", "arith_saturated_sub_constant"]
+           "arith_saturated_sub_image", "arith_saturated_add_constant", "arith_saturated_sub_constant", "clone",
+           "clone_geometry", "convert", "convert_geometry", "copy_into", "convert_into"]
 
 
 class static_nl_se(enum.IntEnum):
@@ -230,6 +231,45 @@ def to_string(img):
     else:
         vals = [str(int(v)) for v in a]
     return " ".join(["Scalar", _PT_INFO[pt][0], str(t.dim())] + [str(d) for d in t.shape] + vals)
+
+
+def convert_into(img, out):
+    """poutre::ConvertInto (copy_convert.hpp:52; pypoutre.convert_into): out = img converted
+    pixel by pixel to out's type (a C cast), on the device."""
+    di, do = _desc(img), _desc(out)
+    _call("pm_convert_into", ctypes.byref(di), ctypes.byref(do), _stream(img))
+    return out
+
+
+def copy_into(img, out):
+    """poutre::CopyInto (copy_convert.hpp:49): out = img (same shape and type)."""
+    a, b = _img(img), _img(out)
+    if tuple(a.shape) != tuple(b.shape):
+        raise PoutreError(-1, "CopyInto images have not compatible sizes")
+    if a.dtype != b.dtype:
+        raise PoutreError(-2, "CopyInto images must have compatible types")
+    return convert_into(img, out)
+
+
+def clone(img):
+    """poutre::Clone (copy_convert.hpp:31): a new image equal to img."""
+    return convert_into(img, _torch().empty_like(_img(img)))
+
+
+def clone_geometry(img):
+    """poutre::CloneGeometry (copy_convert.hpp:34): a zero-filled image of img's shape and type."""
+    return _torch().zeros_like(_img(img))
+
+
+def convert_geometry(img, ctype, ptype):
+    """poutre::ConvertGeometry (copy_convert.hpp:42): a zero-filled image of img's shape in ptype."""
+    t = _img(img)
+    return factory_image(tuple(t.shape), ctype, ptype, device=t.device)
+
+
+def convert(img, ctype, ptype):
+    """poutre::Convert (copy_convert.hpp:38): img converted pixel by pixel to ptype."""
+    return convert_into(img, convert_geometry(img, ctype, ptype))
 
 
 def as_sizes_compatible(a, b):

@@ -80,6 +80,7 @@ SIGNATURES = {
     "pm_arith_sadd_const": [_P, _D, _P, _VP],
     "pm_arith_sadd_image": [_P, _P, _P, _VP],
     "pm_arith_invert": [_P, _P, _VP],
+    "pm_convert_into": [_P, _P, _VP],
     "pm_compare": [_P, _I, ctypes.POINTER(PmOperand), ctypes.POINTER(PmOperand), ctypes.POINTER(PmOperand), _P, _VP],
     "pm_maxplus_fwd": [_VP, _I64, _I64, _I64, _VP, _I, _I, _VP, _VP],
     "pm_maxplus_bwd": [_VP, _VP, _I64, _I64, _I64, _VP, _I, _I, _VP, _VP, _VP],

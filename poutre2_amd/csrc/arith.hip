@@ -94,6 +94,13 @@ __global__ __launch_bounds__(256) void cmp_kernel(const TI *in, int op, const TI
   }
 }
 
+// ConvertInto / CopyInto (copy_convert_t.hpp:40-80): o = static_cast<TO>(a) per pixel
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void cvt_kernel(const TI *__restrict__ a, TO *__restrict__ o, int64_t n)
+{
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) o[i] = (TO)a[i];
+}
+
 // marker of dynamic_pseudo_opening/closing (extrema.cpp:90-93,110-113) in one pass:
 // mask = (ext == tmp) ? img : E;  marker = (ext != E) ? mask : E, E = lowest (opening) / highest
 template <typename T, bool OPENING>
@@ -282,6 +289,27 @@ static int compare_entry(const pm_image *in, int op, const pm_operand *comp, con
   });
 }
 
+static int convert_entry(const pm_image *in, pm_image *out, hipStream_t s)
+{
+  clear_error();
+  PM_TRY(check_image(in, "ConvertInto"));
+  PM_TRY(check_image(out, "ConvertInto"));
+  if (!same_shape(in, out)) PM_FAIL(PM_E_SIZE, "ConvertInto images have not compatible sizes");
+  const int64_t n = numel(*in);
+  if (n == 0 || (in->data == out->data && in->ptype == out->ptype)) return PM_OK;
+  if (in->data == out->data) PM_FAIL(PM_E_ALIAS, "ConvertInto: input and output of different types share memory");
+  return dispatch_ptype(in->ptype, [&](auto ti) -> int {
+    using TI = typename decltype(ti)::type;
+    return dispatch_ptype(out->ptype, [&](auto to) -> int {
+      using TO = typename decltype(to)::type;
+      hipLaunchKernelGGL((cvt_kernel<TI, TO>), dim3(grid1d(n, 256, 8192)), dim3(256), 0, s, (const TI *)in->data,
+                         (TO *)out->data, n);
+      PM_HIP(hipGetLastError());
+      return PM_OK;
+    });
+  });
+}
+
 static int invert_entry(const pm_image *in, pm_image *out, hipStream_t s)
 {
   clear_error();
@@ -305,6 +333,7 @@ extern "C" {
 int pm_arith_sadd_image(const pm_image *a, const pm_image *b, pm_image *out, void *stream)
 { return arith2<EW_SADD>(a, b, out, stream, "ArithSaturatedAddImage"); }
 int pm_arith_invert(const pm_image *in, pm_image *out, void *stream) { return invert_entry(in, out, as_stream(stream)); }
+int pm_convert_into(const pm_image *in, pm_image *out, void *stream) { return convert_entry(in, out, as_stream(stream)); }
 int pm_compare(const pm_image *in, int op, const pm_operand *comp, const pm_operand *vtrue, const pm_operand *vfalse,
                pm_image *out, void *stream)
 {

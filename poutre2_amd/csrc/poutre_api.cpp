@@ -449,6 +449,36 @@ void compare_any(const IInterface &in, CompOpType op, const CmpArg &c, const Cmp
 }
 }  // namespace
 
+std::unique_ptr<IInterface> ConvertGeometry(const IInterface &i, CompoundType ctype, PType ptype)
+{
+  if (ctype != CompoundType::CompoundType_Scalar) fail("ConvertGeometry: only scalar images are supported");
+  if (i.OnDevice()) return CreateDevice(i.GetShape(), ptype);
+  return Create(i.GetShape(), ctype, ptype);
+}
+std::unique_ptr<IInterface> ConvertGeometry(const IInterface &i, PType ptype)
+{
+  return ConvertGeometry(i, CompoundType::CompoundType_Scalar, ptype);
+}
+void ConvertInto(const IInterface &i, IInterface &o)
+{
+  AssertSizesCompatible(i, o, "ConvertInto images have not compatible sizes");
+  Staged si(i, true), so(o, false);
+  pm_check(pm_convert_into(si.desc(), so.desc(), nullptr));
+  so.download();
+}
+std::unique_ptr<IInterface> Convert(const IInterface &i, CompoundType ctype, PType ptype)
+{
+  auto o = ConvertGeometry(i, ctype, ptype);
+  ConvertInto(i, *o);
+  return o;
+}
+std::unique_ptr<IInterface> Clone(const IInterface &i)
+{
+  auto o = CloneGeometry(i);
+  CopyInto(i, *o);
+  return o;
+}
+
 void ArithInvertImage(const IInterface &i, IInterface &o)
 {
   arith_unary(i, o, "ArithInvertImage", [&](pm_image *a, pm_image *b) { return pm_arith_invert(a, b, nullptr); });

@@ -78,6 +78,19 @@ static void run_compare(const std::vector<std::string> &f, bool dev, const IInte
 static int run_case(const std::vector<std::string> &f, bool dev)
 {
   const std::string &op = f[0];
+  if (op == "ConvertInto" || op == "Clone") {
+    auto a = on(dev, f[5]);
+    std::unique_ptr<IInterface> out;
+    if (op == "Clone") out = Clone(*a);
+    else {
+      out = on(dev, f[7]);  // geometry and PType of the expected image
+      ConvertInto(*a, *out);
+    }
+    const std::string got = ImageToString(*out), exp = ImageToString(*ImageFromString(f[7]));
+    if (got == exp) return 0;
+    std::printf("FAIL %s dev=%d\n  got: %s\n  exp: %s\n", op.c_str(), (int)dev, got.c_str(), exp.c_str());
+    return 1;
+  }
   if (op.rfind("compare:", 0) == 0) {
     auto a = on(dev, f[5]);
     auto out = on(dev, f[7]);  // overwritten: only its geometry and PType matter

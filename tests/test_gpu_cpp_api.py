@@ -59,6 +59,13 @@ def test_cpp_api_kats(tmp_path):
             two = op.endswith("Image") and op != "ArithInvertImage"
             lines.append("|".join([op, "", "", "", "77" if "Constant" in op else "", _arr_str(pt, a),
                                    _arr_str(pt, b) if two else "", _arr_str(pt, e)]))
+    # ConvertInto between PTypes (a C cast: numpy's astype) and Clone
+    c = rng.integers(0, 120, size=(5, 8))
+    for pin, din in (("GUINT8", np.uint8), ("GINT32", np.int32), ("D64", np.float64)):
+        for pout, dout in (("GINT64", np.int64), ("F32", np.float32), ("GUINT8", np.uint8)):
+            x = c.astype(din) + (din(0.75) if din is np.float64 else din(0))
+            lines.append("|".join(["ConvertInto", "", "", "", "", _arr_str(pin, x), "", _arr_str(pout, x.astype(dout))]))
+        lines.append("|".join(["Clone", "", "", "", "", _arr_str(pin, c.astype(din)), "", _arr_str(pin, c.astype(din))]))
     f = tmp_path / "cases.txt"
     f.write_text("\n".join(lines) + "\n")
     r = subprocess.run([BIN, str(f)], capture_output=True, text=True, timeout=120)

@@ -610,3 +610,32 @@ def test_sadd_image_and_invert(dtype):
     t = g(a)
     pm.arith_invert(t, out=t)
     np.testing.assert_array_equal(h(t), oracle.invert(a))
+
+
+@pytest.mark.parametrize("tin", DTYPES)
+def test_convert_copy_clone(tin):
+    """ConvertInto / Convert / Clone / CopyInto (copy_convert.hpp:31-52): a C cast per pixel
+    (numpy's astype is one) for values every type holds; negative integers into unsigned types
+    wrap modulo 2^bits"""
+    r = np.random.default_rng(407)
+    a = (r.integers(-100, 200, size=(33, 70)) if np.dtype(tin).kind == "i" else r.integers(0, 200, size=(33, 70)))
+    a = (a + r.random((33, 70)) * 0.9 if np.dtype(tin).kind == "f" else a).astype(tin)
+    for tout, pt in ((np.uint8, pm.PType.GrayUINT8), (np.uint16, pm.PType.GrayUINT16), (np.int32, pm.PType.GrayINT32),
+                     (np.int64, pm.PType.GrayINT64), (np.float32, pm.PType.F32), (np.float64, pm.PType.D64)):
+        if np.dtype(tin).kind == "f" and np.dtype(tout).kind in "iu":
+            src = np.abs(a)  # float to integer: values the integer type holds
+        else:
+            src = a
+        got = pm.convert(g(src), pm.CompoundType.Scalar, pt)
+        np.testing.assert_array_equal(h(got), src.astype(tout), err_msg=f"{tin} -> {tout}")
+    c = pm.clone(g(a))
+    np.testing.assert_array_equal(h(c), a)
+    z = pm.clone_geometry(g(a))
+    assert z.shape == a.shape and z.dtype == c.dtype and int((z != 0).sum()) == 0
+    d = pm.factory_image(a.shape, pm.CompoundType.Scalar, pm.PType(pm._ptype(c)))
+    pm.copy_into(c, d)
+    np.testing.assert_array_equal(h(d), a)
+    with pytest.raises(pm.PoutreError):
+        pm.copy_into(c, g(np.zeros((33, 71), tin)))
+    with pytest.raises(pm.PoutreError):
+        pm.convert_into(c, g(np.zeros((33, 71), np.float64)))
