@@ -35,7 +35,7 @@ __all__ = ["static_nl_se", "static_compound_se", "RecType", "erode", "dilate", "
            "as_types_compatible", "are_images_different", "CompOpType", "compare", "arith_invert", "arith_sadd_image",
            "arith_invert_image", "arith_inf_image", "arith_sup_image", "arith_saturated_add_image",
            "arith_saturated_sub_image", "arith_saturated_add_constant", "arith_saturated_sub_constant", "clone",
-           "clone_geometry", "convert", "convert_geometry", "copy_into", "convert_into"]
+           "clone_geometry", "convert", "convert_geometry", "copy_into", "convert_into", "SE", "se_type"]
 
 
 class static_nl_se(enum.IntEnum):
@@ -50,6 +50,58 @@ class static_nl_se(enum.IntEnum):
     segz3d = 8
     cross3d = 9
     square3d = 10
+
+
+class se_type(enum.IntEnum):
+    """poutre::se::se_type (python/structuring_element/se_types.cpp:12-17)."""
+    runtime = 0
+    neighbor_list = 1
+    image = 2
+    chain = 3
+
+
+class SE:
+    """A runtime structuring element: the neighbour list of poutre::se::neighbor_list_t
+    (neighbor_list_se_t.hpp:36-228) with the methods pypoutre's ``SE`` binds
+    (python/structuring_element/se_interface.cpp:15-23).  Built from offsets ({dx}, {dy, dx}
+    or {dz, dy, dx}); ``dilate`` / ``erode`` take it as the runtime neighbour-list SE."""
+
+    def __init__(self, offsets):
+        self._c = [tuple(int(v) for v in o) for o in offsets]
+        if len({len(c) for c in self._c}) > 1:
+            raise PoutreError(-5, "neighbor_list: coordinate of wrong rank")
+
+    def __iter__(self):
+        return iter(self._c)
+
+    def __len__(self):
+        return len(self._c)
+
+    def __repr__(self):
+        return f"SE({self._c})"
+
+    def transpose(self):
+        return SE([tuple(-v for v in c) for c in self._c])
+
+    def type(self):
+        return se_type.neighbor_list
+
+    def remove_center(self):
+        return SE([c for c in self._c if any(c)])
+
+    def size(self):
+        return len(self._c)
+
+    def clone(self):
+        return SE(self._c)
+
+    def is_equal(self, other):
+        """the same offsets in the same order"""
+        return isinstance(other, SE) and self._c == other._c
+
+    def is_equal_unordered(self, other):
+        """the same sets of offsets (sorted, repeats removed: neighbor_list_se_t.hpp:177-195)"""
+        return isinstance(other, SE) and sorted(set(self._c)) == sorted(set(other._c))
 
 
 class static_compound_se(enum.IntEnum):

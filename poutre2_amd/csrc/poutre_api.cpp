@@ -273,13 +273,16 @@ bool neighbor_list::is_equal(const IStructuringElement &o) const noexcept
   auto p = dynamic_cast<const neighbor_list *>(&o);
   return p && p->m_rank == m_rank && p->m_coords == m_coords;
 }
+// the same SETS of offsets (neighbor_list_se_t.hpp:177-195: sorted, repeats removed)
 bool neighbor_list::is_equal_unordered(const IStructuringElement &o) const noexcept
 {
   auto p = dynamic_cast<const neighbor_list *>(&o);
-  if (!p || p->m_rank != m_rank || p->m_coords.size() != m_coords.size()) return false;
+  if (!p || p->m_rank != m_rank) return false;
   auto a = m_coords, b = p->m_coords;
   std::sort(a.begin(), a.end());
   std::sort(b.begin(), b.end());
+  a.erase(std::unique(a.begin(), a.end()), a.end());
+  b.erase(std::unique(b.begin(), b.end()), b.end());
   return a == b;
 }
 

@@ -207,6 +207,9 @@ int main(int argc, char **argv)
   if (up.is_symmetric() || lo.is_symmetric()) fails++;
   if (se::SESquare2D.maximum_extension(0, true) != 1 || se::SESquare2D.maximum_extension(1, false) != 1) fails++;
   if (se::SECross3D.GetSize() != 7 || se::SESquare3D.GetSize() != 27) fails++;
+  // unordered equality compares the SETS of offsets (repeats do not count)
+  const se::neighbor_list rep(2, {{0, 0}, {0, 0}, {-1, 0}}), once(2, {{-1, 0}, {0, 0}});
+  if (!rep.is_equal_unordered(once) || rep.is_equal(once)) fails++;
   std::printf("api_kats: %d cases run (%d skipped), %d failures\n", n, skipped, fails);
   return fails ? 1 : 0;
 }

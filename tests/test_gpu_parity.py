@@ -639,3 +639,11 @@ def test_convert_copy_clone(tin):
         pm.copy_into(c, g(np.zeros((33, 71), tin)))
     with pytest.raises(pm.PoutreError):
         pm.convert_into(c, g(np.zeros((33, 71), np.float64)))
+
+
+def test_se_object_is_a_runtime_neighbour_list():
+    img = rand((40, 50), np.int32, 408)
+    se = pm.SE([(0, -1), (0, 0), (0, 1), (-1, 2)])
+    for dil in (True, False):
+        f = pm.dilate if dil else pm.erode
+        np.testing.assert_array_equal(h(f(g(img), se)), oracle.nl(img, list(se), dil))

@@ -1,7 +1,8 @@
 """The image-model names of the reference's Python module (python/base/image_interface.cpp:13-31,
 python/base/types.cpp:10-23) on host tensors: factory_image, from_string / to_string (the
-fixture format of image_interface.cpp:407-909), the compatibility predicates, DLPack input.
-No GPU: these functions run on the host."""
+fixture format of image_interface.cpp:407-909), the compatibility predicates, DLPack input,
+and the SE object of python/structuring_element/se_interface.cpp.  No GPU: these functions run
+on the host."""
 import numpy as np
 import pytest
 
@@ -86,3 +87,20 @@ def test_dlpack_producers_are_images():
     assert not pm.are_images_different(arr, torch.from_dlpack(arr))
     with pytest.raises(pm.PoutreError):
         pm.to_string([1, 2, 3])
+
+
+def test_se_object():
+    """pypoutre's SE methods (python/structuring_element/se_interface.cpp:15-23) on a neighbour
+    list; unordered equality compares sets (neighbor_list_se_t.hpp:177-195)"""
+    sq = pm.SE([(dy, dx) for dy in (-1, 0, 1) for dx in (-1, 0, 1)])
+    assert sq.size() == 9 and sq.type() == pm.se_type.neighbor_list
+    rc = sq.remove_center()
+    assert rc.size() == 8 and (0, 0) not in list(rc)
+    assert rc.is_equal_unordered(rc.transpose()) and not rc.is_equal(rc.transpose())
+    seg = pm.SE([(0, -1), (0, 0), (0, 1), (0, 2)])
+    assert list(seg.transpose()) == [(0, 1), (0, 0), (0, -1), (0, -2)]
+    assert not seg.is_equal_unordered(seg.transpose())
+    assert pm.SE([(0, 0), (0, 0), (-1, 0)]).is_equal_unordered(pm.SE([(-1, 0), (0, 0)]))
+    assert seg.clone().is_equal(seg) and not seg.is_equal([(0, -1), (0, 0), (0, 1), (0, 2)])
+    with pytest.raises(pm.PoutreError):
+        pm.SE([(0, 0), (0, 0, 1)])
