@@ -38,6 +38,7 @@ struct BatchLayout {
   int64_t tmp_bytes = 0;
   size_t first[NG] = {};
   std::vector<BatchImage> all;
+  int *queue = nullptr;  // per group: the next item of the fused kernel (scratch)
 };
 
 static int batch_layout(int n, const pm_image *ins, const int *radii, pm_image *dil_outs, pm_image *open_outs,
@@ -82,12 +83,17 @@ static int batch_layout(int n, const pm_image *ins, const int *radii, pm_image *
   return PM_OK;
 }
 
-// Scratch layout: eroded images (256-byte aligned each), then the descriptor table.
-static size_t batch_scratch_bytes(const BatchLayout &L)
+// Scratch layout: eroded images (256-byte aligned each), the descriptor table, then one queue
+// counter per group (256-byte aligned).
+static size_t batch_desc_bytes(const BatchLayout &L)
 {
   size_t d = 0;
   for (int g = 0; g < BatchLayout::NG; g++) d += L.grp[g].size() * sizeof(BatchImage);
-  return (size_t)L.tmp_bytes + d;
+  return (d + 255) & ~(size_t)255;
+}
+static size_t batch_scratch_bytes(const BatchLayout &L)
+{
+  return (size_t)L.tmp_bytes + batch_desc_bytes(L) + BatchLayout::NG * sizeof(int);
 }
 
 // point the descriptors at scratch `base` and build the flat host table
@@ -105,6 +111,7 @@ static BatchImage *batch_bind(BatchLayout &L, uint8_t *base)
     L.first[g] = L.all.size();
     L.all.insert(L.all.end(), L.grp[g].begin(), L.grp[g].end());
   }
+  L.queue = (int *)(base + L.tmp_bytes + batch_desc_bytes(L));
   return (BatchImage *)(base + L.tmp_bytes);
 }
 
@@ -131,8 +138,9 @@ static int batch_enqueue(const BatchLayout &L, const BatchImage *dimg, const hip
       const int64_t items = L.tiles[g] + L.otiles[g];
       int64_t G = (int64_t)(occ > 0 ? occ : 1) * cus;
       if (G > items) G = items;
+      PM_TRY(fill_bytes(L.queue + g, 0, sizeof(int), s));  // a kernel: safe under capture
       hipLaunchKernelGGL(k, dim3((unsigned)G), dim3(FTHREADS), lds, s, dimg + L.first[g], (int)L.grp[g].size(),
-                         (int)L.tiles[g], (int)items);
+                         (int)L.tiles[g], (int)items, L.queue + g);
       PM_HIP(hipGetLastError());
       continue;
     }

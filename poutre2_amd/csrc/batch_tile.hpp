@@ -67,23 +67,34 @@ __global__ __launch_bounds__(FTHREADS) void batch_sq_kernel(const BatchImage *__
 // radius group costs more than the fused tile's larger halo.
 constexpr int BATCH_FUSED_RMAX = 15;
 
-// Radius groups up to BATCH_FUSED_RMAX: one launch.  Items [0, ndil) are the dilation tiles,
-// items [ndil, nitems) the fused-opening tiles (sq_open.hpp) of the same images, so no
-// eroded image goes through memory and no second launch is needed.
+// Radius groups up to BATCH_FUSED_RMAX: one launch over ndil dilation tiles and nitems - ndil
+// fused-opening tiles (sq_open.hpp) of the same images, so no eroded image goes through
+// memory and no second launch is needed.  Workgroups take items from a queue (*queue: the
+// next item, zero at launch): a workgroup that becomes resident late, when other groups'
+// kernels share the GPU, takes fewer items instead of a fixed share.  The opening tiles, the
+// heavier items, come first.
 template <int R>
 __global__ __launch_bounds__(FTHREADS) void batch_fused_kernel(const BatchImage *__restrict__ im, int nimg, int ndil,
-                                                               int nitems)
+                                                               int nitems, int *queue)
 {
   constexpr int K = sq_k_of(R), TW = sq_tw(K, R), FO = open_fo(R);
   static_assert(R <= BATCH_FUSED_RMAX && TW == OPEN_TW, "fused batch geometry");
   extern __shared__ __align__(16) uint8_t smem[];
-  for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
-    if (item < ndil) {
-      const BatchImage b = im[batch_find(im, nimg, item)];
-      const int tl = item - b.tile0, ty = tl / b.tiles_x, tx = tl - ty * b.tiles_x;
+  __shared__ int s_item;
+  const int nopen = nitems - ndil;
+  for (;;) {
+    __syncthreads();  // every thread has read the previous item
+    if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
+    __syncthreads();
+    const int item = s_item;
+    if (item >= nitems) break;
+    if (item >= nopen) {
+      const int d = item - nopen;
+      const BatchImage b = im[batch_find(im, nimg, d)];
+      const int tl = d - b.tile0, ty = tl / b.tiles_x, tx = tl - ty * b.tiles_x;
       batch_tile<R, true>(smem, b.in, b.dil, b, ty * FT, tx * TW);
     } else {
-      const int t = item - ndil;
+      const int t = item;
       const BatchImage b = im[batch_find<true>(im, nimg, t)];
       const int tl = t - b.otile0, ty = tl / b.tiles_x, tx = tl - ty * b.tiles_x;
       const int y0 = ty * FO, x0 = tx * OPEN_TW;
@@ -96,7 +107,7 @@ __global__ __launch_bounds__(FTHREADS) void batch_fused_kernel(const BatchImage 
 }
 
 typedef void (*batch_kernel_t)(const BatchImage *, int, int);
-typedef void (*batch_fused_t)(const BatchImage *, int, int, int);
+typedef void (*batch_fused_t)(const BatchImage *, int, int, int, int *);
 // fused kernel of radius r in [1, BATCH_FUSED_RMAX] (batch_open.hip)
 batch_fused_t batch_fused_for(int r);
 
