@@ -21,7 +21,11 @@
 
 namespace pm {
 
-constexpr int MTW = 64, MTH = 16, MTHREADS = 256;
+// tiles of MTW columns; MTH_F rows in the forward, MTH_B in the backward, whose four LDS arrays
+// (inputs with a 2R halo, gradients, selected taps) limit residency (measured on [64,32,256,256]
+// 3x3: forward 16 / 32 / 64 rows 0.36 / 0.32 / 0.31 ms, backward best at 16-32 rows, 64
+// rows 1.5-1.6x slower)
+constexpr int MTW = 64, MTH_F = 64, MTH_B = 32, MTHREADS = 256;
 
 template <bool DIL> __device__ __forceinline__ float pad_value() { return DIL ? -INFINITY : INFINITY; }
 
@@ -63,18 +67,18 @@ template <int K, bool DIL, bool FLAT>
 __global__ __launch_bounds__(MTHREADS) void mp_fwd_kernel(const float *__restrict__ x, const float *__restrict__ wg,
                                                           float *__restrict__ y, int H, int W, int tiles_x, int tiles_per_img)
 {
-  constexpr int R = (K - 1) / 2, RC = MTW + K - 1, RR = MTH + K - 1;
+  constexpr int R = (K - 1) / 2, RC = MTW + K - 1, RR = MTH_F + K - 1;
   __shared__ float X[RR * RC];
   __shared__ float w[K * K];
   const int img = blockIdx.x / tiles_per_img, t = blockIdx.x - img * tiles_per_img;
   const int ty = t / tiles_x, tx = t - ty * tiles_x;
-  const int y0 = ty * MTH, x0 = tx * MTW;
+  const int y0 = ty * MTH_F, x0 = tx * MTW;
   const float *xi = x + (int64_t)img * H * W;
   if (!FLAT && threadIdx.x < K * K) w[threadIdx.x] = wg[threadIdx.x];
   load_region(X, RR, RC, xi, H, W, y0 - R, x0 - R, pad_value<DIL>());
   __syncthreads();
   float *yi = y + (int64_t)img * H * W;
-  for (int e = threadIdx.x; e < MTH * MTW; e += MTHREADS) {
+  for (int e = threadIdx.x; e < MTH_F * MTW; e += MTHREADS) {
     const int i = e / MTW, j = e - i * MTW;
     const int gy = y0 + i, gx = x0 + j;
     if (gy >= H || gx >= W) continue;
@@ -91,8 +95,8 @@ __global__ __launch_bounds__(MTHREADS) void mp_bwd_kernel(const float *__restric
                                                           int tiles_per_img)
 {
   constexpr int R = (K - 1) / 2;
-  constexpr int QC = MTW + K - 1, QR = MTH + K - 1;            // outputs q whose window reaches the tile
-  constexpr int XC = MTW + 2 * (K - 1), XR = MTH + 2 * (K - 1);  // inputs those q read
+  constexpr int QC = MTW + K - 1, QR = MTH_B + K - 1;            // outputs q whose window reaches the tile
+  constexpr int XC = MTW + 2 * (K - 1), XR = MTH_B + 2 * (K - 1);  // inputs those q read
   __shared__ float X[XR * XC];
   __shared__ float G[QR * QC];
   __shared__ unsigned char S[QR * QC];
@@ -100,7 +104,7 @@ __global__ __launch_bounds__(MTHREADS) void mp_bwd_kernel(const float *__restric
   __shared__ float red[MTHREADS];
   const int img = blockIdx.x / tiles_per_img, t = blockIdx.x - img * tiles_per_img;
   const int ty = t / tiles_x, tx = t - ty * tiles_x;
-  const int y0 = ty * MTH, x0 = tx * MTW;
+  const int y0 = ty * MTH_B, x0 = tx * MTW;
   const int64_t ib = (int64_t)img * H * W;
   if (!FLAT && threadIdx.x < K * K) w[threadIdx.x] = wg[threadIdx.x];
   load_region(X, XR, XC, x + ib, H, W, y0 - 2 * R, x0 - 2 * R, pad_value<DIL>());
@@ -115,7 +119,7 @@ __global__ __launch_bounds__(MTHREADS) void mp_bwd_kernel(const float *__restric
   }
   __syncthreads();
   // gather: p = (y0 + i, x0 + j) receives gy[q] from q = p - t when sel(q) == t
-  for (int e = threadIdx.x; e < MTH * MTW; e += MTHREADS) {
+  for (int e = threadIdx.x; e < MTH_B * MTW; e += MTHREADS) {
     const int i = e / MTW, j = e - i * MTW;
     const int py = y0 + i, px = x0 + j;
     float acc = 0.0f;
@@ -136,7 +140,7 @@ __global__ __launch_bounds__(MTHREADS) void mp_bwd_kernel(const float *__restric
   float part[K * K];
 #pragma unroll
   for (int k = 0; k < K * K; k++) part[k] = 0.0f;
-  for (int e = threadIdx.x; e < MTH * MTW; e += MTHREADS) {
+  for (int e = threadIdx.x; e < MTH_B * MTW; e += MTHREADS) {
     const int i = e / MTW, j = e - i * MTW;
     const int qe = (i + R) * QC + (j + R);
     const int s = S[qe];
@@ -184,10 +188,10 @@ static int mp_check(int64_t B, int64_t H, int64_t W, int K, const void *a, const
   return PM_OK;
 }
 
-static int64_t mp_blocks(int64_t B, int64_t H, int64_t W, int &tiles_x, int &per)
+static int64_t mp_blocks(int64_t B, int64_t H, int64_t W, int th, int &tiles_x, int &per)
 {
   tiles_x = (int)((W + MTW - 1) / MTW);
-  per = tiles_x * (int)((H + MTH - 1) / MTH);
+  per = tiles_x * (int)((H + th - 1) / th);
   return B * per;
 }
 
@@ -209,7 +213,7 @@ extern "C" int pm_maxplus_fwd(const float *x, int64_t B, int64_t H, int64_t W, c
   PM_TRY(mp_check(B, H, W, K, x, y));
   if (x == y && B * H * W > 0) PM_FAIL(PM_E_ALIAS, "maxplus: output must be != input");
   int tiles_x, per;
-  const int64_t nb = mp_blocks(B, H, W, tiles_x, per);
+  const int64_t nb = mp_blocks(B, H, W, MTH_F, tiles_x, per);
   if (nb == 0) return PM_OK;
   if (nb > INT32_MAX) PM_FAIL(PM_E_SIZE, "maxplus: batch too large");
   hipStream_t s = as_stream(stream);
@@ -239,7 +243,7 @@ extern "C" int pm_maxplus_bwd(const float *x, const float *gy, int64_t B, int64_
   const bool flat = (w == nullptr);
   if (!flat && !gw) PM_FAIL(PM_E_ARG, "maxplus: gw buffer required for a structuring function");
   int tiles_x, per;
-  const int64_t nb = mp_blocks(B, H, W, tiles_x, per);
+  const int64_t nb = mp_blocks(B, H, W, MTH_B, tiles_x, per);
   hipStream_t s = as_stream(stream);
   if (nb == 0) {
     if (!flat) return fill_bytes(gw, 0, sizeof(float) * K * K, s);
