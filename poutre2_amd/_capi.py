@@ -84,6 +84,8 @@ SIGNATURES = {
     "pm_compare": [_P, _I, ctypes.POINTER(PmOperand), ctypes.POINTER(PmOperand), ctypes.POINTER(PmOperand), _P, _VP],
     "pm_maxplus_fwd": [_VP, _I64, _I64, _I64, _VP, _I, _I, _VP, _VP],
     "pm_maxplus_bwd": [_VP, _VP, _I64, _I64, _I64, _VP, _I, _I, _VP, _VP, _VP],
+    "pm_maxplus_fwd_sel": [_VP, _I64, _I64, _I64, _VP, _I, _I, _VP, _VP, _VP],
+    "pm_maxplus_bwd_sel": [_VP, _VP, _I64, _I64, _I64, _I, _I, _I, _VP, _VP, _VP],
     "pm_batch_dilate_open": [_I, _P, ctypes.POINTER(_I), _P, _P, _VP],
     "pm_batch_reconstruct": [_I, _I, _P, _P, _I, _P, _VP],
     "pm_batch_plan_create": [_I, _P, ctypes.POINTER(_I), _P, _P, ctypes.POINTER(ctypes.c_void_p)],

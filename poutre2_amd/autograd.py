@@ -1,5 +1,8 @@
 """Differentiable float32 morphology on PyTorch-ROCm, backed by the hand-written kernels of
-libpoutre_amd.so (pm_maxplus_fwd / pm_maxplus_bwd).
+libpoutre_amd.so.  When autograd records the call (grad mode on and x or w requiring grad) the
+forward keeps the selected tap of every output, one byte, for the backward
+(pm_maxplus_fwd_sel / pm_maxplus_bwd_sel); otherwise it runs the plain max / min kernel
+(pm_maxplus_fwd) and stores nothing.
 
     y = dilation2d(x, k=3)            flat dilation by a k x k square
     y = erosion2d(x, k=7)             flat erosion
@@ -36,7 +39,7 @@ def _dims(x):
 
 class _MorphFn(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, x, w, k, dilate):
+    def forward(ctx, x, w, k, dilate, record):
         if x.dtype != torch.float32 or not x.is_cuda:
             raise TypeError("x must be a float32 device tensor")
         x = x.contiguous()
@@ -47,31 +50,43 @@ class _MorphFn(torch.autograd.Function):
                 raise ValueError("w must be (K, K)")
         B, H, W = _dims(x)
         y = torch.empty_like(x)
-        _capi.check(_capi.lib().pm_maxplus_fwd(_ptr(x), B, H, W, _ptr(w), int(k), int(dilate), _ptr(y), _stream(x)))
-        ctx.save_for_backward(x, w)
-        ctx.k, ctx.dilate = int(k), int(dilate)
+        ctx.k, ctx.dilate, ctx.with_w = int(k), int(dilate), w is not None
+        if not record:
+            _capi.check(_capi.lib().pm_maxplus_fwd(_ptr(x), B, H, W, _ptr(w), int(k), int(dilate), _ptr(y),
+                                                   _stream(x)))
+            return y
+        # the selected tap of every output (1 byte) is what the backward needs, not x
+        sel = torch.empty(x.shape, dtype=torch.uint8, device=x.device)
+        _capi.check(_capi.lib().pm_maxplus_fwd_sel(_ptr(x), B, H, W, _ptr(w), int(k), int(dilate), _ptr(y), _ptr(sel),
+                                                   _stream(x)))
+        ctx.save_for_backward(sel)
         return y
 
     @staticmethod
     def backward(ctx, gy):
-        x, w = ctx.saved_tensors
+        (sel,) = ctx.saved_tensors
         gy = gy.contiguous()
-        B, H, W = _dims(x)
-        gx = torch.empty_like(x)
-        gw = torch.empty_like(w) if w is not None else None
-        _capi.check(_capi.lib().pm_maxplus_bwd(_ptr(x), _ptr(gy), B, H, W, _ptr(w), ctx.k, ctx.dilate, _ptr(gx),
-                                               _ptr(gw), _stream(x)))
-        return gx, (gw if (w is not None and ctx.needs_input_grad[1]) else None), None, None
+        B, H, W = _dims(sel)
+        gx = torch.empty(sel.shape, dtype=torch.float32, device=sel.device)
+        gw = torch.empty((ctx.k, ctx.k), dtype=torch.float32, device=sel.device) if ctx.with_w else None
+        _capi.check(_capi.lib().pm_maxplus_bwd_sel(_ptr(sel), _ptr(gy), B, H, W, ctx.k, ctx.dilate, int(ctx.with_w),
+                                                   _ptr(gx), _ptr(gw), _stream(sel)))
+        return gx, (gw if (ctx.with_w and ctx.needs_input_grad[1]) else None), None, None, None
+
+
+def _recorded(x, w):
+    """True when autograd will record the call, so that a backward can follow"""
+    return torch.is_grad_enabled() and (x.requires_grad or (w is not None and w.requires_grad))
 
 
 def dilation2d(x, k=3, w=None):
     """Flat (k x k square) or max-plus (structuring function w) dilation, differentiable."""
-    return _MorphFn.apply(x, w, k, True)
+    return _MorphFn.apply(x, w, k, True, _recorded(x, w))
 
 
 def erosion2d(x, k=3, w=None):
     """Flat (k x k square) or max-plus (structuring function w) erosion, differentiable."""
-    return _MorphFn.apply(x, w, k, False)
+    return _MorphFn.apply(x, w, k, False, _recorded(x, w))
 
 
 def reference_cpu(x, k=3, w=None, dilate=True):

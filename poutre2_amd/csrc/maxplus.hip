@@ -13,7 +13,8 @@
 // gw is reduced per workgroup in a fixed order into a partials buffer, then summed over
 // workgroups in a fixed order by a second kernel.
 // Memory per pixel: forward reads x, writes y (8 B); backward reads x and gy, writes gx
-// (12 B); the selected taps are recomputed in the backward instead of being stored.
+// (12 B), re-selecting the taps.  The _sel forms store the selected tap of every output in the
+// forward (9 B) and read it in the backward instead of x (9 B, no re-selection).
 #include <cmath>
 
 #include "maxplus.hpp"
@@ -25,7 +26,7 @@ namespace pm {
 // (inputs with a 2R halo, gradients, selected taps) limit residency (measured on [64,32,256,256]
 // 3x3: forward 16 / 32 / 64 rows 0.36 / 0.32 / 0.31 ms, backward best at 16-32 rows, 64
 // rows 1.5-1.6x slower)
-constexpr int MTW = 64, MTH_F = 64, MTH_B = 32, MTHREADS = 256;
+constexpr int MTW = 64, MTH_F = 64, MTH_B = 32, MTH_S = 64, MTHREADS = 256;  // MTH_S: backward from stored taps
 
 template <bool DIL> __device__ __forceinline__ float pad_value() { return DIL ? -INFINITY : INFINITY; }
 
@@ -63,9 +64,12 @@ __device__ __forceinline__ int select_tap(const float *p, int pitch, const float
   return sel;
 }
 
-template <int K, bool DIL, bool FLAT>
+// SEL: also store the selected tap of every output (without it the tap index is dead code and
+// the window reduces to a plain max / min chain)
+template <int K, bool DIL, bool FLAT, bool SEL>
 __global__ __launch_bounds__(MTHREADS) void mp_fwd_kernel(const float *__restrict__ x, const float *__restrict__ wg,
-                                                          float *__restrict__ y, int H, int W, int tiles_x, int tiles_per_img)
+                                                          float *__restrict__ y, uint8_t *__restrict__ selo, int H,
+                                                          int W, int tiles_x, int tiles_per_img)
 {
   constexpr int R = (K - 1) / 2, RC = MTW + K - 1, RR = MTH_F + K - 1;
   __shared__ float X[RR * RC];
@@ -83,8 +87,9 @@ __global__ __launch_bounds__(MTHREADS) void mp_fwd_kernel(const float *__restric
     const int gy = y0 + i, gx = x0 + j;
     if (gy >= H || gx >= W) continue;
     float best;
-    select_tap<K, DIL, FLAT>(&X[i * RC + j], RC, w, best);
+    const int sel = select_tap<K, DIL, FLAT>(&X[i * RC + j], RC, w, best);
     yi[(int64_t)gy * W + gx] = best;
+    if (SEL) selo[(int64_t)img * H * W + (int64_t)gy * W + gx] = (uint8_t)sel;
   }
 }
 
@@ -161,6 +166,75 @@ __global__ __launch_bounds__(MTHREADS) void mp_bwd_kernel(const float *__restric
   }
 }
 
+// the selected taps of a (rows x cols) region at (y0, x0); outside the image: no tap (255)
+__device__ __forceinline__ void load_sel(uint8_t *dst, int rows, int cols, const uint8_t *__restrict__ src, int H, int W,
+                                         int y0, int x0)
+{
+  const int n = rows * cols;
+  for (int e = threadIdx.x; e < n; e += MTHREADS) {
+    const int i = e / cols, j = e - i * cols;
+    const int gy = y0 + i, gx = x0 + j;
+    dst[e] = ((unsigned)gy < (unsigned)H && (unsigned)gx < (unsigned)W) ? src[(int64_t)gy * W + gx] : (uint8_t)255;
+  }
+}
+
+// Backward from the taps the forward stored (sel): the gather and the structuring-function
+// gradient of mp_bwd_kernel without reloading x or re-selecting taps.  WITH_W: compute gw.
+template <int K, bool DIL, bool WITH_W>
+__global__ __launch_bounds__(MTHREADS) void mp_bwd_sel_kernel(const uint8_t *__restrict__ sel, const float *__restrict__ gy,
+                                                              float *__restrict__ gx, float *__restrict__ gw_part, int H,
+                                                              int W, int tiles_x, int tiles_per_img)
+{
+  constexpr int R = (K - 1) / 2;
+  constexpr int QC = MTW + K - 1, QR = MTH_S + K - 1;  // outputs q whose window reaches the tile
+  __shared__ float G[QR * QC];
+  __shared__ uint8_t S[QR * QC];
+  __shared__ float red[MTHREADS];
+  const int img = blockIdx.x / tiles_per_img, t = blockIdx.x - img * tiles_per_img;
+  const int ty = t / tiles_x, tx = t - ty * tiles_x;
+  const int y0 = ty * MTH_S, x0 = tx * MTW;
+  const int64_t ib = (int64_t)img * H * W;
+  load_region(G, QR, QC, gy + ib, H, W, y0 - R, x0 - R, 0.0f);
+  load_sel(S, QR, QC, sel + ib, H, W, y0 - R, x0 - R);
+  __syncthreads();
+  for (int e = threadIdx.x; e < MTH_S * MTW; e += MTHREADS) {
+    const int i = e / MTW, j = e - i * MTW;
+    const int py = y0 + i, px = x0 + j;
+    float acc = 0.0f;
+#pragma unroll
+    for (int dy = 0; dy < K; dy++)
+#pragma unroll
+      for (int dx = 0; dx < K; dx++) {
+        const int qe = (i + 2 * R - dy) * QC + (j + 2 * R - dx);
+        if (S[qe] == dy * K + dx) acc += G[qe];
+      }
+    if (py < H && px < W) gx[ib + (int64_t)py * W + px] = acc;
+  }
+  if (!WITH_W) return;
+  float part[K * K];
+#pragma unroll
+  for (int k = 0; k < K * K; k++) part[k] = 0.0f;
+  for (int e = threadIdx.x; e < MTH_S * MTW; e += MTHREADS) {
+    const int i = e / MTW, j = e - i * MTW;
+    const int qe = (i + R) * QC + (j + R);
+    const int s = S[qe];
+    const float g = G[qe];
+#pragma unroll
+    for (int k = 0; k < K * K; k++) part[k] += (s == k) ? g : 0.0f;
+  }
+  float *dst = gw_part + (int64_t)blockIdx.x * K * K;
+  for (int k = 0; k < K * K; k++) {
+    red[threadIdx.x] = part[k];
+    __syncthreads();
+    for (int sft = MTHREADS / 2; sft > 0; sft >>= 1) {
+      if (threadIdx.x < sft) red[threadIdx.x] += red[threadIdx.x + sft];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) dst[k] = DIL ? red[0] : -red[0];
+    __syncthreads();
+  }
+}
+
 // gw[k] = sum over blocks b (in order) of part[b][k]: one workgroup per tap, fixed-order tree
 __global__ __launch_bounds__(256) void gw_reduce_kernel(const float *__restrict__ part, int nblocks, int KK,
                                                         float *__restrict__ gw)
@@ -207,8 +281,8 @@ using namespace pm;
   default: MACRO(9); break;                                                                                  \
   }
 
-extern "C" int pm_maxplus_fwd(const float *x, int64_t B, int64_t H, int64_t W, const float *w, int K, int dilate,
-                              float *y, void *stream)
+static int maxplus_fwd(const float *x, int64_t B, int64_t H, int64_t W, const float *w, int K, int dilate, float *y,
+                       uint8_t *selo, void *stream)
 {
   PM_TRY(mp_check(B, H, W, K, x, y));
   if (x == y && B * H * W > 0) PM_FAIL(PM_E_ALIAS, "maxplus: output must be != input");
@@ -218,19 +292,78 @@ extern "C" int pm_maxplus_fwd(const float *x, int64_t B, int64_t H, int64_t W, c
   if (nb > INT32_MAX) PM_FAIL(PM_E_SIZE, "maxplus: batch too large");
   hipStream_t s = as_stream(stream);
   const bool flat = (w == nullptr);
+#define L(KK, SELV)                                                                                          \
+  do {                                                                                                       \
+    if (dilate) {                                                                                            \
+      if (flat) hipLaunchKernelGGL((mp_fwd_kernel<KK, true, true, SELV>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, x, w, y, selo, (int)H, (int)W, tiles_x, per); \
+      else hipLaunchKernelGGL((mp_fwd_kernel<KK, true, false, SELV>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, x, w, y, selo, (int)H, (int)W, tiles_x, per); \
+    } else {                                                                                                 \
+      if (flat) hipLaunchKernelGGL((mp_fwd_kernel<KK, false, true, SELV>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, x, w, y, selo, (int)H, (int)W, tiles_x, per); \
+      else hipLaunchKernelGGL((mp_fwd_kernel<KK, false, false, SELV>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, x, w, y, selo, (int)H, (int)W, tiles_x, per); \
+    }                                                                                                        \
+  } while (0)
+#define LS(KK) L(KK, true)
+#define LN(KK) L(KK, false)
+  if (selo) {
+    PM_MP_DISPATCH(LS)
+  } else {
+    PM_MP_DISPATCH(LN)
+  }
+#undef LN
+#undef LS
+#undef L
+  PM_HIP(hipGetLastError());
+  return PM_OK;
+}
+
+extern "C" int pm_maxplus_fwd(const float *x, int64_t B, int64_t H, int64_t W, const float *w, int K, int dilate,
+                              float *y, void *stream)
+{
+  return maxplus_fwd(x, B, H, W, w, K, dilate, y, nullptr, stream);
+}
+
+extern "C" int pm_maxplus_fwd_sel(const float *x, int64_t B, int64_t H, int64_t W, const float *w, int K, int dilate,
+                                  float *y, uint8_t *sel, void *stream)
+{
+  if (!sel && B * H * W > 0) PM_FAIL(PM_E_ARG, "maxplus: null tap buffer");
+  return maxplus_fwd(x, B, H, W, w, K, dilate, y, sel, stream);
+}
+
+extern "C" int pm_maxplus_bwd_sel(const uint8_t *sel, const float *gy, int64_t B, int64_t H, int64_t W, int K,
+                                  int dilate, int with_w, float *gx, float *gw, void *stream)
+{
+  PM_TRY(mp_check(B, H, W, K, gy, gx));
+  if (!sel && B * H * W > 0) PM_FAIL(PM_E_ARG, "maxplus: null tap buffer");
+  if (gx == gy && B * H * W > 0) PM_FAIL(PM_E_ALIAS, "maxplus: gx must be a distinct buffer");
+  if (with_w && !gw) PM_FAIL(PM_E_ARG, "maxplus: gw buffer required for a structuring function");
+  hipStream_t s = as_stream(stream);
+  const int tiles_x = (int)((W + MTW - 1) / MTW);
+  const int64_t per = (int64_t)tiles_x * ((H + MTH_S - 1) / MTH_S), nb = B * per;
+  if (nb == 0) {
+    if (with_w) return fill_bytes(gw, 0, sizeof(float) * K * K, s);
+    return PM_OK;
+  }
+  if (nb > INT32_MAX) PM_FAIL(PM_E_SIZE, "maxplus: batch too large");
+  Scratch part;
+  if (with_w) PM_TRY(part.alloc(sizeof(float) * (size_t)nb * K * K, s));
+  float *pp = (float *)part.ptr;
 #define L(KK)                                                                                                \
   do {                                                                                                       \
     if (dilate) {                                                                                            \
-      if (flat) hipLaunchKernelGGL((mp_fwd_kernel<KK, true, true>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, x, w, y, (int)H, (int)W, tiles_x, per); \
-      else hipLaunchKernelGGL((mp_fwd_kernel<KK, true, false>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, x, w, y, (int)H, (int)W, tiles_x, per); \
+      if (with_w) hipLaunchKernelGGL((mp_bwd_sel_kernel<KK, true, true>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, sel, gy, gx, pp, (int)H, (int)W, tiles_x, (int)per); \
+      else hipLaunchKernelGGL((mp_bwd_sel_kernel<KK, true, false>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, sel, gy, gx, pp, (int)H, (int)W, tiles_x, (int)per); \
     } else {                                                                                                 \
-      if (flat) hipLaunchKernelGGL((mp_fwd_kernel<KK, false, true>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, x, w, y, (int)H, (int)W, tiles_x, per); \
-      else hipLaunchKernelGGL((mp_fwd_kernel<KK, false, false>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, x, w, y, (int)H, (int)W, tiles_x, per); \
+      if (with_w) hipLaunchKernelGGL((mp_bwd_sel_kernel<KK, false, true>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, sel, gy, gx, pp, (int)H, (int)W, tiles_x, (int)per); \
+      else hipLaunchKernelGGL((mp_bwd_sel_kernel<KK, false, false>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, sel, gy, gx, pp, (int)H, (int)W, tiles_x, (int)per); \
     }                                                                                                        \
   } while (0)
   PM_MP_DISPATCH(L)
 #undef L
   PM_HIP(hipGetLastError());
+  if (with_w) {
+    hipLaunchKernelGGL(gw_reduce_kernel, dim3(K * K), dim3(256), 0, s, pp, (int)nb, K * K, gw);
+    PM_HIP(hipGetLastError());
+  }
   return PM_OK;
 }
 

@@ -125,3 +125,53 @@ def test_bad_arguments():
         ag.dilation2d(x, k=4)
     with pytest.raises(TypeError):
         ag.dilation2d(x.double(), k=3)
+
+
+@pytest.mark.parametrize("k,with_w,dilate", [(3, False, True), (5, True, False), (7, False, False), (3, True, True), (7, True, True), (9, True, False)])
+def test_stored_taps_match_recomputed_taps(k, with_w, dilate):
+    """the C ABI pair that stores the selected taps (pm_maxplus_fwd_sel / _bwd_sel, used by the
+    autograd Function) and the pair that re-selects them from x (pm_maxplus_fwd / _bwd) select
+    the same taps: y and gx agree bit for bit (gx gathers taps in one fixed order in both); gw
+    sums per-tile partials over tiles of different heights (64 rows against 32), so it agrees to
+    fp32 rounding of that order (rtol 1e-5), each path being deterministic on its own"""
+    import ctypes
+    from poutre2_amd import _capi
+    g = torch.Generator().manual_seed(40 + k)
+    x = torch.randn(2, 3, 67, 130, generator=g).to(DEV)
+    x[0, 0, 10:20, 10:20] = 0.5  # plateaus: ties resolved by the raster rule
+    gy = torch.randn(x.shape, generator=g).to(DEV)
+    w = (0.1 * torch.randn(k, k, generator=g)).to(DEV) if with_w else None
+    B, H, W = 6, 67, 130
+    p = (lambda t: ctypes.c_void_p(t.data_ptr() if t is not None else 0))
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    lib = _capi.lib()
+    y1, y2 = torch.empty_like(x), torch.empty_like(x)
+    sel = torch.empty(x.shape, dtype=torch.uint8, device=DEV)
+    _capi.check(lib.pm_maxplus_fwd(p(x), B, H, W, p(w), k, int(dilate), p(y1), s))
+    _capi.check(lib.pm_maxplus_fwd_sel(p(x), B, H, W, p(w), k, int(dilate), p(y2), p(sel), s))
+    gx1, gx2 = torch.empty_like(x), torch.empty_like(x)
+    gw1 = torch.empty(k, k, device=DEV) if with_w else None
+    gw2 = torch.empty(k, k, device=DEV) if with_w else None
+    _capi.check(lib.pm_maxplus_bwd(p(x), p(gy), B, H, W, p(w), k, int(dilate), p(gx1), p(gw1), s))
+    _capi.check(lib.pm_maxplus_bwd_sel(p(sel), p(gy), B, H, W, k, int(dilate), int(with_w), p(gx2), p(gw2), s))
+    torch.cuda.synchronize()
+    assert int(sel.max()) < k * k
+    assert torch.equal(y1, y2) and torch.equal(gx1, gx2)
+    if with_w:
+        torch.testing.assert_close(gw1, gw2, rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.parametrize("k,with_w", [(3, False), (7, True)])
+def test_recorded_and_plain_forward_agree(k, with_w):
+    """a call autograd records stores the selected taps; one it does not (no_grad, or no input
+    requiring grad) runs the max-only kernel: both give the same y bit for bit"""
+    x = _x((2, 45, 77), 7).to(DEV)
+    w = _w(k, 8).to(DEV) if with_w else None
+    plain = ag.dilation2d(x, k=k, w=w)
+    with torch.no_grad():
+        nograd = ag.erosion2d(x.clone().requires_grad_(True), k=k, w=w)
+    xr = x.clone().requires_grad_(True)
+    rec = ag.dilation2d(xr, k=k, w=w)
+    rec_e = ag.erosion2d(xr, k=k, w=w)
+    assert rec.requires_grad and not nograd.requires_grad
+    assert torch.equal(plain, rec.detach()) and torch.equal(nograd, rec_e.detach())
