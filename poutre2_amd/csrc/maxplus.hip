@@ -14,7 +14,9 @@
 // workgroups in a fixed order by a second kernel.
 // Memory per pixel: forward reads x, writes y (8 B); backward reads x and gy, writes gx
 // (12 B), re-selecting the taps.  The _sel forms store the selected tap of every output in the
-// forward (9 B) and read it in the backward instead of x (9 B, no re-selection).
+// forward (9 B) and read it in the backward instead of x (9 B, no re-selection).  Flat windows
+// are separable (row extremum, then column extremum; tap rows, then tap columns in the
+// backward) and run one wave per strip with DPP lane shifts: 2K work per pixel instead of K^2.
 #include <cmath>
 
 #include "maxplus.hpp"
@@ -90,6 +92,116 @@ __global__ __launch_bounds__(MTHREADS) void mp_fwd_kernel(const float *__restric
     const int sel = select_tap<K, DIL, FLAT>(&X[i * RC + j], RC, w, best);
     yi[(int64_t)gy * W + gx] = best;
     if (SEL) selo[(int64_t)img * H * W + (int64_t)gy * W + gx] = (uint8_t)sel;
+  }
+}
+
+// Flat forward and flat backward run one wave per strip of WNR rows and 64 - 2R output columns,
+// lane l holding input column (first output column) - R + l: a row's K taps are lanes l..l+2R,
+// reached by DPP shifts (no LDS, no barrier); the vertical window is a ring of K rows in
+// registers.  Lanes 64 - 2R.. only feed their left neighbours.
+constexpr int WNR = 32, WCH = 8;  // WCH: rows whose loads a wave issues together
+
+// lane i receives lane i + 1's value (lane 63 receives 0; no output lane reads it)
+__device__ __forceinline__ float from_right(float v)
+{
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
+}
+__device__ __forceinline__ int from_right(int v) { return __builtin_amdgcn_mov_dpp(v, 0x130, 0xf, 0xf, true); }
+
+struct WaveStrip {
+  int img, r0, ox, c, cc, lane;
+  bool cin;
+};
+
+// the strip of wave w: image, first row, this lane's output column ox and input column c
+template <int R>
+__device__ __forceinline__ bool wave_strip(int nwaves, int strips_x, int strips_per_img, int W, WaveStrip &ws)
+{
+  const int wave = blockIdx.x * (MTHREADS / 64) + (threadIdx.x >> 6);
+  if (wave >= nwaves) return false;
+  ws.lane = threadIdx.x & 63;
+  ws.img = wave / strips_per_img;
+  const int t = wave - ws.img * strips_per_img;
+  const int sy = t / strips_x, sx = t - sy * strips_x;
+  ws.r0 = sy * WNR;
+  ws.ox = sx * (64 - 2 * R) + ws.lane;
+  ws.c = ws.ox - R;
+  ws.cin = (unsigned)ws.c < (unsigned)W;
+  ws.cc = ws.c < 0 ? 0 : (ws.c >= W ? W - 1 : ws.c);
+  return true;
+}
+
+// Flat forward, separable: each row of the window gives its extremum over the K taps and the
+// first arg dx; the column pass takes the first row dy whose value beats the running best.
+// That row holds the window's extremum and no earlier row does, so dy * K + dx is the first
+// maximiser in raster order (the tap select_tap finds) and y has the same bits.  2K compares
+// per output instead of K^2.
+template <int K, bool DIL, bool SEL>
+__global__ __launch_bounds__(MTHREADS) void mp_fwd_flat_kernel(const float *__restrict__ x, float *__restrict__ y,
+                                                               uint8_t *__restrict__ selo, int H, int W, int strips_x,
+                                                               int strips_per_img, int nwaves)
+{
+  constexpr int R = (K - 1) / 2;
+  WaveStrip ws;
+  if (!wave_strip<R>(nwaves, strips_x, strips_per_img, W, ws)) return;  // whole waves only
+  const int64_t ib = (int64_t)ws.img * H * W;
+  const float *xi = x + ib;
+  const bool out_lane = ws.lane < 64 - 2 * R && ws.ox < W;
+  auto load = [&](int r) {
+    const int rr = r < 0 ? 0 : (r >= H ? H - 1 : r);
+    const float v = xi[(int64_t)rr * W + ws.cc];
+    return (ws.cin && (unsigned)r < (unsigned)H) ? v : pad_value<DIL>();
+  };
+  float m[K];
+  int a[K];
+  auto row_pass = [&](float v, float &mo, int &ao) {
+    float best = pad_value<DIL>();
+    int arg = 0;
+#pragma unroll
+    for (int dx = 0; dx < K; dx++) {
+      if (dx) v = from_right(v);
+      const bool better = DIL ? (v > best) : (v < best);
+      best = better ? v : best;
+      arg = better ? dx : arg;
+    }
+    mo = best;
+    ao = arg;
+  };
+  {
+    float v[K - 1];  // all loads issued before the first use
+#pragma unroll
+    for (int k = 0; k < K - 1; k++) v[k] = load(ws.r0 - R + k);
+#pragma unroll
+    for (int k = 0; k < K - 1; k++) row_pass(v[k], m[k], a[k]);
+  }
+  for (int i0 = 0; i0 < WNR; i0 += WCH) {
+    if (ws.r0 + i0 >= H) break;  // uniform over the wave
+    float v[WCH];  // WCH rows in flight per wave
+#pragma unroll
+    for (int u = 0; u < WCH; u++) v[u] = load(ws.r0 + i0 + u + R);
+#pragma unroll
+    for (int u = 0; u < WCH; u++) {
+      const int r = ws.r0 + i0 + u;
+      if (r >= H) break;
+      row_pass(v[u], m[K - 1], a[K - 1]);
+      float best = pad_value<DIL>();
+      int sel = 0;
+#pragma unroll
+      for (int dy = 0; dy < K; dy++) {
+        const bool better = DIL ? (m[dy] > best) : (m[dy] < best);
+        best = better ? m[dy] : best;
+        if (SEL) sel = better ? dy * K + a[dy] : sel;
+      }
+      if (out_lane) {
+        y[ib + (int64_t)r * W + ws.ox] = best;
+        if (SEL) selo[ib + (int64_t)r * W + ws.ox] = (uint8_t)sel;
+      }
+#pragma unroll
+      for (int k = 0; k < K - 1; k++) {
+        m[k] = m[k + 1];
+        a[k] = a[k + 1];
+      }
+    }
   }
 }
 
@@ -235,6 +347,105 @@ __global__ __launch_bounds__(MTHREADS) void mp_bwd_sel_kernel(const uint8_t *__r
   }
 }
 
+// Backward from stored taps, separable (gx only).  For input p = (r, c + dx - R):
+//   gx[p] = sum_dx sum_dy [sel(q) == (dy, dx)] gy[q],  q = (r - dy + R, c)
+// Stage A sums, for input row r and the output column c of this lane, gy over the q of column
+// c whose tap row lands on r (GR), and keeps their tap column (AD).  When every such q agrees
+// on dx (always so for taps of the flat forward: a q's dx is the first arg of row r at column
+// c), stage B gathers gx[p] = sum_dx [AD(c) == dx] GR(c) over the lanes c = p + R - dx.  A row
+// where some lane mixes tap columns (taps of a structuring function) keeps one sum per tap
+// column instead (K^2 work).  Sums run dy then dx in ascending order in both (deterministic,
+// the same bits where both apply).
+template <int K>
+__global__ __launch_bounds__(MTHREADS) void mp_bwd_flat_kernel(const uint8_t *__restrict__ sel,
+                                                               const float *__restrict__ gy, float *__restrict__ gx,
+                                                               int H, int W, int strips_x, int strips_per_img,
+                                                               int nwaves)
+{
+  constexpr int R = (K - 1) / 2;
+  WaveStrip ws;
+  if (!wave_strip<R>(nwaves, strips_x, strips_per_img, W, ws)) return;  // whole waves only
+  const int64_t ib = (int64_t)ws.img * H * W;
+  const bool out_lane = ws.lane < 64 - 2 * R && ws.ox < W;
+  // ring k <-> output row r - R + k (q rows of the window of input row r); outside: no tap, gy 0
+  int sv[K];
+  float gv[K];
+  auto load = [&](int q, int &so, float &go) {
+    const int qq = q < 0 ? 0 : (q >= H ? H - 1 : q);
+    const int64_t e = ib + (int64_t)qq * W + ws.cc;
+    const int s8 = sel[e];
+    const float g = gy[e];
+    const bool in = ws.cin && (unsigned)q < (unsigned)H;
+    so = in ? s8 : 255;
+    go = in ? g : 0.0f;
+  };
+#pragma unroll
+  for (int k = 0; k < K - 1; k++) load(ws.r0 - R + k, sv[k], gv[k]);
+  for (int i0 = 0; i0 < WNR; i0 += WCH) {
+    if (ws.r0 + i0 >= H) break;  // uniform over the wave
+    int cs[WCH];
+    float cg[WCH];
+#pragma unroll
+    for (int u = 0; u < WCH; u++) load(ws.r0 + i0 + u + R, cs[u], cg[u]);
+#pragma unroll
+    for (int u = 0; u < WCH; u++) {
+      const int r = ws.r0 + i0 + u;
+      if (r >= H) break;
+      sv[K - 1] = cs[u];
+      gv[K - 1] = cg[u];
+      float GR = 0.0f;
+      int ad = -1, bad = 0;
+#pragma unroll
+      for (int dy = 0; dy < K; dy++) {
+        // q = (r + R - dy, c) is ring entry 2R - dy; its tap column when its tap row is dy
+        const unsigned d = (unsigned)(sv[2 * R - dy] - dy * K);
+        const bool hit = d < (unsigned)K;
+        GR += hit ? gv[2 * R - dy] : 0.0f;
+        bad |= (int)(hit & (ad >= 0) & (ad != (int)d));
+        ad = hit ? (int)d : ad;
+      }
+      float acc = 0.0f;
+      if (!__any(bad)) {
+        // stage B: lane l + k holds output column ox + k - R, whose tap column 2R - k lands on ox
+        float gs[2 * R + 1];
+        int as[2 * R + 1];
+        gs[0] = GR;
+        as[0] = ad;
+#pragma unroll
+        for (int k = 1; k <= 2 * R; k++) {
+          gs[k] = from_right(gs[k - 1]);
+          as[k] = from_right(as[k - 1]);
+        }
+#pragma unroll
+        for (int dx = 0; dx < K; dx++) acc += (as[2 * R - dx] == dx) ? gs[2 * R - dx] : 0.0f;
+      } else {
+        float part[K];
+#pragma unroll
+        for (int dx = 0; dx < K; dx++) part[dx] = 0.0f;
+#pragma unroll
+        for (int dy = 0; dy < K; dy++) {
+          const int d = sv[2 * R - dy] - dy * K;
+#pragma unroll
+          for (int dx = 0; dx < K; dx++) part[dx] += (d == dx) ? gv[2 * R - dy] : 0.0f;
+        }
+#pragma unroll
+        for (int dx = 0; dx < K; dx++) {
+          float v = part[dx];
+#pragma unroll
+          for (int k = 0; k < 2 * R - dx; k++) v = from_right(v);
+          acc += v;
+        }
+      }
+      if (out_lane) gx[ib + (int64_t)r * W + ws.ox] = acc;
+#pragma unroll
+      for (int k = 0; k < K - 1; k++) {
+        sv[k] = sv[k + 1];
+        gv[k] = gv[k + 1];
+      }
+    }
+  }
+}
+
 // gw[k] = sum over blocks b (in order) of part[b][k]: one workgroup per tap, fixed-order tree
 __global__ __launch_bounds__(256) void gw_reduce_kernel(const float *__restrict__ part, int nblocks, int KK,
                                                         float *__restrict__ gw)
@@ -269,6 +480,20 @@ static int64_t mp_blocks(int64_t B, int64_t H, int64_t W, int th, int &tiles_x, 
   return B * per;
 }
 
+// strips of the flat kernels: WNR rows x (64 - (K - 1)) output columns, one wave each, four
+// waves per workgroup; returns the workgroup count (0: nothing to do, -1: too many)
+static int64_t mp_strips(int64_t B, int64_t H, int64_t W, int K, int &strips_x, int &per, int &nwaves)
+{
+  const int64_t wout = 64 - (K - 1);
+  const int64_t sx = (W + wout - 1) / wout, sy = (H + WNR - 1) / WNR, n = B * sx * sy;
+  if (n == 0) return 0;
+  if (n > INT32_MAX - 3) return -1;
+  strips_x = (int)sx;
+  per = (int)(sx * sy);
+  nwaves = (int)n;
+  return (n + MTHREADS / 64 - 1) / (MTHREADS / 64);
+}
+
 }  // namespace pm
 
 using namespace pm;
@@ -286,19 +511,21 @@ static int maxplus_fwd(const float *x, int64_t B, int64_t H, int64_t W, const fl
 {
   PM_TRY(mp_check(B, H, W, K, x, y));
   if (x == y && B * H * W > 0) PM_FAIL(PM_E_ALIAS, "maxplus: output must be != input");
-  int tiles_x, per;
+  int tiles_x, per, strips_x = 0, sper = 0, nwaves = 0;
   const int64_t nb = mp_blocks(B, H, W, MTH_F, tiles_x, per);
   if (nb == 0) return PM_OK;
   if (nb > INT32_MAX) PM_FAIL(PM_E_SIZE, "maxplus: batch too large");
   hipStream_t s = as_stream(stream);
   const bool flat = (w == nullptr);
+  const int64_t nbs = mp_strips(B, H, W, K, strips_x, sper, nwaves);
+  if (flat && nbs < 0) PM_FAIL(PM_E_SIZE, "maxplus: batch too large");
 #define L(KK, SELV)                                                                                          \
   do {                                                                                                       \
     if (dilate) {                                                                                            \
-      if (flat) hipLaunchKernelGGL((mp_fwd_kernel<KK, true, true, SELV>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, x, w, y, selo, (int)H, (int)W, tiles_x, per); \
+      if (flat) hipLaunchKernelGGL((mp_fwd_flat_kernel<KK, true, SELV>), dim3((unsigned)nbs), dim3(MTHREADS), 0, s, x, y, selo, (int)H, (int)W, strips_x, sper, nwaves); \
       else hipLaunchKernelGGL((mp_fwd_kernel<KK, true, false, SELV>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, x, w, y, selo, (int)H, (int)W, tiles_x, per); \
     } else {                                                                                                 \
-      if (flat) hipLaunchKernelGGL((mp_fwd_kernel<KK, false, true, SELV>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, x, w, y, selo, (int)H, (int)W, tiles_x, per); \
+      if (flat) hipLaunchKernelGGL((mp_fwd_flat_kernel<KK, false, SELV>), dim3((unsigned)nbs), dim3(MTHREADS), 0, s, x, y, selo, (int)H, (int)W, strips_x, sper, nwaves); \
       else hipLaunchKernelGGL((mp_fwd_kernel<KK, false, false, SELV>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, x, w, y, selo, (int)H, (int)W, tiles_x, per); \
     }                                                                                                        \
   } while (0)
@@ -344,6 +571,9 @@ extern "C" int pm_maxplus_bwd_sel(const uint8_t *sel, const float *gy, int64_t B
     return PM_OK;
   }
   if (nb > INT32_MAX) PM_FAIL(PM_E_SIZE, "maxplus: batch too large");
+  int strips_x = 0, sper = 0, nwaves = 0;
+  const int64_t nbs = mp_strips(B, H, W, K, strips_x, sper, nwaves);
+  if (!with_w && nbs < 0) PM_FAIL(PM_E_SIZE, "maxplus: batch too large");
   Scratch part;
   if (with_w) PM_TRY(part.alloc(sizeof(float) * (size_t)nb * K * K, s));
   float *pp = (float *)part.ptr;
@@ -351,10 +581,10 @@ extern "C" int pm_maxplus_bwd_sel(const uint8_t *sel, const float *gy, int64_t B
   do {                                                                                                       \
     if (dilate) {                                                                                            \
       if (with_w) hipLaunchKernelGGL((mp_bwd_sel_kernel<KK, true, true>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, sel, gy, gx, pp, (int)H, (int)W, tiles_x, (int)per); \
-      else hipLaunchKernelGGL((mp_bwd_sel_kernel<KK, true, false>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, sel, gy, gx, pp, (int)H, (int)W, tiles_x, (int)per); \
+      else hipLaunchKernelGGL((mp_bwd_flat_kernel<KK>), dim3((unsigned)nbs), dim3(MTHREADS), 0, s, sel, gy, gx, (int)H, (int)W, strips_x, sper, nwaves); \
     } else {                                                                                                 \
       if (with_w) hipLaunchKernelGGL((mp_bwd_sel_kernel<KK, false, true>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, sel, gy, gx, pp, (int)H, (int)W, tiles_x, (int)per); \
-      else hipLaunchKernelGGL((mp_bwd_sel_kernel<KK, false, false>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, sel, gy, gx, pp, (int)H, (int)W, tiles_x, (int)per); \
+      else hipLaunchKernelGGL((mp_bwd_flat_kernel<KK>), dim3((unsigned)nbs), dim3(MTHREADS), 0, s, sel, gy, gx, (int)H, (int)W, strips_x, sper, nwaves); \
     }                                                                                                        \
   } while (0)
   PM_MP_DISPATCH(L)

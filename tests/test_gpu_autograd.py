@@ -131,9 +131,11 @@ def test_bad_arguments():
 def test_stored_taps_match_recomputed_taps(k, with_w, dilate):
     """the C ABI pair that stores the selected taps (pm_maxplus_fwd_sel / _bwd_sel, used by the
     autograd Function) and the pair that re-selects them from x (pm_maxplus_fwd / _bwd) select
-    the same taps: y and gx agree bit for bit (gx gathers taps in one fixed order in both); gw
-    sums per-tile partials over tiles of different heights (64 rows against 32), so it agrees to
-    fp32 rounding of that order (rtol 1e-5), each path being deterministic on its own"""
+    the same taps: y agrees bit for bit.  gx with a structuring function gathers taps in one
+    fixed order in both, so it agrees bit for bit too; flat gx from stored taps is summed
+    separably (tap rows, then tap columns), and gw sums per-tile partials over tiles of
+    different heights (64 rows against 32), so those agree to fp32 rounding of the order
+    (rtol 1e-5), each path being deterministic on its own"""
     import ctypes
     from poutre2_amd import _capi
     g = torch.Generator().manual_seed(40 + k)
@@ -156,9 +158,38 @@ def test_stored_taps_match_recomputed_taps(k, with_w, dilate):
     _capi.check(lib.pm_maxplus_bwd_sel(p(sel), p(gy), B, H, W, k, int(dilate), int(with_w), p(gx2), p(gw2), s))
     torch.cuda.synchronize()
     assert int(sel.max()) < k * k
-    assert torch.equal(y1, y2) and torch.equal(gx1, gx2)
+    assert torch.equal(y1, y2)
     if with_w:
+        assert torch.equal(gx1, gx2)
         torch.testing.assert_close(gw1, gw2, rtol=1e-5, atol=1e-5)
+    else:
+        torch.testing.assert_close(gx1, gx2, rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.parametrize("k", [3, 7])
+def test_stored_taps_of_a_structuring_function_without_gw(k):
+    """pm_maxplus_bwd_sel with with_w = 0 sums gx separably when every output column agrees on
+    the tap column (taps of a flat forward); taps of a structuring function break that in
+    most tiles, which then gather directly: gx still equals the re-selecting backward's (to
+    fp32 rounding, rtol 1e-5: a tile where the columns happen to agree sums separably)"""
+    import ctypes
+    from poutre2_amd import _capi
+    x = _x((3, 70, 140), 60 + k).to(DEV)
+    w = _w(k, 61 + k).to(DEV)
+    gy = _x((3, 70, 140), 62 + k).to(DEV)
+    B, H, W = 3, 70, 140
+    p = (lambda t: ctypes.c_void_p(t.data_ptr() if t is not None else 0))
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    lib = _capi.lib()
+    y = torch.empty_like(x)
+    sel = torch.empty(x.shape, dtype=torch.uint8, device=DEV)
+    _capi.check(lib.pm_maxplus_fwd_sel(p(x), B, H, W, p(w), k, 1, p(y), p(sel), s))
+    gx1, gx2 = torch.empty_like(x), torch.empty_like(x)
+    gw1 = torch.empty(k, k, device=DEV)
+    _capi.check(lib.pm_maxplus_bwd(p(x), p(gy), B, H, W, p(w), k, 1, p(gx1), p(gw1), s))
+    _capi.check(lib.pm_maxplus_bwd_sel(p(sel), p(gy), B, H, W, k, 1, 0, p(gx2), p(None), s))
+    torch.cuda.synchronize()
+    torch.testing.assert_close(gx1, gx2, rtol=1e-5, atol=1e-5)
 
 
 @pytest.mark.parametrize("k,with_w", [(3, False), (7, True)])
