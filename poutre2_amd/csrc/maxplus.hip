@@ -24,11 +24,12 @@
 
 namespace pm {
 
-// tiles of MTW columns; MTH_F rows in the forward, MTH_B in the backward, whose four LDS arrays
-// (inputs with a 2R halo, gradients, selected taps) limit residency (measured on [64,32,256,256]
-// 3x3: forward 16 / 32 / 64 rows 0.36 / 0.32 / 0.31 ms, backward best at 16-32 rows, 64
-// rows 1.5-1.6x slower)
-constexpr int MTW = 64, MTH_F = 64, MTH_B = 32, MTH_S = 64, MTHREADS = 256;  // MTH_S: backward from stored taps
+// LDS tiles of the max-plus forward and of the re-selecting backward: MTW columns; MTH_F rows
+// in the forward, MTH_B in the backward, whose four LDS arrays (inputs with a 2R halo,
+// gradients, selected taps) limit residency (measured on [64,32,256,256] 3x3: forward
+// 16 / 32 / 64 rows 0.36 / 0.32 / 0.31 ms, backward best at 16-32 rows, 64 rows 1.5-1.6x
+// slower)
+constexpr int MTW = 64, MTH_F = 64, MTH_B = 32, MTHREADS = 256;
 
 template <bool DIL> __device__ __forceinline__ float pad_value() { return DIL ? -INFINITY : INFINITY; }
 
@@ -278,76 +279,7 @@ __global__ __launch_bounds__(MTHREADS) void mp_bwd_kernel(const float *__restric
   }
 }
 
-// the selected taps of a (rows x cols) region at (y0, x0); outside the image: no tap (255)
-__device__ __forceinline__ void load_sel(uint8_t *dst, int rows, int cols, const uint8_t *__restrict__ src, int H, int W,
-                                         int y0, int x0)
-{
-  const int n = rows * cols;
-  for (int e = threadIdx.x; e < n; e += MTHREADS) {
-    const int i = e / cols, j = e - i * cols;
-    const int gy = y0 + i, gx = x0 + j;
-    dst[e] = ((unsigned)gy < (unsigned)H && (unsigned)gx < (unsigned)W) ? src[(int64_t)gy * W + gx] : (uint8_t)255;
-  }
-}
-
-// Backward from the taps the forward stored (sel): the gather and the structuring-function
-// gradient of mp_bwd_kernel without reloading x or re-selecting taps.  WITH_W: compute gw.
-template <int K, bool DIL, bool WITH_W>
-__global__ __launch_bounds__(MTHREADS) void mp_bwd_sel_kernel(const uint8_t *__restrict__ sel, const float *__restrict__ gy,
-                                                              float *__restrict__ gx, float *__restrict__ gw_part, int H,
-                                                              int W, int tiles_x, int tiles_per_img)
-{
-  constexpr int R = (K - 1) / 2;
-  constexpr int QC = MTW + K - 1, QR = MTH_S + K - 1;  // outputs q whose window reaches the tile
-  __shared__ float G[QR * QC];
-  __shared__ uint8_t S[QR * QC];
-  __shared__ float red[MTHREADS];
-  const int img = blockIdx.x / tiles_per_img, t = blockIdx.x - img * tiles_per_img;
-  const int ty = t / tiles_x, tx = t - ty * tiles_x;
-  const int y0 = ty * MTH_S, x0 = tx * MTW;
-  const int64_t ib = (int64_t)img * H * W;
-  load_region(G, QR, QC, gy + ib, H, W, y0 - R, x0 - R, 0.0f);
-  load_sel(S, QR, QC, sel + ib, H, W, y0 - R, x0 - R);
-  __syncthreads();
-  for (int e = threadIdx.x; e < MTH_S * MTW; e += MTHREADS) {
-    const int i = e / MTW, j = e - i * MTW;
-    const int py = y0 + i, px = x0 + j;
-    float acc = 0.0f;
-#pragma unroll
-    for (int dy = 0; dy < K; dy++)
-#pragma unroll
-      for (int dx = 0; dx < K; dx++) {
-        const int qe = (i + 2 * R - dy) * QC + (j + 2 * R - dx);
-        if (S[qe] == dy * K + dx) acc += G[qe];
-      }
-    if (py < H && px < W) gx[ib + (int64_t)py * W + px] = acc;
-  }
-  if (!WITH_W) return;
-  float part[K * K];
-#pragma unroll
-  for (int k = 0; k < K * K; k++) part[k] = 0.0f;
-  for (int e = threadIdx.x; e < MTH_S * MTW; e += MTHREADS) {
-    const int i = e / MTW, j = e - i * MTW;
-    const int qe = (i + R) * QC + (j + R);
-    const int s = S[qe];
-    const float g = G[qe];
-#pragma unroll
-    for (int k = 0; k < K * K; k++) part[k] += (s == k) ? g : 0.0f;
-  }
-  float *dst = gw_part + (int64_t)blockIdx.x * K * K;
-  for (int k = 0; k < K * K; k++) {
-    red[threadIdx.x] = part[k];
-    __syncthreads();
-    for (int sft = MTHREADS / 2; sft > 0; sft >>= 1) {
-      if (threadIdx.x < sft) red[threadIdx.x] += red[threadIdx.x + sft];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) dst[k] = DIL ? red[0] : -red[0];
-    __syncthreads();
-  }
-}
-
-// Backward from stored taps, separable (gx only).  For input p = (r, c + dx - R):
+// Backward from stored taps, separable.  For input p = (r, c + dx - R):
 //   gx[p] = sum_dx sum_dy [sel(q) == (dy, dx)] gy[q],  q = (r - dy + R, c)
 // Stage A sums, for input row r and the output column c of this lane, gy over the q of column
 // c whose tap row lands on r (GR), and keeps their tap column (AD).  When every such q agrees
@@ -356,11 +288,15 @@ __global__ __launch_bounds__(MTHREADS) void mp_bwd_sel_kernel(const uint8_t *__r
 // where some lane mixes tap columns (taps of a structuring function) keeps one sum per tap
 // column instead (K^2 work).  Sums run dy then dx in ascending order in both (deterministic,
 // the same bits where both apply).
-template <int K>
+// GW: also the structuring-function gradient gw[t] = +-sum_q [sel(q) == t] gy[q].  Lanes
+// R .. R + WOUT - 1 own the q of their column in the strip's rows (every q once over the
+// grid); each keeps K^2 partials, a fixed-order butterfly sums them over the wave, and the
+// wave's K^2 sums go to gw_part[wave] for gw_reduce_kernel.
+template <int K, bool GW, bool DIL>
 __global__ __launch_bounds__(MTHREADS) void mp_bwd_flat_kernel(const uint8_t *__restrict__ sel,
                                                                const float *__restrict__ gy, float *__restrict__ gx,
-                                                               int H, int W, int strips_x, int strips_per_img,
-                                                               int nwaves)
+                                                               float *__restrict__ gw_part, int H, int W,
+                                                               int strips_x, int strips_per_img, int nwaves)
 {
   constexpr int R = (K - 1) / 2;
   WaveStrip ws;
@@ -370,6 +306,12 @@ __global__ __launch_bounds__(MTHREADS) void mp_bwd_flat_kernel(const uint8_t *__
   // ring k <-> output row r - R + k (q rows of the window of input row r); outside: no tap, gy 0
   int sv[K];
   float gv[K];
+  float pw[GW ? K * K : 1];
+  const bool own = ws.lane >= R && ws.lane < R + 64 - 2 * R;
+  if (GW) {
+#pragma unroll
+    for (int t = 0; t < K * K; t++) pw[t] = 0.0f;
+  }
   auto load = [&](int q, int &so, float &go) {
     const int qq = q < 0 ? 0 : (q >= H ? H - 1 : q);
     const int64_t e = ib + (int64_t)qq * W + ws.cc;
@@ -437,11 +379,32 @@ __global__ __launch_bounds__(MTHREADS) void mp_bwd_flat_kernel(const uint8_t *__
         }
       }
       if (out_lane) gx[ib + (int64_t)r * W + ws.ox] = acc;
+      if (GW) {
+        // q = (r, c) is ring entry R; outside the image its tap is 255 and matches nothing
+        const int sq = sv[R];
+        const float g = own ? gv[R] : 0.0f;
+#pragma unroll
+        for (int t = 0; t < K * K; t++) pw[t] += (sq == t) ? g : 0.0f;
+      }
 #pragma unroll
       for (int k = 0; k < K - 1; k++) {
         sv[k] = sv[k + 1];
         gv[k] = gv[k + 1];
       }
+    }
+  }
+  if (GW) {
+    const int wave = blockIdx.x * (MTHREADS / 64) + (threadIdx.x >> 6);
+#pragma unroll
+    for (int t = 0; t < K * K; t++) {
+      float v = pw[t];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+      pw[t] = v;
+    }
+    if (ws.lane == 0) {
+#pragma unroll
+      for (int t = 0; t < K * K; t++) gw_part[(int64_t)wave * K * K + t] = DIL ? pw[t] : -pw[t];
     }
   }
 }
@@ -564,34 +527,27 @@ extern "C" int pm_maxplus_bwd_sel(const uint8_t *sel, const float *gy, int64_t B
   if (gx == gy && B * H * W > 0) PM_FAIL(PM_E_ALIAS, "maxplus: gx must be a distinct buffer");
   if (with_w && !gw) PM_FAIL(PM_E_ARG, "maxplus: gw buffer required for a structuring function");
   hipStream_t s = as_stream(stream);
-  const int tiles_x = (int)((W + MTW - 1) / MTW);
-  const int64_t per = (int64_t)tiles_x * ((H + MTH_S - 1) / MTH_S), nb = B * per;
-  if (nb == 0) {
+  int strips_x = 0, sper = 0, nwaves = 0;
+  const int64_t nbs = mp_strips(B, H, W, K, strips_x, sper, nwaves);
+  if (nbs < 0) PM_FAIL(PM_E_SIZE, "maxplus: batch too large");
+  if (nbs == 0) {
     if (with_w) return fill_bytes(gw, 0, sizeof(float) * K * K, s);
     return PM_OK;
   }
-  if (nb > INT32_MAX) PM_FAIL(PM_E_SIZE, "maxplus: batch too large");
-  int strips_x = 0, sper = 0, nwaves = 0;
-  const int64_t nbs = mp_strips(B, H, W, K, strips_x, sper, nwaves);
-  if (!with_w && nbs < 0) PM_FAIL(PM_E_SIZE, "maxplus: batch too large");
   Scratch part;
-  if (with_w) PM_TRY(part.alloc(sizeof(float) * (size_t)nb * K * K, s));
+  if (with_w) PM_TRY(part.alloc(sizeof(float) * (size_t)nwaves * K * K, s));
   float *pp = (float *)part.ptr;
 #define L(KK)                                                                                                \
   do {                                                                                                       \
-    if (dilate) {                                                                                            \
-      if (with_w) hipLaunchKernelGGL((mp_bwd_sel_kernel<KK, true, true>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, sel, gy, gx, pp, (int)H, (int)W, tiles_x, (int)per); \
-      else hipLaunchKernelGGL((mp_bwd_flat_kernel<KK>), dim3((unsigned)nbs), dim3(MTHREADS), 0, s, sel, gy, gx, (int)H, (int)W, strips_x, sper, nwaves); \
-    } else {                                                                                                 \
-      if (with_w) hipLaunchKernelGGL((mp_bwd_sel_kernel<KK, false, true>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, sel, gy, gx, pp, (int)H, (int)W, tiles_x, (int)per); \
-      else hipLaunchKernelGGL((mp_bwd_flat_kernel<KK>), dim3((unsigned)nbs), dim3(MTHREADS), 0, s, sel, gy, gx, (int)H, (int)W, strips_x, sper, nwaves); \
-    }                                                                                                        \
+    if (!with_w) hipLaunchKernelGGL((mp_bwd_flat_kernel<KK, false, true>), dim3((unsigned)nbs), dim3(MTHREADS), 0, s, sel, gy, gx, pp, (int)H, (int)W, strips_x, sper, nwaves); \
+    else if (dilate) hipLaunchKernelGGL((mp_bwd_flat_kernel<KK, true, true>), dim3((unsigned)nbs), dim3(MTHREADS), 0, s, sel, gy, gx, pp, (int)H, (int)W, strips_x, sper, nwaves); \
+    else hipLaunchKernelGGL((mp_bwd_flat_kernel<KK, true, false>), dim3((unsigned)nbs), dim3(MTHREADS), 0, s, sel, gy, gx, pp, (int)H, (int)W, strips_x, sper, nwaves); \
   } while (0)
   PM_MP_DISPATCH(L)
 #undef L
   PM_HIP(hipGetLastError());
   if (with_w) {
-    hipLaunchKernelGGL(gw_reduce_kernel, dim3(K * K), dim3(256), 0, s, pp, (int)nb, K * K, gw);
+    hipLaunchKernelGGL(gw_reduce_kernel, dim3(K * K), dim3(256), 0, s, pp, nwaves, K * K, gw);
     PM_HIP(hipGetLastError());
   }
   return PM_OK;

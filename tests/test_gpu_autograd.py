@@ -131,11 +131,10 @@ def test_bad_arguments():
 def test_stored_taps_match_recomputed_taps(k, with_w, dilate):
     """the C ABI pair that stores the selected taps (pm_maxplus_fwd_sel / _bwd_sel, used by the
     autograd Function) and the pair that re-selects them from x (pm_maxplus_fwd / _bwd) select
-    the same taps: y agrees bit for bit.  gx with a structuring function gathers taps in one
-    fixed order in both, so it agrees bit for bit too; flat gx from stored taps is summed
-    separably (tap rows, then tap columns), and gw sums per-tile partials over tiles of
-    different heights (64 rows against 32), so those agree to fp32 rounding of the order
-    (rtol 1e-5), each path being deterministic on its own"""
+    the same taps: y agrees bit for bit.  gx from stored taps is summed over tap rows, then
+    tap columns, where the re-selecting backward runs the taps in raster order, and gw sums
+    per-wave partials where the other sums per-tile ones: those agree to fp32 rounding of the
+    order (rtol 1e-5), each path being deterministic on its own"""
     import ctypes
     from poutre2_amd import _capi
     g = torch.Generator().manual_seed(40 + k)
@@ -159,11 +158,9 @@ def test_stored_taps_match_recomputed_taps(k, with_w, dilate):
     torch.cuda.synchronize()
     assert int(sel.max()) < k * k
     assert torch.equal(y1, y2)
+    torch.testing.assert_close(gx1, gx2, rtol=1e-5, atol=1e-5)
     if with_w:
-        assert torch.equal(gx1, gx2)
         torch.testing.assert_close(gw1, gw2, rtol=1e-5, atol=1e-5)
-    else:
-        torch.testing.assert_close(gx1, gx2, rtol=1e-5, atol=1e-5)
 
 
 @pytest.mark.parametrize("k", [3, 7])
@@ -206,3 +203,23 @@ def test_recorded_and_plain_forward_agree(k, with_w):
     rec_e = ag.erosion2d(xr, k=k, w=w)
     assert rec.requires_grad and not nograd.requires_grad
     assert torch.equal(plain, rec.detach()) and torch.equal(nograd, rec_e.detach())
+
+
+@pytest.mark.parametrize("k,with_w,dilate", [(3, False, True), (5, True, True), (7, True, False)])
+def test_backward_is_deterministic(k, with_w, dilate):
+    """gx and gw come out bit for bit the same on every run: every sum runs in a fixed order
+    (no atomics), over tap rows, tap columns, lanes and waves"""
+    f = ag.dilation2d if dilate else ag.erosion2d
+    x = _x((2, 3, 150, 270), 90 + k).to(DEV)
+    gy = _x((2, 3, 150, 270), 91 + k).to(DEV)
+    w0 = _w(k, 92 + k).to(DEV) if with_w else None
+    outs = []
+    for _ in range(3):
+        xr = x.clone().requires_grad_(True)
+        wr = w0.clone().requires_grad_(True) if with_w else None
+        f(xr, k=k, w=wr).backward(gy)
+        outs.append((xr.grad.clone(), wr.grad.clone() if with_w else None))
+    for gx_, gw_ in outs[1:]:
+        assert torch.equal(gx_, outs[0][0])
+        if with_w:
+            assert torch.equal(gw_, outs[0][1])
