@@ -197,7 +197,9 @@ int pm_maxplus_bwd(const float *x, const float *gy, int64_t B, int64_t H, int64_
 /* the same operator with the selected taps kept: the forward also writes, per output, the
  * raster index dy*K + dx of its selected tap (B*H*W bytes); the backward reads those instead of
  * x and w (with_w != 0: also write gw).  For training, where the forward's input need not be
- * kept: 1 byte per pixel instead of 4, and no tap re-selection in the backward. */
+ * kept: 1 byte per pixel instead of 4, and no tap re-selection in the backward.  gx is summed
+ * over tap rows, then tap columns (gw: per-wave partials, then over waves), in a fixed order:
+ * deterministic, and equal to pm_maxplus_bwd's up to float rounding of the order. */
 int pm_maxplus_fwd_sel(const float *x, int64_t B, int64_t H, int64_t W, const float *w, int K, int dilate, float *y,
                        uint8_t *sel, void *stream);
 int pm_maxplus_bwd_sel(const uint8_t *sel, const float *gy, int64_t B, int64_t H, int64_t W, int K, int dilate,

@@ -223,3 +223,28 @@ def test_backward_is_deterministic(k, with_w, dilate):
         assert torch.equal(gx_, outs[0][0])
         if with_w:
             assert torch.equal(gw_, outs[0][1])
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (2, 3), (1, 70), (70, 1), (5, 130), (33, 2), (67, 200)])
+@pytest.mark.parametrize("k,nonflat,dilate", [(3, False, True), (7, False, False), (9, False, True), (5, True, False)])
+def test_backward_small_and_ragged_with_plateaus(shape, k, nonflat, dilate):
+    """images smaller than the window, one row or one column, widths across several strips of
+    64 - 2R columns and heights across 32-row strips; integer values in 0..3 make plateaus, so
+    the first-maximiser rule decides most routes.  Against torch CPU autograd (rtol 1e-5)."""
+    g0 = np.random.default_rng(hash((shape, k)) % 2**32)
+    x = torch.from_numpy(g0.integers(0, 4, (2,) + shape).astype(np.float32))
+    gy = torch.from_numpy(g0.standard_normal((2,) + shape).astype(np.float32))
+    w = _w(k, 9) if nonflat else None
+    xc = x.clone().requires_grad_(True)
+    wc = w.clone().requires_grad_(True) if w is not None else None
+    yc = ag.reference_cpu(xc, k=k, w=wc, dilate=dilate)
+    (yc * gy).sum().backward()
+    xd = x.to(DEV).requires_grad_(True)
+    wd = w.to(DEV).requires_grad_(True) if w is not None else None
+    f = ag.dilation2d if dilate else ag.erosion2d
+    yd = f(xd, k=k, w=wd)
+    (yd * gy.to(DEV)).sum().backward()
+    assert torch.equal(yd.detach().cpu(), yc.detach())
+    torch.testing.assert_close(xd.grad.cpu(), xc.grad, rtol=1e-5, atol=1e-5)
+    if w is not None:
+        torch.testing.assert_close(wd.grad.cpu(), wc.grad, rtol=1e-5, atol=1e-5)
