@@ -100,7 +100,11 @@ __global__ __launch_bounds__(MTHREADS) void mp_fwd_kernel(const float *__restric
 // lane l holding input column (first output column) - R + l: a row's K taps are lanes l..l+2R,
 // reached by DPP shifts (no LDS, no barrier); the vertical window is a ring of K rows in
 // registers.  Lanes 64 - 2R.. only feed their left neighbours.
-constexpr int WNR = 32, WCH = 8;  // WCH: rows whose loads a wave issues together
+// WNR: strip rows (the max-plus backward with gw, K^2 work per row, takes 2 * WNR: fewer
+// halo rows per output row; measured on [64,32,256,256] 5x5: 1.18 -> 1.11 ms forward +
+// backward, while 64-row strips slowed the flat 3x3 backward, 0.65 -> 0.71 ms).  WCH: rows
+// whose loads a wave issues together (16 gained nothing over 8).
+constexpr int WNR = 32, WCH = 8;
 
 // lane i receives lane i + 1's value (lane 63 receives 0; no output lane reads it)
 __device__ __forceinline__ float from_right(float v)
@@ -115,7 +119,7 @@ struct WaveStrip {
 };
 
 // the strip of wave w: image, first row, this lane's output column ox and input column c
-template <int R>
+template <int R, int NR>
 __device__ __forceinline__ bool wave_strip(int nwaves, int strips_x, int strips_per_img, int W, WaveStrip &ws)
 {
   const int wave = blockIdx.x * (MTHREADS / 64) + (threadIdx.x >> 6);
@@ -124,7 +128,7 @@ __device__ __forceinline__ bool wave_strip(int nwaves, int strips_x, int strips_
   ws.img = wave / strips_per_img;
   const int t = wave - ws.img * strips_per_img;
   const int sy = t / strips_x, sx = t - sy * strips_x;
-  ws.r0 = sy * WNR;
+  ws.r0 = sy * NR;
   ws.ox = sx * (64 - 2 * R) + ws.lane;
   ws.c = ws.ox - R;
   ws.cin = (unsigned)ws.c < (unsigned)W;
@@ -144,7 +148,7 @@ __global__ __launch_bounds__(MTHREADS) void mp_fwd_flat_kernel(const float *__re
 {
   constexpr int R = (K - 1) / 2;
   WaveStrip ws;
-  if (!wave_strip<R>(nwaves, strips_x, strips_per_img, W, ws)) return;  // whole waves only
+  if (!wave_strip<R, WNR>(nwaves, strips_x, strips_per_img, W, ws)) return;  // whole waves only
   const int64_t ib = (int64_t)ws.img * H * W;
   const float *xi = x + ib;
   const bool out_lane = ws.lane < 64 - 2 * R && ws.ox < W;
@@ -298,9 +302,9 @@ __global__ __launch_bounds__(MTHREADS) void mp_bwd_flat_kernel(const uint8_t *__
                                                                float *__restrict__ gw_part, int H, int W,
                                                                int strips_x, int strips_per_img, int nwaves)
 {
-  constexpr int R = (K - 1) / 2;
+  constexpr int R = (K - 1) / 2, NR = GW ? 2 * WNR : WNR;
   WaveStrip ws;
-  if (!wave_strip<R>(nwaves, strips_x, strips_per_img, W, ws)) return;  // whole waves only
+  if (!wave_strip<R, NR>(nwaves, strips_x, strips_per_img, W, ws)) return;  // whole waves only
   const int64_t ib = (int64_t)ws.img * H * W;
   const bool out_lane = ws.lane < 64 - 2 * R && ws.ox < W;
   // ring k <-> output row r - R + k (q rows of the window of input row r); outside: no tap, gy 0
@@ -323,7 +327,7 @@ __global__ __launch_bounds__(MTHREADS) void mp_bwd_flat_kernel(const uint8_t *__
   };
 #pragma unroll
   for (int k = 0; k < K - 1; k++) load(ws.r0 - R + k, sv[k], gv[k]);
-  for (int i0 = 0; i0 < WNR; i0 += WCH) {
+  for (int i0 = 0; i0 < NR; i0 += WCH) {
     if (ws.r0 + i0 >= H) break;  // uniform over the wave
     int cs[WCH];
     float cg[WCH];
@@ -443,12 +447,12 @@ static int64_t mp_blocks(int64_t B, int64_t H, int64_t W, int th, int &tiles_x, 
   return B * per;
 }
 
-// strips of the flat kernels: WNR rows x (64 - (K - 1)) output columns, one wave each, four
+// strips of the wave kernels: nr rows x (64 - (K - 1)) output columns, one wave each, four
 // waves per workgroup; returns the workgroup count (0: nothing to do, -1: too many)
-static int64_t mp_strips(int64_t B, int64_t H, int64_t W, int K, int &strips_x, int &per, int &nwaves)
+static int64_t mp_strips(int64_t B, int64_t H, int64_t W, int K, int nr, int &strips_x, int &per, int &nwaves)
 {
   const int64_t wout = 64 - (K - 1);
-  const int64_t sx = (W + wout - 1) / wout, sy = (H + WNR - 1) / WNR, n = B * sx * sy;
+  const int64_t sx = (W + wout - 1) / wout, sy = (H + nr - 1) / nr, n = B * sx * sy;
   if (n == 0) return 0;
   if (n > INT32_MAX - 3) return -1;
   strips_x = (int)sx;
@@ -480,7 +484,7 @@ static int maxplus_fwd(const float *x, int64_t B, int64_t H, int64_t W, const fl
   if (nb > INT32_MAX) PM_FAIL(PM_E_SIZE, "maxplus: batch too large");
   hipStream_t s = as_stream(stream);
   const bool flat = (w == nullptr);
-  const int64_t nbs = mp_strips(B, H, W, K, strips_x, sper, nwaves);
+  const int64_t nbs = mp_strips(B, H, W, K, WNR, strips_x, sper, nwaves);
   if (flat && nbs < 0) PM_FAIL(PM_E_SIZE, "maxplus: batch too large");
 #define L(KK, SELV)                                                                                          \
   do {                                                                                                       \
@@ -528,7 +532,7 @@ extern "C" int pm_maxplus_bwd_sel(const uint8_t *sel, const float *gy, int64_t B
   if (with_w && !gw) PM_FAIL(PM_E_ARG, "maxplus: gw buffer required for a structuring function");
   hipStream_t s = as_stream(stream);
   int strips_x = 0, sper = 0, nwaves = 0;
-  const int64_t nbs = mp_strips(B, H, W, K, strips_x, sper, nwaves);
+  const int64_t nbs = mp_strips(B, H, W, K, with_w ? 2 * WNR : WNR, strips_x, sper, nwaves);
   if (nbs < 0) PM_FAIL(PM_E_SIZE, "maxplus: batch too large");
   if (nbs == 0) {
     if (with_w) return fill_bytes(gw, 0, sizeof(float) * K * K, s);
