@@ -7,16 +7,17 @@
 // t = (dy, dx) in [-R, R]^2, R = (K-1)/2; taps falling outside the image are skipped (the
 // reference's neutral padding).  The selected tap of q is the FIRST maximiser (minimiser) in
 // raster order of t (dy outer, dx inner), i.e. torch.max's first-occurrence rule over an
-// F.unfold window.  Backward (gather form, deterministic):
-//   gx[p] = sum over taps t in raster order of [sel(p - t) == t] * gy[p - t]
+// F.unfold window.  Backward (gather form, deterministic: no atomics, every sum in a fixed order):
+//   gx[p] = sum over taps t of [sel(p - t) == t] * gy[p - t]
 //   gw[t] = +-sum over q of [sel(q) == t] * gy[q]      (+ for dilation, - for erosion)
-// gw is reduced per workgroup in a fixed order into a partials buffer, then summed over
-// workgroups in a fixed order by a second kernel.
-// Memory per pixel: forward reads x, writes y (8 B); backward reads x and gy, writes gx
-// (12 B), re-selecting the taps.  The _sel forms store the selected tap of every output in the
-// forward (9 B) and read it in the backward instead of x (9 B, no re-selection).  Flat windows
-// are separable (row extremum, then column extremum; tap rows, then tap columns in the
-// backward) and run one wave per strip with DPP lane shifts: 2K work per pixel instead of K^2.
+// gw is reduced per workgroup (re-selecting backward) or per wave (stored-tap backward) into a
+// partials buffer, then summed over those in a fixed order by a second kernel.
+// Memory per pixel: forward reads x, writes y (8 B); the re-selecting backward (pm_maxplus_bwd,
+// LDS tiles, taps in raster order) reads x and gy and writes gx (12 B).  The _sel forms store
+// the selected tap of every output in the forward (9 B) and read it in the backward instead
+// of x (9 B, no re-selection); that backward sums gx over tap rows, then tap columns.  Flat
+// windows are separable (row extremum, then column extremum) and, like the stored-tap
+// backward, run one wave per strip with DPP lane shifts: 2K work per pixel instead of K^2.
 #include <cmath>
 
 #include "maxplus.hpp"
