@@ -19,8 +19,9 @@
  *   or_static                  KATs ero_dil_static_se_t.cpp + ero_dil_runtime_se.cpp + or_nl
  *   or_static_iter             composition identity vs or_line (square) and or_nl (diamond)
  *   or_nl                      KATs ero_dil_runtime_se.cpp
- *   or_compound                composition of pinned or_static calls; no KAT (parity unpinned
- *                              beyond the composition itself)
+ *   or_compound                one or_nl dilation by the explicitly enumerated Minkowski sum
+ *                              diamond(n-m) + square(m) (3D: octahedron + cube), m from a
+ *                              hand-written table of the round-half-up rule, n = 0..12; errors
  *   or_reconstruct             KATs test/geodesy/mreconstruct.cpp + or_reconstruct_bf
  *   or_reconstruct_bf          KATs + invariants (marker^mask <= R <= mask, idempotence)
  *   or_low/high_leveling       KATs test/geodesy/leveling.cpp + closed forms via or_reconstruct

@@ -211,6 +211,67 @@ def test_static_iter_negative_rejected():
         oracle.static_iter(img, "SESquare2D", -1)
 
 
+# Number of square dilations m in the compound SE of size n: the rule of
+# ero_dil_compound_static_se_t.hpp:47-50 (m = round-half-up(n / (1 + sqrt 2))), written out
+# by hand for n = 2..12 (n / 2.41421356...: 0.83 1.24 1.66 2.07 2.49 2.90 3.31 3.73 4.14 4.56 4.97).
+_COMPOUND_SQUARES = {2: 1, 3: 1, 4: 2, 5: 2, 6: 2, 7: 3, 8: 3, 9: 4, 10: 4, 11: 5, 12: 5}
+
+
+def _minkowski(a, b):
+    return sorted({tuple(x + y for x, y in zip(p, q)) for p in a for q in b})
+
+
+def _ball(rank, r, norm):
+    import itertools
+    pts = itertools.product(range(-r, r + 1), repeat=rank)
+    if norm == 1:
+        return [p for p in pts if sum(abs(c) for c in p) <= r]
+    return list(pts)
+
+
+@pytest.mark.parametrize("n", range(0, 13))
+def test_compound_octagon_is_minkowski_sum(n):
+    """The octagon of size n (ero_dil_compound_static_se_t.hpp:33-65) is the cross applied
+    n - m times then the square m times; on a box domain with out-of-image taps ignored,
+    chained dilations by these L1 / L-inf balls equal ONE dilation by their Minkowski sum
+    diamond(n - m) + square(m), taken here over the explicitly enumerated offsets by the
+    KAT-pinned neighbour-list brute force.  n = 1 is the cross itself (:42-45), n = 0 a copy."""
+    img = RNG.integers(0, 256, size=(23, 29)).astype(np.uint8)
+    if n == 0:
+        offs = [(0, 0)]
+    elif n == 1:
+        offs = _ball(2, 1, 1)
+    else:
+        m = _COMPOUND_SQUARES[n]
+        offs = _minkowski(_ball(2, n - m, 1), _ball(2, m, 0))
+    for d in (True, False):
+        np.testing.assert_array_equal(oracle.compound(img, 1, n, d), oracle.nl(img, offs, d))
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 5])
+def test_compound_rhombicuboctahedron_is_minkowski_sum(n):
+    """3D analogue (:67-99): SECross3D n - m times, SESquare3D m times = one dilation by
+    octahedron(n - m) + cube(m)."""
+    img = RNG.integers(0, 256, size=(9, 11, 10)).astype(np.int32)
+    m = _COMPOUND_SQUARES[n]
+    offs = _minkowski(_ball(3, n - m, 1), _ball(3, m, 0))
+    for d in (True, False):
+        np.testing.assert_array_equal(oracle.compound(img, 2, n, d), oracle.nl(img, offs, d))
+
+
+def test_compound_errors():
+    """size 1 on a volume applies SECross2D to a 3D image, which throws (:77-79); each compound
+    SE needs its own rank (ero_dil_compound_static.cpp:68-70,134-136); negative size rejected."""
+    with pytest.raises(oracle.OracleError):
+        oracle.compound(np.zeros((3, 4, 5), np.uint8), 2, 1)
+    with pytest.raises(oracle.OracleError):
+        oracle.compound(np.zeros((3, 4, 5), np.uint8), 1, 2)
+    with pytest.raises(oracle.OracleError):
+        oracle.compound(np.zeros((4, 5), np.uint8), 2, 2)
+    with pytest.raises(oracle.OracleError):
+        oracle.compound(np.zeros((4, 5), np.uint8), 1, -1)
+
+
 def test_static_3d_vs_window():
     img = RNG.integers(0, 256, size=(5, 6, 7)).astype(np.uint8)
     out = oracle.static(img, "SESquare3D", True)
