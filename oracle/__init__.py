@@ -117,6 +117,8 @@ def line_y(img, r, dilate=True):
 
 
 def transpose(img):
+    """t_transpose (transpose_op_t.hpp:45-70) as the reference indexes it: a true transpose of
+    square images only (the reference rejects other shapes); not a general transpose."""
     img, t = _t(img)
     out = np.empty((img.shape[1], img.shape[0]), dtype=img.dtype)
     _check(lib().or_transpose(t, _p(img), _p(out), _shape(img)), "transpose")

@@ -171,6 +171,7 @@ def _desc(t):
 def _descs(ts):
     """pm_image array for a list of device tensors, built as one numpy record array (the
     per-image Python work of a 512-image batch would otherwise rival its kernel time)."""
+    ts = [_img(t) for t in ts]
     n = len(ts)
     rec = np.zeros(n, dtype=_PM_IMAGE_NP)
     data, ptype, rank, shape = [], [], [], []
@@ -646,8 +647,9 @@ def batch_dilate_open(images, radii, dil_outs=None, open_outs=None):
     n = len(images)
     if len(radii) != n:
         raise PoutreError(-6, "batch: one radius per image")
-    dil_outs = [_out(im, None) for im in images] if dil_outs is None else dil_outs
-    open_outs = [_out(im, None) for im in images] if open_outs is None else open_outs
+    images = [_img(t) for t in images]
+    dil_outs = [_out(im, None) for im in images] if dil_outs is None else [_img(t) for t in dil_outs]
+    open_outs = [_out(im, None) for im in images] if open_outs is None else [_img(t) for t in open_outs]
     if n == 0:
         return dil_outs, open_outs
     di, dd, do = _descs(images), _descs(dil_outs), _descs(open_outs)
@@ -711,7 +713,8 @@ def batch_reconstruction_packed(rec_type, marker_flat, mask_flat, shapes, se, ou
 def batch_reconstruction(rec_type, markers, masks, se, outs=None):
     """n uint8 2D reconstructions (geo::Reconstruction semantics) sharing one sequence of global rounds."""
     n = len(markers)
-    outs = [_out(m, None) for m in masks] if outs is None else outs
+    markers, masks = [_img(t) for t in markers], [_img(t) for t in masks]
+    outs = [_out(m, None) for m in masks] if outs is None else [_img(t) for t in outs]
     if n == 0:
         return outs
     dm, dk, do = _descs(markers), _descs(masks), _descs(outs)
@@ -724,15 +727,19 @@ class BatchPlan:
     """batch_dilate_open captured once into a HIP graph (pm_batch_plan_create): its scratch and
     descriptor table are allocated at creation, and run() is one graph launch on the current
     stream with no allocation and no per-image host work.  The plan keeps references to its
-    tensors; their contents may change between runs, their storage may not."""
+    tensors; their contents may change between runs, their storage may not.  A plan owns one
+    scratch area (eroded tiles, queue counters): runs of the same plan must be serialised
+    (one stream, or an event between streams); runs that overlap race on that area."""
 
     def __init__(self, images, radii, dil_outs=None, open_outs=None):
         n = len(images)
         if len(radii) != n:
             raise PoutreError(-6, "batch: one radius per image")
-        self.dil_outs = [_out(im, None) for im in images] if dil_outs is None else dil_outs
-        self.open_outs = [_out(im, None) for im in images] if open_outs is None else open_outs
-        self._keep = (list(images), self.dil_outs, self.open_outs)
+        images = [_img(t) for t in images]
+        self.dil_outs = [_out(im, None) for im in images] if dil_outs is None else [_img(t) for t in dil_outs]
+        self.open_outs = [_out(im, None) for im in images] if open_outs is None else [_img(t) for t in open_outs]
+        # the tensors (DLPack producers converted to zero-copy views) stay alive with the plan
+        self._keep = (images, self.dil_outs, self.open_outs)
         recs = (_descs(images), _descs(self.dil_outs), _descs(self.open_outs))
         self._create(n, recs, radii, images[0] if n else None)
 

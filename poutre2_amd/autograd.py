@@ -74,6 +74,14 @@ class _MorphFn(torch.autograd.Function):
         return gx, (gw if (ctx.with_w and ctx.needs_input_grad[1]) else None), None, None, None
 
 
+def _check(x, w):
+    """the kernels read x and w through device pointers: both must be tensors on one device"""
+    if not isinstance(x, torch.Tensor) or x.dtype != torch.float32 or not x.is_cuda:
+        raise TypeError("x must be a float32 device tensor")
+    if w is not None and (not isinstance(w, torch.Tensor) or not w.is_cuda or w.device != x.device):
+        raise TypeError("w must be a tensor on the same device as x")
+
+
 def _recorded(x, w):
     """True when autograd will record the call, so that a backward can follow"""
     return torch.is_grad_enabled() and (x.requires_grad or (w is not None and w.requires_grad))
@@ -81,29 +89,12 @@ def _recorded(x, w):
 
 def dilation2d(x, k=3, w=None):
     """Flat (k x k square) or max-plus (structuring function w) dilation, differentiable."""
+    _check(x, w)
     return _MorphFn.apply(x, w, k, True, _recorded(x, w))
 
 
 def erosion2d(x, k=3, w=None):
     """Flat (k x k square) or max-plus (structuring function w) erosion, differentiable."""
+    _check(x, w)
     return _MorphFn.apply(x, w, k, False, _recorded(x, w))
 
-
-def reference_cpu(x, k=3, w=None, dilate=True):
-    """torch CPU reference of the same operator (F.unfold + max/min, first occurrence), used by
-    the tests as the autograd reference: out-of-image taps are -inf (+inf) so they are never
-    selected."""
-    import torch.nn.functional as F
-    H, W = x.shape[-2], x.shape[-1]
-    if w is not None:
-        k = w.shape[-1]
-    r = (k - 1) // 2
-    pad = float("-inf") if dilate else float("inf")
-    xb = x.reshape(-1, 1, H, W)
-    xp = F.pad(xb, (r, r, r, r), value=pad)
-    u = F.unfold(xp, k)  # (B, k*k, H*W), taps in raster order
-    if w is not None:
-        wv = w.reshape(1, k * k, 1)
-        u = u + wv if dilate else u - wv
-    y = u.max(dim=1).values if dilate else u.min(dim=1).values
-    return y.reshape(x.shape)

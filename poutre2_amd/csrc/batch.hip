@@ -208,7 +208,13 @@ extern "C" int pm_batch_plan_create(int n, const pm_image *ins, const int *radii
   BatchLayout L;
   PM_TRY(batch_layout(n, ins, radii, dil_outs, open_outs, L));
   pm_batch_plan *p = new pm_batch_plan();
-  PM_HIP(hipGetDevice(&p->device));
+  {
+    const hipError_t ge = hipGetDevice(&p->device);
+    if (ge != hipSuccess) {
+      plan_free(p);
+      PM_FAIL(PM_E_HIP, "batch plan: hipGetDevice failed: %s", hipGetErrorString(ge));
+    }
+  }
   const size_t bytes = batch_scratch_bytes(L);
   hipError_t e = hipMalloc(&p->scratch, bytes ? bytes : 256);
   if (e != hipSuccess) {

@@ -13,6 +13,7 @@ torch = pytest.importorskip("torch")
 pm = pytest.importorskip("poutre2_amd")
 from poutre2_amd import autograd as ag  # noqa: E402
 from poutre2_amd import synth  # noqa: E402
+from tests import autograd_ref  # noqa: E402
 
 DEV = "cuda:0"
 
@@ -35,7 +36,7 @@ CASES = [(3, False), (7, False), (5, True), (3, True), (9, False)]
 def test_forward_bit_exact(k, nonflat, dilate, shape):
     x = _x(shape, 7)
     w = _w(k, 3) if nonflat else None
-    ref = ag.reference_cpu(x, k=k, w=w, dilate=dilate)
+    ref = autograd_ref.reference_cpu(x, k=k, w=w, dilate=dilate)
     f = ag.dilation2d if dilate else ag.erosion2d
     got = f(x.to(DEV), k=k, w=None if w is None else w.to(DEV))
     torch.cuda.synchronize()
@@ -51,7 +52,7 @@ def test_backward_vs_cpu_autograd(k, nonflat, dilate):
     g = _x(shape, 12)
     xc = x.clone().requires_grad_(True)
     wc = w.clone().requires_grad_(True) if w is not None else None
-    (ag.reference_cpu(xc, k=k, w=wc, dilate=dilate) * g).sum().backward()
+    (autograd_ref.reference_cpu(xc, k=k, w=wc, dilate=dilate) * g).sum().backward()
     xd = x.to(DEV).requires_grad_(True)
     wd = w.to(DEV).requires_grad_(True) if w is not None else None
     f = ag.dilation2d if dilate else ag.erosion2d
@@ -125,6 +126,11 @@ def test_bad_arguments():
         ag.dilation2d(x, k=4)
     with pytest.raises(TypeError):
         ag.dilation2d(x.double(), k=3)
+    # a structuring function in host memory is rejected before any launch
+    with pytest.raises(TypeError):
+        ag.dilation2d(x, w=torch.zeros((3, 3)))
+    with pytest.raises(TypeError):
+        ag.erosion2d(x, w=np.zeros((3, 3), np.float32))
 
 
 @pytest.mark.parametrize("k,with_w,dilate", [(3, False, True), (5, True, False), (7, False, False), (3, True, True), (7, True, True), (9, True, False)])
@@ -237,7 +243,7 @@ def test_backward_small_and_ragged_with_plateaus(shape, k, nonflat, dilate):
     w = _w(k, 9) if nonflat else None
     xc = x.clone().requires_grad_(True)
     wc = w.clone().requires_grad_(True) if w is not None else None
-    yc = ag.reference_cpu(xc, k=k, w=wc, dilate=dilate)
+    yc = autograd_ref.reference_cpu(xc, k=k, w=wc, dilate=dilate)
     (yc * gy).sum().backward()
     xd = x.to(DEV).requires_grad_(True)
     wd = w.to(DEV).requires_grad_(True) if w is not None else None
