@@ -127,6 +127,15 @@ int pm_gradient(const pm_image *in, int se, int iter, pm_image *out, void *strea
  * src/geodesy/mreconstruct.cpp:48-132 (mreconstruct_t.hpp:75-512). */
 int pm_reconstruct(int rec_type, const pm_image *marker, const pm_image *mask, int se, pm_image *out,
                    void *stream);
+/* pm_reconstruct for uint8 1D/2D images and 3x3-class SEs (square, cross, segments) that
+ * allocates nothing.  No counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * scratch is a 1 MiB array per GPU held until the process ends, so images are limited to
+ * 196608 tiles of 62 x 62 pixels (about 755 Mpixel); larger ones return PM_E_ARG, other pixel
+ * types or SEs PM_E_UNSUPPORTED.  Calls on one GPU share that array: calls that overlap (on
+ * two streams, or from two threads) race and must be serialised by the caller. */
+int pm_reconstruct_noalloc(int rec_type, const pm_image *marker, const pm_image *mask, int se, pm_image *out,
+                           void *stream);
 /* poutre::geo::low_leveling / high_leveling / leveling, include/poutre/geodesy/leveling.hpp:37-52 ->
  * src/geodesy/leveling.cpp:67-237 */
 int pm_low_leveling(const pm_image *ref, const pm_image *marker, int se, pm_image *out, void *stream);

@@ -60,6 +60,7 @@ SIGNATURES = {
     "pm_closing": [_P, _I, _I, _P, _VP],
     "pm_gradient": [_P, _I, _I, _P, _VP],
     "pm_reconstruct": [_I, _P, _P, _I, _P, _VP],
+    "pm_reconstruct_noalloc": [_I, _P, _P, _I, _P, _VP],
     "pm_low_leveling": [_P, _P, _I, _P, _VP],
     "pm_high_leveling": [_P, _P, _I, _P, _VP],
     "pm_leveling": [_P, _P, _I, _P, _VP],

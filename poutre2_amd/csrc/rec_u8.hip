@@ -22,6 +22,7 @@
 // in moved[R].  Rounds are enqueued in batches without host synchronisation; round R exits
 // at once if moved[R-1] == 0.  Round 0 reads marker ^ mask (marker v mask for erosion)
 // instead of the current image, fusing the reference's select_marker step.
+#include <mutex>
 #include <vector>
 
 #include "erodil.hpp"
@@ -263,20 +264,22 @@ bool rec_u8_applicable(int se, Dims d)
           se == PM_SE_SEGMENT_Y_2D);
 }
 
-// Rounds driver shared by the single-image and batched entry points.
+// ints of device scratch a reconstruction over nt tiles uses: one activity tag per tile, one
+// move counter per round
+static size_t rec_scratch_ints(int64_t nt) { return (size_t)nt + REC_MAX_ROUNDS; }
+
+// Rounds driver shared by the single-image, batched and scratch-free entry points.  `sc` is
+// device scratch of rec_scratch_ints(nt) ints (cleared here by a fill kernel), `hmoved` host
+// memory for 64 counters (pinned, or any host memory when the caller cannot allocate).
 template <bool DIL>
-static int rec_rounds(const RecImage &one, const RecImage *many, int nimg, int nt, int se, hipStream_t s)
+static int rec_rounds(const RecImage &one, const RecImage *many, int nimg, int nt, int se, int *sc, int *hmoved,
+                      hipStream_t s)
 {
-  Scratch sc;
-  PM_TRY(sc.alloc(sizeof(int) * ((size_t)nt + REC_MAX_ROUNDS), s));
-  int *act = (int *)sc.ptr;
+  int *act = sc;
   int *moved = act + nt;
   const int ninit = nt > REC_MAX_ROUNDS ? nt : REC_MAX_ROUNDS;
   hipLaunchKernelGGL(rec_init_kernel, dim3((ninit + 255) / 256), dim3(256), 0, s, act, nt, moved, REC_MAX_ROUNDS);
   PM_HIP(hipGetLastError());
-  // pinned host buffer for the convergence counters, allocated once per host thread
-  static thread_local int *hmoved = nullptr;
-  if (!hmoved) PM_HIP(hipHostMalloc((void **)&hmoved, sizeof(int) * 64, hipHostMallocDefault));
   int nbk = NBK_SQUARE;
   if (se == PM_SE_CROSS_2D) nbk = NBK_CROSS;
   else if (se == PM_SE_SEGMENT_X_1D || se == PM_SE_SEGMENT_X_2D) nbk = NBK_SEGX;
@@ -327,12 +330,57 @@ static int rec_describe(const uint8_t *marker, const uint8_t *mask, uint8_t *out
   return PM_OK;
 }
 
+// pinned host buffer for the convergence counters, allocated once per host thread
+static int pinned_counters(int **h)
+{
+  static thread_local int *hmoved = nullptr;
+  if (!hmoved) PM_HIP(hipHostMalloc((void **)&hmoved, sizeof(int) * 64, hipHostMallocDefault));
+  *h = hmoved;
+  return PM_OK;
+}
+
+template <bool DIL>
+static int rec_rounds_alloc(const RecImage &one, const RecImage *many, int nimg, int nt, int se, hipStream_t s)
+{
+  Scratch sc;
+  PM_TRY(sc.alloc(sizeof(int) * rec_scratch_ints(nt), s));
+  int *h = nullptr;
+  PM_TRY(pinned_counters(&h));
+  return rec_rounds<DIL>(one, many, nimg, nt, se, (int *)sc.ptr, h, s);
+}
+
 template <bool DIL>
 int rec_u8(const uint8_t *marker, const uint8_t *mask, uint8_t *out, Dims d, int se, hipStream_t s)
 {
   RecImage one;
   PM_TRY(rec_describe(marker, mask, out, d, 0, one));
-  return rec_rounds<DIL>(one, nullptr, 1, one.tiles_x * one.tiles_y, se, s);
+  return rec_rounds_alloc<DIL>(one, nullptr, 1, one.tiles_x * one.tiles_y, se, s);
+}
+
+// ---------------------------------------------------------------- scratch-free reconstruction
+// Device scratch of the scratch-free entry point: one array per GPU, held until the process
+// ends.  Host code never uses the array's name as an address; it uses the address that
+// hipGetSymbolAddress returns for the current device, kept per device.
+constexpr size_t NOALLOC_SCRATCH_INTS = (size_t)1 << 18;  // 1 MiB
+static __device__ __align__(256) int scratch[NOALLOC_SCRATCH_INTS];
+
+static int noalloc_scratch(int **ptr)
+{
+  static std::mutex mu;
+  static int *addr[PM_MAX_DEVICES] = {};
+  int dev = 0;
+  PM_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= PM_MAX_DEVICES) PM_FAIL(PM_E_ARG, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      PM_FAIL(PM_E_HIP, "scratch-free reconstruction: hipGetSymbolAddress failed: %s", hipGetErrorString(e));
+    addr[dev] = (int *)p;
+  }
+  *ptr = addr[dev];
+  return PM_OK;
 }
 
 template int rec_u8<true>(const uint8_t *, const uint8_t *, uint8_t *, Dims, int, hipStream_t);
@@ -366,6 +414,36 @@ extern "C" int pm_batch_reconstruct(int n, int rec_type, const pm_image *markers
   Scratch dsc;
   PM_TRY(dsc.alloc(sizeof(RecImage) * (size_t)n, s));
   PM_HIP(hipMemcpyAsync(dsc.ptr, host.data(), sizeof(RecImage) * (size_t)n, hipMemcpyHostToDevice, s));
-  if (rec_type == PM_REC_DILATE) return rec_rounds<true>(host[0], (const RecImage *)dsc.ptr, n, (int)tiles, se, s);
-  return rec_rounds<false>(host[0], (const RecImage *)dsc.ptr, n, (int)tiles, se, s);
+  if (rec_type == PM_REC_DILATE) return rec_rounds_alloc<true>(host[0], (const RecImage *)dsc.ptr, n, (int)tiles, se, s);
+  return rec_rounds_alloc<false>(host[0], (const RecImage *)dsc.ptr, n, (int)tiles, se, s);
+}
+
+// pm_reconstruct for uint8 2D images and 3x3-class SEs, allocating nothing: the activity tags
+// and round counters live in the file-scope device array `scratch` (the convergence counters
+// are read into host stack memory).  Calls on one GPU share that array: they must not overlap.
+extern "C" int pm_reconstruct_noalloc(int rec_type, const pm_image *marker, const pm_image *mask, int se,
+                                      pm_image *out, void *stream)
+{
+  clear_error();
+  PM_TRY(check_binary(marker, mask, out, "Reconstruction"));
+  if (rec_type != PM_REC_DILATE && rec_type != PM_REC_ERODE) PM_FAIL(PM_E_ARG, "Unsupported reconstruction type");
+  PM_TRY(check_se_rank(se, marker->rank));
+  if (marker->ptype != PM_PTYPE_U8)
+    PM_FAIL(PM_E_UNSUPPORTED, "scratch-free reconstruction: uint8 images only");
+  const Dims d = dims3(*marker);
+  if (d.D * d.H * d.W == 0) return PM_OK;
+  if (!rec_u8_applicable(se, d))
+    PM_FAIL(PM_E_UNSUPPORTED, "scratch-free reconstruction: 1D/2D images with a 3x3-class SE only");
+  RecImage one;
+  PM_TRY(rec_describe((const uint8_t *)marker->data, (const uint8_t *)mask->data, (uint8_t *)out->data, d, 0, one));
+  const int64_t nt = (int64_t)one.tiles_x * one.tiles_y;
+  if (rec_scratch_ints(nt) > NOALLOC_SCRATCH_INTS)
+    PM_FAIL(PM_E_ARG, "scratch-free reconstruction: %lld tiles of %dx%d exceed the %zu-tile limit", (long long)nt, RW,
+            RW, NOALLOC_SCRATCH_INTS - REC_MAX_ROUNDS);
+  int *sc = nullptr;
+  PM_TRY(noalloc_scratch(&sc));
+  int hmoved[64];
+  hipStream_t s = as_stream(stream);
+  if (rec_type == PM_REC_DILATE) return rec_rounds<true>(one, nullptr, 1, (int)nt, se, sc, hmoved, s);
+  return rec_rounds<false>(one, nullptr, 1, (int)nt, se, sc, hmoved, s);
 }
