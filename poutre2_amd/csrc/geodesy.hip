@@ -220,7 +220,66 @@ static int leveling_half(const T *ref, const T *marker, T *out, Dims d, int se, 
   return ew2<T, LOW ? EW_INF : EW_SUP>(r, ref, out, n, s);
 }
 
+// Order of the marker g against the reference f: bit 0 set when some g > f, bit 1 when some
+// g < f (both set when a comparison fails, e.g. NaN).  One atomicOr per wave.
+template <typename T>
+__global__ __launch_bounds__(256) void order_kernel(const T *__restrict__ f, const T *__restrict__ g, int64_t n,
+                                                    int *__restrict__ bits)
+{
+  constexpr int E = 16 / (int)sizeof(T);  // elements per 16-byte chunk
+  const int64_t nchunks = n / E;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int b = 0;
+  for (int64_t c = tid; c < nchunks; c += stride) {
+    const uint4 fa = load16_any(reinterpret_cast<const uint8_t *>(f + c * E));
+    const uint4 ga = load16_any(reinterpret_cast<const uint8_t *>(g + c * E));
+    T fv[E], gv[E];
+    __builtin_memcpy(fv, &fa, 16);
+    __builtin_memcpy(gv, &ga, 16);
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      if (!(gv[e] <= fv[e])) b |= 1;
+      if (!(gv[e] >= fv[e])) b |= 2;
+    }
+  }
+  for (int64_t i = nchunks * E + tid; i < n; i += stride) {
+    const T a = f[i], c = g[i];
+    if (!(c <= a)) b |= 1;
+    if (!(c >= a)) b |= 2;
+  }
+  // one global atomic per block: atomics from every wave to one address serialise in L2
+  __shared__ int blk;
+  if (threadIdx.x == 0) blk = 0;
+  __syncthreads();
+  const bool gt = __any(b & 1), lt = __any(b & 2);
+  if ((threadIdx.x & 63) == 0 && (gt || lt)) atomicOr(&blk, (gt ? 1 : 0) | (lt ? 2 : 0));
+  __syncthreads();
+  if (threadIdx.x == 0 && blk) atomicOr(bits, blk);
+}
+
+template <typename T> static int marker_order(const T *f, const T *g, int64_t n, int *bits, hipStream_t s)
+{
+  Scratch sc;
+  PM_TRY(sc.alloc(sizeof(int), s));
+  hipLaunchKernelGGL(zero_flag_kernel, dim3(1), dim3(64), 0, s, (int *)sc.ptr);
+  hipLaunchKernelGGL((order_kernel<T>), dim3(grid1d(n / (16 / (int64_t)sizeof(T)) + 1, 256, 1024)), dim3(256), 0, s, f, g, n,
+                     (int *)sc.ptr);
+  PM_HIP(hipGetLastError());
+  static thread_local int *hbits = nullptr;  // pinned, allocated once per host thread
+  if (!hbits) PM_HIP(hipHostMalloc((void **)&hbits, sizeof(int), hipHostMallocDefault));
+  PM_HIP(hipMemcpyAsync(hbits, sc.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+  PM_HIP(hipStreamSynchronize(s));
+  *bits = *hbits;
+  return PM_OK;
+}
+
 enum { LEV_LOW, LEV_HIGH, LEV_BOTH };
+// leveling_t.hpp:34-202 through the closed forms low(f, g) = f ^ R^d_{f v g}(g) and
+// high(f, g) = f v R^e_{f ^ g}(g), leveling = low(high(f, g), g) (leveling.cpp:231-237).
+// When g <= f everywhere: f v g = f and R^d_f(g) <= f, so low(f, g) = R^d_f(g); f ^ g = g and
+// R^e_g(g) = g, so high(f, g) = f v g = f, hence leveling(f, g) = R^d_f(g).  Dually, g >= f
+// gives high(f, g) = R^e_f(g) <= g, low(f, g) = f and leveling(f, g) = R^e_f(g).
 template <int K>
 static int leveling_entry(const pm_image *ref, const pm_image *marker, int se, pm_image *out, void *stream)
 {
@@ -232,6 +291,20 @@ static int leveling_entry(const pm_image *ref, const pm_image *marker, int se, p
       using T = typename decltype(tag)::type;
       const T *r = (const T *)rf->data, *g = (const T *)mk->data;
       T *dst = (T *)o->data;
+      const int64_t n = d.D * d.H * d.W;
+      if (n == 0) return PM_OK;
+      // With the marker on one side of the reference the closed forms collapse (see the
+      // function comment): g <= f gives low(f, g) = leveling(f, g) = R^d_f(g) and
+      // high(f, g) = f; g >= f gives high(f, g) = leveling(f, g) = R^e_f(g) and low(f, g) = f.
+      int bits = 3;
+      PM_TRY(marker_order<T>(r, g, n, &bits, s));
+      const bool g_le_f = !(bits & 1), g_ge_f = !(bits & 2);
+      if (g_le_f && K != LEV_HIGH) return reconstruct_dev<T, true>(g, r, dst, d, se, s);
+      if (g_ge_f && K != LEV_LOW) return reconstruct_dev<T, false>(g, r, dst, d, se, s);
+      if ((g_le_f && K == LEV_HIGH) || (g_ge_f && K == LEV_LOW)) {
+        PM_HIP(hipMemcpyAsync(dst, r, (size_t)n * sizeof(T), hipMemcpyDeviceToDevice, s));
+        return PM_OK;
+      }
       if (K == LEV_LOW) return leveling_half<T, true>(r, g, dst, d, se, s);
       if (K == LEV_HIGH) return leveling_half<T, false>(r, g, dst, d, se, s);
       Scratch t;

@@ -135,6 +135,34 @@ struct Scratch {
   ~Scratch();
 };
 
+// 16 and 4 bytes at any byte address.  gfx950 under ROCm runs with unaligned memory
+// access enabled (the compiler emits these as single global_load/store_dwordx4 / _dwordx3 /
+// _dword / _short instructions at the unaligned address); each access reads or writes exactly its own
+// bytes, so no neighbouring byte of another tile or row is touched.
+typedef unsigned int u32x4_a1 __attribute__((ext_vector_type(4), aligned(1)));
+typedef unsigned int u32_a1 __attribute__((aligned(1)));
+
+// 16 bytes at any address p, all of them inside the image
+__device__ __forceinline__ uint4 load16_any(const uint8_t *p)
+{
+  const u32x4_a1 v = *reinterpret_cast<const u32x4_a1 *>(p);
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void store16_any(uint8_t *p, uint32_t a, uint32_t b, uint32_t c, uint32_t d)
+{
+  const u32x4_a1 v = {a, b, c, d};
+  *reinterpret_cast<u32x4_a1 *>(p) = v;
+}
+__device__ __forceinline__ void store4_any(uint8_t *p, uint32_t a) { *reinterpret_cast<u32_a1 *>(p) = a; }
+typedef unsigned int u32x3_a1 __attribute__((ext_vector_type(3), aligned(1)));
+typedef unsigned short u16_a1 __attribute__((aligned(1)));
+__device__ __forceinline__ void store12_any(uint8_t *p, uint32_t a, uint32_t b, uint32_t c)
+{
+  const u32x3_a1 v = {a, b, c};
+  *reinterpret_cast<u32x3_a1 *>(p) = v;
+}
+__device__ __forceinline__ void store2_any(uint8_t *p, uint32_t a) { *reinterpret_cast<u16_a1 *>(p) = (unsigned short)a; }
+
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
 constexpr int PM_MAX_DEVICES = 64;

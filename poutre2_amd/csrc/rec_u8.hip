@@ -44,6 +44,13 @@ struct RecImage {
   int H, W, tiles_x, tiles_y, tile0;
 };
 
+template <typename P> __device__ __forceinline__ P *uniform_ptr(P *p)
+{
+  const uint64_t u = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u), hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+  return (P *)(((uint64_t)hi << 32) | lo);
+}
+
 __device__ __forceinline__ int rec_find(const RecImage *im, int n, int t)
 {
   int lo = 0, hi = n - 1;
@@ -67,13 +74,30 @@ __device__ __forceinline__ uint32_t from_right(uint32_t v) { return __builtin_am
 
 __device__ __forceinline__ uint32_t mbyte(const uint32_t (&m)[16], int y) { return (m[y >> 2] >> ((y & 3) * 8)) & 0xffu; }
 
+// The packed masks are loop-invariant; unpacked once outside the sweep loop they would take
+// 128 registers.  An empty asm that "rewrites" them before each sweep keeps the unpacking in
+// the sweep (one bit-field extract per row) and the kernel at a few waves per SIMD.
+__device__ __forceinline__ void opaque(uint32_t (&m)[16])
+{
+#pragma unroll
+  for (int k = 0; k < 16; k++) asm volatile("" : "+v"(m[k]));
+}
+
 // One sweep along the register (row) index in direction DOWN (increasing y) or up, over
 // rows 1..62; returns the lane's sum of values after the sweep (monotone updates: the sum
 // moves iff a pixel moved).  NB is the neighbourhood as seen in this layout.
+// The window max distributes over rows: for the square, c = max(h(new[y-1]), h(old[y]),
+// h(old[y+1])) with h the max over the lane and its two neighbours; the rows not yet
+// updated in this sweep (y, y+1) do not depend on the previous row's result, so their part
+// is computed off the dependency chain, and the chain from one row to the next is DPP,
+// max3, min.
 template <bool DIL, int NB, bool DOWN>
 __device__ __forceinline__ uint32_t sweep(uint32_t (&v)[64], const uint32_t (&m)[16])
 {
   uint32_t sum = v[0] + v[63];
+  auto op2 = [](uint32_t a, uint32_t b) { return DIL ? max(a, b) : min(a, b); };
+  auto h = [](uint32_t a) { return op3<DIL>(a, from_left(a), from_right(a)); };
+  uint32_t h_cur = (NB == NBK_SQUARE) ? h(v[DOWN ? 1 : 62]) : 0u;  // h(old[y]) of the first row
 #pragma unroll
   for (int i = 1; i <= 62; i++) {
     const int y = DOWN ? i : 63 - i;
@@ -81,10 +105,13 @@ __device__ __forceinline__ uint32_t sweep(uint32_t (&v)[64], const uint32_t (&m)
     const int yn = DOWN ? y + 1 : y - 1;
     uint32_t c;
     if (NB == NBK_SQUARE) {
-      const uint32_t w = op3<DIL>(v[yp], v[y], v[yn]);
-      c = op3<DIL>(w, from_left(w), from_right(w));
+      const uint32_t h_next = h(v[yn]);            // old row y+1: off the chain
+      const uint32_t x = op2(h_cur, h_next);
+      c = op3<DIL>(from_left(v[yp]), from_right(v[yp]), op2(v[yp], x));
+      h_cur = h_next;
     } else if (NB == NBK_CROSS) {
-      c = op3<DIL>(op3<DIL>(v[yp], v[y], v[yn]), from_left(v[y]), from_right(v[y]));
+      const uint32_t x = op3<DIL>(v[y], from_left(v[y]), from_right(v[y]));
+      c = op3<DIL>(v[yp], v[yn], x);
     } else if (NB == NBK_SEGY) {  // neighbours along the register index only
       c = op3<DIL>(v[yp], v[y], v[yn]);
     } else {  // NBK_SEGX: neighbours across lanes only
@@ -97,144 +124,233 @@ __device__ __forceinline__ uint32_t sweep(uint32_t (&v)[64], const uint32_t (&m)
   return sum;
 }
 
+// Transpose the 64x64 byte tile held one byte per register (lane = column, register = row)
+// through LDS, into lane = row, register = column (the same call transposes back); returns
+// the lane's sum of values in the new layout.
+__device__ __forceinline__ uint32_t to_other_layout(uint32_t (&v)[64], uint8_t *L, int lane)
+{
+#pragma unroll
+  for (int y = 0; y < 64; y++) L[y * LP + lane] = (uint8_t)v[y];
+  __syncthreads();
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    const uint32_t w = *reinterpret_cast<const uint32_t *>(&L[lane * LP + 4 * k]);
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      v[4 * k + b] = (w >> (8 * b)) & 0xffu;
+      sum += v[4 * k + b];
+    }
+  }
+  __syncthreads();
+  return sum;
+}
+
 template <bool DIL, int NB>
-__global__ __launch_bounds__(64) void rec_u8_kernel(RecImage one, const RecImage *__restrict__ many, int nimg,
-                                                    int round, int *__restrict__ act, int *__restrict__ moved)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void rec_u8_kernel(
+    RecImage one, const RecImage *__restrict__ many, int nimg, int nt, int round, int *__restrict__ act,
+    int *__restrict__ moved)
 {
   if (round > 0 && moved[round - 1] == 0) return;  // converged in an earlier round
-  const int t = blockIdx.x;
+  // XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin, so XCD x (block
+  // x mod 8) takes the contiguous tile range [x * per, (x + 1) * per): horizontally adjacent
+  // tiles, which share cache lines, meet in the same L2
+  const int per = (nt + 7) >> 3;
+  const int t = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+  if (t >= nt) return;
   // every tile runs in round 0 (it initialises its part of cur); later rounds run the tiles
   // tagged by a neighbour whose border moved in the previous round
   if (round > 0 && act[t] != round) return;
-  const RecImage im = many ? many[rec_find(many, nimg, t)] : one;
+  RecImage im = many ? many[rec_find(many, nimg, t)] : one;
+  // the descriptor is the same for the whole wave: keep it (and everything derived from it)
+  // in scalar registers
+  im.cur = uniform_ptr(im.cur);
+  im.marker = uniform_ptr(im.marker);
+  im.mask = uniform_ptr(im.mask);
+  im.H = __builtin_amdgcn_readfirstlane(im.H);
+  im.W = __builtin_amdgcn_readfirstlane(im.W);
+  im.tiles_x = __builtin_amdgcn_readfirstlane(im.tiles_x);
+  im.tiles_y = __builtin_amdgcn_readfirstlane(im.tiles_y);
+  im.tile0 = __builtin_amdgcn_readfirstlane(im.tile0);
   uint8_t *__restrict__ cur = im.cur;
-  const uint8_t *__restrict__ marker = im.marker;
   const uint8_t *__restrict__ mask = im.mask;
   const int H = im.H, W = im.W, tiles_x = im.tiles_x;
   __shared__ uint8_t L[64 * LP];
   const int lane = threadIdx.x;
   const int tl = t - im.tile0;
   const int ty = tl / tiles_x, tx = tl - ty * tiles_x;
-  const int gx = tx * RW - 1 + lane;
-  const int gy0 = ty * RW - 1;
-  const uint32_t NEU = DIL ? 0u : 255u;
+  const int x0 = tx * RW;         // first interior column; the tile spans columns x0-1 .. x0+62
+  const int gy = ty * RW - 1 + lane;  // this lane's row (layout B: lane = row)
+  const bool row_in = (unsigned)gy < (unsigned)H;
   const bool lane_halo = (lane == 0 || lane == 63);
-  const bool col_in = (unsigned)gx < (unsigned)W;
+  const uint32_t NEU = DIL ? 0u : 255u;
+  const uint8_t *__restrict__ vsrc = (round == 0) ? im.marker : cur;
+  // interior tiles read and write whole row segments: 16-byte loads at any byte address
+  const bool wide = x0 >= 1 && x0 + 63 <= W;
 
-  // ---- load the column of this lane (layout A: lane = column, register = row).  Addresses
-  // are clamped into the image and the loads issued without branches, so all of them are in
-  // flight together; out-of-image pixels are replaced by the neutral value afterwards.
-  uint32_t v[64], mA[16];
-  const int cgx = gx < 0 ? 0 : (gx >= W ? W - 1 : gx);
-  const uint8_t *vsrc = (round == 0) ? marker : cur;
-  uint32_t mk_raw[64];
+  // ---- load (layout B: lane = row, register j = column x0 - 1 + j).  mB holds the mask of
+  // the lane's row packed four columns per register.
+  uint32_t v[64], mB[16];
+  if (wide) {
+    if (row_in) {
+      const uint8_t *pr = vsrc + (int64_t)gy * W + (x0 - 1);
+      const uint8_t *pm = mask + (int64_t)gy * W + (x0 - 1);
+      uint4 a[4], m[4];
 #pragma unroll
-  for (int y = 0; y < 64; y++) {
-    int gy = gy0 + y;
-    gy = gy < 0 ? 0 : (gy >= H ? H - 1 : gy);
-    const int64_t o = (int64_t)gy * W + cgx;
-    v[y] = vsrc[o];
-    mk_raw[y] = mask[o];
+      for (int q = 0; q < 4; q++) {
+        a[q] = load16_any(pr + 16 * q);
+        m[q] = load16_any(pm + 16 * q);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const uint32_t w[4] = {a[q].x, a[q].y, a[q].z, a[q].w};
+        mB[4 * q] = m[q].x;
+        mB[4 * q + 1] = m[q].y;
+        mB[4 * q + 2] = m[q].z;
+        mB[4 * q + 3] = m[q].w;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+          for (int b = 0; b < 4; b++) v[16 * q + 4 * k + b] = (w[k] >> (8 * b)) & 0xffu;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 64; j++) v[j] = NEU;
+#pragma unroll
+      for (int k = 0; k < 16; k++) mB[k] = NEU * 0x01010101u;
+    }
+  } else {
+    // tiles at the left or right edge of the image: byte loads at clamped addresses, 16
+    // columns at a time; pixels outside the image become the neutral value
+    const int cgy = gy < 0 ? 0 : (gy >= H ? H - 1 : gy);
+    const uint8_t *pr = vsrc + (int64_t)cgy * W;
+    const uint8_t *pm = mask + (int64_t)cgy * W;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      uint32_t mk_raw[16];
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        const int x = x0 - 1 + 16 * c + i;
+        const int cx = x < 0 ? 0 : (x >= W ? W - 1 : x);
+        v[16 * c + i] = pr[cx];
+        mk_raw[i] = pm[cx];
+      }
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        const int j = 16 * c + i;
+        const int x = x0 - 1 + j;
+        const bool in = row_in && (unsigned)x < (unsigned)W;
+        v[j] = in ? v[j] : NEU;
+        const uint32_t mb = (in ? mk_raw[i] : NEU) << ((j & 3) * 8);
+        mB[j >> 2] = (j & 3) ? (mB[j >> 2] | mb) : mb;
+      }
+      __builtin_amdgcn_sched_barrier(0);  // the next chunk's loads stay after this chunk
+    }
   }
+  // round 0 starts from marker ^ mask (marker v mask for erosion): the reference's
+  // select_marker step; halo pixels (rows 0 / 63 = lanes 0 / 63, columns 0 / 63) get
+  // mask = value, which pins them
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    uint32_t mw = 0;
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      const int j = 4 * k + b;
+      uint32_t mk = (mB[k] >> (8 * b)) & 0xffu, val = v[j];
+      if (round == 0) val = DIL ? min(val, mk) : max(val, mk);
+      if (lane_halo || j == 0 || j == 63) mk = val;
+      v[j] = val;
+      mw |= mk << (8 * b);
+    }
+    mB[k] = mw;
+  }
+
+  // border values at the start: rows 1 / 62 (lanes 1 / 62: their row sums), columns 1 / 62
+  uint32_t rowsum0 = 0;
+#pragma unroll
+  for (int j = 1; j <= 62; j++) rowsum0 += v[j];
+  const uint32_t left0 = v[1], right0 = v[62];
+
+  constexpr int NBA = NB;  // neighbourhood seen by the sweep along rows of layout A (image y)
+  constexpr int NBB = (NB == NBK_SEGX) ? NBK_SEGY : (NB == NBK_SEGY) ? NBK_SEGX : NB;
+  // Sweeps right, left (layout B), then down, up (layout A), repeated.  Every sweep step
+  // updates a pixel from its whole neighbourhood, so a sweep that moves no pixel proves the
+  // tile is at its local fixed point: the loop stops at the first such sweep (sums of the
+  // lane's pixels detect a move, the updates being monotone).
+  uint32_t sp = 0;
+#pragma unroll
+  for (int j = 0; j < 64; j++) sp += v[j];
+  bool any = false, in_a = false, have_ma = false;
+  uint32_t mA[16];
 #pragma unroll
   for (int k = 0; k < 16; k++) mA[k] = 0;
-#pragma unroll
-  for (int y = 0; y < 64; y++) {
-    const int gy = gy0 + y;
-    const bool in = col_in && (unsigned)gy < (unsigned)H;
-    uint32_t mk = mk_raw[y], val = v[y];
-    if (round == 0) val = DIL ? min(val, mk) : max(val, mk);  // select_marker
-    if (lane_halo || y == 0 || y == 63) mk = val;             // halo pixels are pinned
-    v[y] = in ? val : NEU;
-    mA[y >> 2] |= (in ? mk : NEU) << ((y & 3) * 8);
-  }
-
-  // ---- mask in layout B (lane = row) through LDS
-  uint32_t mB[16];
-#pragma unroll
-  for (int y = 0; y < 64; y++) L[y * LP + lane] = (uint8_t)mbyte(mA, y);
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < 16; k++) mB[k] = *reinterpret_cast<const uint32_t *>(&L[lane * LP + 4 * k]);
-  __syncthreads();
-
-  const uint32_t top0 = v[1], bot0 = v[62];
-  uint32_t colsum0 = 0;
-#pragma unroll
-  for (int y = 1; y <= 62; y++) colsum0 += v[y];
-
-  constexpr int NBA = NB;  // neighbourhood seen by the vertical sweep
-  constexpr int NBB = (NB == NBK_SEGX) ? NBK_SEGY : (NB == NBK_SEGY) ? NBK_SEGX : NB;
-  uint32_t s_prev = 0;
-#pragma unroll
-  for (int y = 0; y < 64; y++) s_prev += v[y];
-  bool any = false;
   for (int it = 0; it < 4096; it++) {
-    bool ch = false;
-    uint32_t s = sweep<DIL, NBA, true>(v, mA);
-    ch |= s != s_prev;
-    s_prev = s;
-    s = sweep<DIL, NBA, false>(v, mA);
-    ch |= s != s_prev;
-    // transpose to layout B
-#pragma unroll
-    for (int y = 0; y < 64; y++) L[y * LP + lane] = (uint8_t)v[y];
-    __syncthreads();
-    uint32_t sb = 0;
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
-      const uint32_t w = *reinterpret_cast<const uint32_t *>(&L[lane * LP + 4 * k]);
-#pragma unroll
-      for (int b = 0; b < 4; b++) {
-        v[4 * k + b] = (w >> (8 * b)) & 0xffu;
-        sb += v[4 * k + b];
-      }
-    }
-    __syncthreads();
-    s = sweep<DIL, NBB, true>(v, mB);
-    ch |= s != sb;
-    sb = s;
-    s = sweep<DIL, NBB, false>(v, mB);
-    ch |= s != sb;
-    // back to layout A
-#pragma unroll
-    for (int x = 0; x < 64; x++) L[x * LP + lane] = (uint8_t)v[x];
-    __syncthreads();
-    s_prev = 0;
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
-      const uint32_t w = *reinterpret_cast<const uint32_t *>(&L[lane * LP + 4 * k]);
-#pragma unroll
-      for (int b = 0; b < 4; b++) {
-        v[4 * k + b] = (w >> (8 * b)) & 0xffu;
-        s_prev += v[4 * k + b];
-      }
-    }
-    __syncthreads();
-    if (!__any(ch)) break;
+    opaque(mB);
+    uint32_t s = sweep<DIL, NBB, true>(v, mB);
+    if (!__any(s != sp)) break;
+    sp = s;
     any = true;
-  }
-
-  // ---- write back the interior (always in round 0: it initialises cur)
-  if (any || round == 0) {
-    if (!lane_halo && col_in) {
+    opaque(mB);
+    s = sweep<DIL, NBB, false>(v, mB);
+    if (!__any(s != sp)) break;
+    sp = to_other_layout(v, L, lane);
+    in_a = true;
+    if (!have_ma) {
+      // the mask in layout A (lane = column), built through LDS the first time the tile
+      // needs it (tiles that converge in their first row sweeps never do)
 #pragma unroll
-      for (int y = 1; y <= 62; y++) {
-        const int gy = gy0 + y;
-        if (gy < H) cur[(int64_t)gy * W + gx] = (uint8_t)v[y];
+      for (int j = 0; j < 64; j++) L[j * LP + lane] = (uint8_t)mbyte(mB, j);
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 16; k++) mA[k] = *reinterpret_cast<const uint32_t *>(&L[lane * LP + 4 * k]);
+      __syncthreads();
+      have_ma = true;
+    }
+    opaque(mA);
+    s = sweep<DIL, NBA, true>(v, mA);
+    if (!__any(s != sp)) break;
+    sp = s;
+    opaque(mA);
+    s = sweep<DIL, NBA, false>(v, mA);
+    if (!__any(s != sp)) break;
+    sp = to_other_layout(v, L, lane);
+    in_a = false;
+  }
+  if (in_a) (void)to_other_layout(v, L, lane);
+
+  // ---- write back the interior (always in round 0: it initialises cur), layout B
+  if ((any || round == 0) && !lane_halo && row_in) {
+    uint8_t *q = cur + (int64_t)gy * W + x0;  // interior columns x0 .. x0 + 61 = v[1 .. 62]
+    if (wide) {
+      uint32_t P[16];
+#pragma unroll
+      for (int k = 0; k < 16; k++) {
+        const int j = 1 + 4 * k;
+        P[k] = v[j] | (v[j + 1] << 8) | (v[j + 2] << 16) | (j + 3 < 64 ? (v[j + 3] << 24) : 0u);
       }
+      store16_any(q, P[0], P[1], P[2], P[3]);
+      store16_any(q + 16, P[4], P[5], P[6], P[7]);
+      store16_any(q + 32, P[8], P[9], P[10], P[11]);
+      store12_any(q + 48, P[12], P[13], P[14]);
+      store2_any(q + 60, P[15]);
+    } else {
+      int sx = x0;  // recomputed here: store addresses are not kept across the sweeps
+      asm volatile("" : "+v"(sx));
+      uint8_t *qq = cur + (int64_t)gy * W + sx;
+#pragma unroll
+      for (int j = 1; j <= 62; j++)
+        if (sx + j - 1 < W) qq[j - 1] = (uint8_t)v[j];
     }
   }
   if (!any) return;
-  // ---- which borders moved: rows 1/62 in any lane, columns 1/62 (lanes 1 and 62)
-  uint32_t colsum = 0;
+  // ---- which borders moved: rows 1 / 62 (lanes 1 / 62), columns 1 / 62 in any interior lane
+  uint32_t rowsum = 0;
 #pragma unroll
-  for (int y = 1; y <= 62; y++) colsum += v[y];
-  const bool interior_lane = !lane_halo;
-  const bool top = __any(interior_lane && v[1] != top0);
-  const bool bot = __any(interior_lane && v[62] != bot0);
-  const bool left = __any(lane == 1 && colsum != colsum0);
-  const bool right = __any(lane == 62 && colsum != colsum0);
+  for (int j = 1; j <= 62; j++) rowsum += v[j];
+  const bool top = __any(lane == 1 && rowsum != rowsum0);
+  const bool bot = __any(lane == 62 && rowsum != rowsum0);
+  const bool left = __any(!lane_halo && v[1] != left0);
+  const bool right = __any(!lane_halo && v[62] != right0);
   if (lane == 0 && (top || bot || left || right)) {
     const int tiles_y = im.tiles_y;
     const int nr = round + 1;
@@ -284,6 +400,7 @@ static int rec_rounds(const RecImage &one, const RecImage *many, int nimg, int n
   if (se == PM_SE_CROSS_2D) nbk = NBK_CROSS;
   else if (se == PM_SE_SEGMENT_X_1D || se == PM_SE_SEGMENT_X_2D) nbk = NBK_SEGX;
   else if (se == PM_SE_SEGMENT_Y_2D) nbk = NBK_SEGY;
+  const unsigned grid = 8u * (unsigned)((nt + 7) / 8);  // every tile index below 8 * ceil(nt / 8)
   int round = 0, batch = 8;
   while (round < REC_MAX_ROUNDS) {
     const int nb = (round + batch <= REC_MAX_ROUNDS) ? batch : REC_MAX_ROUNDS - round;
@@ -291,16 +408,16 @@ static int rec_rounds(const RecImage &one, const RecImage *many, int nimg, int n
       const int R = round + i;
       switch (nbk) {
       case NBK_SQUARE:
-        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SQUARE>), dim3(nt), dim3(64), 0, s, one, many, nimg, R, act, moved);
+        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SQUARE>), dim3(grid), dim3(64), 0, s, one, many, nimg, nt, R, act, moved);
         break;
       case NBK_CROSS:
-        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_CROSS>), dim3(nt), dim3(64), 0, s, one, many, nimg, R, act, moved);
+        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_CROSS>), dim3(grid), dim3(64), 0, s, one, many, nimg, nt, R, act, moved);
         break;
       case NBK_SEGX:
-        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SEGX>), dim3(nt), dim3(64), 0, s, one, many, nimg, R, act, moved);
+        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SEGX>), dim3(grid), dim3(64), 0, s, one, many, nimg, nt, R, act, moved);
         break;
       default:
-        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SEGY>), dim3(nt), dim3(64), 0, s, one, many, nimg, R, act, moved);
+        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SEGY>), dim3(grid), dim3(64), 0, s, one, many, nimg, nt, R, act, moved);
         break;
       }
     }

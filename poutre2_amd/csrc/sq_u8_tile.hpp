@@ -60,25 +60,7 @@ template <bool OPQ> __device__ __forceinline__ int tile_tid()
 
 __device__ __forceinline__ uint32_t lds_word(const uint8_t *p) { return *reinterpret_cast<const uint32_t *>(p); }
 
-// 16 and 4 bytes at any byte address.  gfx950 under ROCm runs with unaligned memory
-// access enabled (the compiler emits these as single global_load/store_dwordx4 / _dword
-// instructions at the unaligned address); each access reads or writes exactly its own
-// bytes, so no neighbouring byte of another tile or row is touched.
-typedef unsigned int u32x4_a1 __attribute__((ext_vector_type(4), aligned(1)));
-typedef unsigned int u32_a1 __attribute__((aligned(1)));
-
-// 16 bytes at any address p, all of them inside the image
-__device__ __forceinline__ uint4 load16_any(const uint8_t *p)
-{
-  const u32x4_a1 v = *reinterpret_cast<const u32x4_a1 *>(p);
-  return make_uint4(v.x, v.y, v.z, v.w);
-}
-__device__ __forceinline__ void store16_any(uint8_t *p, uint32_t a, uint32_t b, uint32_t c, uint32_t d)
-{
-  const u32x4_a1 v = {a, b, c, d};
-  *reinterpret_cast<u32x4_a1 *>(p) = v;
-}
-__device__ __forceinline__ void store4_any(uint8_t *p, uint32_t a) { *reinterpret_cast<u32_a1 *>(p) = a; }
+// load16_any / store16_any / store4_any (unaligned 16- and 4-byte accesses): pm_common.hpp
 
 // Shift the 16 bytes of (v0..v3) by b bytes towards higher (LEFT = true: out byte j =
 // in byte j - b) or lower (out byte j = in byte j + b) positions, 0 <= b <= 15, filling

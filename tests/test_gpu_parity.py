@@ -268,6 +268,24 @@ def test_leveling(op, dtype, se, shape):
     np.testing.assert_array_equal(got, getattr(oracle, op)(ref, marker, se))
 
 
+@pytest.mark.parametrize("op", ["low_leveling", "high_leveling", "leveling"])
+@pytest.mark.parametrize("dtype", [np.uint8, np.int32, np.float32])
+@pytest.mark.parametrize("order", ["opening", "closing", "equal"])
+def test_leveling_ordered_marker(op, dtype, order):
+    """markers below the reference (an opening: the benchmark's case), above it (a closing)
+    and equal to it take the library's single-reconstruction forms; the oracle is the
+    priority-queue restatement"""
+    from poutre2_amd import synth
+    ref = synth.blurred_noise_mask(150, 170, 12, sigma=3.0, radius=10).astype(dtype)
+    if order == "equal":
+        marker = ref.copy()
+    else:
+        first, second = (False, True) if order == "opening" else (True, False)
+        marker = oracle.static_iter(oracle.static_iter(ref, "SESquare2D", 3, first), "SESquare2D", 3, second)
+    got = h(getattr(pm, op)(g(ref), g(marker), PM_SE["SESquare2D"]))
+    np.testing.assert_array_equal(got, getattr(oracle, op)(ref, marker, "SESquare2D"))
+
+
 @pytest.mark.parametrize("op", ["h_maxima", "h_minima", "h_convex", "h_concave", "dynamic_pseudo_opening",
                                 "dynamic_pseudo_closing"])
 @pytest.mark.parametrize("dtype", [np.uint8, np.int32])
