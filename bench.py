@@ -122,30 +122,45 @@ def bench_ops_8k(pm, torch, dev, reps=9):
     return out
 
 
-def bench_recon_4k(pm, torch, dev, reps=3):
-    import numpy as np
+def bench_recon_4k(pm, torch, dev, reps=6):
+    """4096^2 h-maxima reconstruction (h = 32, blurred-noise mask seed 4) and leveling with the
+    15x15-opening marker.  Timed on 6 rotating sets (the mask rolled by a different offset
+    each: the same statistics, 288 MiB in all, more than the 256 MiB MALL holds); the
+    figure on one set (48 MiB, MALL-hot) is reported beside it."""
     from poutre2_amd import synth
     H = W = 4096
     npx = H * W
-    mask_np = synth.blurred_noise_mask(H, W, 4)
-    marker_np = synth.hmax_marker(mask_np, 32)
-    mask = torch.from_numpy(mask_np).to(dev)
-    marker = torch.from_numpy(marker_np).to(dev)
-    out = torch.empty_like(mask)
+    mask0 = synth.blurred_noise_mask(H, W, 4, device=dev)
+    sets = []
+    for i in range(6):
+        m = mask0 if i == 0 else mask0.roll((977 * i, 1931 * i), dims=(0, 1)).contiguous()
+        sets.append((synth.hmax_marker(m, 32), m, torch.empty_like(m)))
     stream = torch.cuda.current_stream(dev)
-    fn = lambda: pm.reconstruction(pm.RecType.dilate, marker, mask, pm.static_nl_se.square2d, out=out)  # noqa: E731
-    fn()
-    t = timed(fn, reps, stream) / reps
-    res = {"reconstruct_hmax32": {"ms": round(t, 3), "gpx_s": round(npx / (t * 1e-3) / 1e9, 3),
-                                  "pct_hbm": round(100 * 3 * npx / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 2)}}
+    k = [0]
+
+    def rec(rot=True):
+        mk, ms, o = sets[k[0] % 6 if rot else 0]
+        k[0] += 1
+        pm.reconstruction(pm.RecType.dilate, mk, ms, pm.static_nl_se.square2d, out=o)
+    rec()
+    t = timed(rec, reps, stream) / reps
+    th = timed(lambda: rec(False), reps, stream) / reps
+
+    def fig(t):
+        return {"ms": round(t, 3), "gpx_s": round(npx / (t * 1e-3) / 1e9, 3),
+                "pct_hbm": round(100 * 3 * npx / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 2)}
+    res = {"reconstruct_hmax32": dict(fig(t), rotating_buffers_MiB=288, mall_hot_ms=round(th, 3))}
     # leveling with marker = 15x15 opening of the mask
-    g = pm.opening(mask, pm.static_nl_se.square2d, 7)
-    fl = lambda: pm.leveling(mask, g, pm.static_nl_se.square2d, out=out)  # noqa: E731
-    fl()
-    t = timed(fl, reps, stream) / reps
-    res["leveling_open15"] = {"ms": round(t, 3), "gpx_s": round(npx / (t * 1e-3) / 1e9, 3),
-                              "pct_hbm": round(100 * 3 * npx / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 2)}
-    del np
+    lsets = [(ms, pm.opening(ms, pm.static_nl_se.square2d, 7), o) for _, ms, o in sets]
+
+    def lev(rot=True):
+        f, g, o = lsets[k[0] % 6 if rot else 0]
+        k[0] += 1
+        pm.leveling(f, g, pm.static_nl_se.square2d, out=o)
+    lev()
+    t = timed(lev, reps, stream) / reps
+    th = timed(lambda: lev(False), reps, stream) / reps
+    res["leveling_open15"] = dict(fig(t), rotating_buffers_MiB=288, mall_hot_ms=round(th, 3))
     return res
 
 
@@ -174,29 +189,40 @@ def bench_3d(pm, torch, dev, reps=3):
     return out
 
 
-def bench_types(pm, torch, dev, reps=5):
-    """4096x4096 uint16 / int32 / float32 dilation by squares of side 3 and 15 (van Herk x and y
-    line passes through a scratch image); %HBM at 2 * sizeof(T) B/px algorithmic."""
+def bench_types(pm, torch, dev, reps=6):
+    """4096x4096 uint16 / int32 / float32 dilation by squares of side 3 and 15; %HBM at
+    2 * sizeof(T) B/px algorithmic.  Input/output pairs rotate over more than the 256 MiB MALL
+    (3 pairs = 384 MiB for 4-byte types, 5 pairs = 320 MiB for uint16)."""
     import numpy as np
     from poutre2_amd import synth
     g = synth.rng(9)
     out = {}
-    for name, npt in (("uint16", np.uint16), ("int32", np.int32), ("float32", np.float32)):
+    for name, npt, npairs in (("uint16", np.uint16, 5), ("int32", np.int32, 3), ("float32", np.float32, 3)):
         if npt == np.float32:
             a_np = g.standard_normal((4096, 4096), dtype=np.float32)
         else:
             a_np = g.integers(0, 60000, size=(4096, 4096)).astype(npt)
-        a = torch.from_numpy(a_np).to(dev)
-        b = torch.empty_like(a)
+        a0 = torch.from_numpy(a_np).to(dev)
+        # rolled on the host: torch has no device roll for uint16
+        pairs = [(a0 if i == 0 else torch.from_numpy(np.roll(a_np, 613 * i, axis=1)).to(dev), torch.empty_like(a0))
+                 for i in range(npairs)]
         stream = torch.cuda.current_stream(dev)
         per = {}
         for side in (3, 15):
             r = (side - 1) // 2
-            pm.dilate(a, pm.static_nl_se.square2d, r, out=b)
-            t = timed(lambda: pm.dilate(a, pm.static_nl_se.square2d, r, out=b), reps, stream) / reps
-            per[str(side)] = {"ms": round(t, 4), "gpx_s": round(a.numel() / (t * 1e-3) / 1e9, 2),
-                              "pct_hbm": round(100 * 2 * a.element_size() * a.numel() / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 2)}
+            k = [0]
+
+            def one(r=r):
+                a, b = pairs[k[0] % npairs]
+                k[0] += 1
+                pm.dilate(a, pm.static_nl_se.square2d, r, out=b)
+            one()
+            t = timed(one, reps, stream) / reps
+            per[str(side)] = {"ms": round(t, 4), "gpx_s": round(a0.numel() / (t * 1e-3) / 1e9, 2),
+                              "pct_hbm": round(100 * 2 * a0.element_size() * a0.numel() / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 2)}
+        per["rotating_buffers_MiB"] = npairs * 2 * a0.numel() * a0.element_size() >> 20
         out[name] = per
+        del pairs, a0
     return out
 
 
@@ -211,6 +237,7 @@ def bench_small_1k(pm, torch, dev, reps=50):
         fn(a, pm.static_nl_se.square2d, 1, out=b)
         t = timed(lambda: fn(a, pm.static_nl_se.square2d, 1, out=b), reps, stream) / reps
         out[name] = {"us": round(t * 1e3, 2), "gpx_s": round(a.numel() / (t * 1e-3) / 1e9, 2)}
+    out["note"] = "cache-hot: a 2 MiB working set stays in the MALL; a call is launch-bound at this size"
     return out
 
 
@@ -224,7 +251,7 @@ def bench_label_4k(pm, torch, dev, reps=5, cpu=True):
     from poutre2_amd import synth
     H = W = 4096
     npx = H * W
-    mask = synth.blurred_noise_mask(H, W, 4)
+    mask = synth.blurred_noise_mask(H, W, 4, device=dev).cpu().numpy()
     binary = (mask > 127).astype(np.uint8)
     zones = (mask // 32).astype(np.uint8)
     b = torch.from_numpy(binary).to(dev)
@@ -253,10 +280,9 @@ def bench_batched(pm, torch, dev, reps=3):
     U{128..1024}, blurred-noise masks, h = 16) in one round sequence vs one call per image."""
     import numpy as np
     from poutre2_amd import synth
-    g = synth.rng(5)
-    shapes = [(int(g.integers(64, 2049)), int(g.integers(64, 2049))) for _ in range(512)]
-    radii = [int(g.integers(1, 16)) for _ in range(512)]
-    imgs = [torch.from_numpy(g.integers(0, 256, size=s, dtype=np.uint8)).to(dev) for s in shapes]
+    shapes, radii, imgs_np = synth.batch_dilate_open_workload()
+    imgs = [torch.from_numpy(a).to(dev) for a in imgs_np]
+    del imgs_np
     dil = [torch.empty_like(a) for a in imgs]
     opn = [torch.empty_like(a) for a in imgs]
     npx = sum(a.numel() for a in imgs)
@@ -291,13 +317,7 @@ def bench_batched(pm, torch, dev, reps=3):
                                "packed_call_ms": round(tpk, 3), "list_api_ms": round(tb, 3),
                                "per_image_ms": round(tp, 3), "speedup_vs_per_image": round(tp / tbest, 2)}}
     del imgs, dil, opn, flat, dflat, oflat
-    g = synth.rng(6)
-    masks, markers = [], []
-    for i in range(256):
-        H, W = int(g.integers(128, 1025)), int(g.integers(128, 1025))
-        m = synth.blurred_noise_mask_fast(H, W, 1000 + i)
-        masks.append(torch.from_numpy(m).to(dev))
-        markers.append(torch.from_numpy(synth.hmax_marker(m, 16)).to(dev))
+    markers, masks = synth.batch_reconstruction_workload(device=dev)
     outs = [torch.empty_like(m) for m in masks]
     rpx = sum(m.numel() for m in masks)
     fr = lambda: pm.batch_reconstruction(pm.RecType.dilate, markers, masks, pm.static_nl_se.square2d, outs)  # noqa: E731
@@ -447,12 +467,38 @@ def measure_traffic(side, tw):
     return out["FETCH_SIZE"]["bytes"] + out["WRITE_SIZE"]["bytes"], out
 
 
-def cpu_baseline_8k():
-    """Single-thread oracle (restatement of DilateX + DilateY, ero_dil_line_se_t.hpp:49-280)
-    on a bounded sample of the headline workload: one 8192x8192 uint8 dilation per side."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def _cpu_square(oracle, np, img, r, dil):
+    """(2r+1)^2 window: the oracle's DilateX along rows, then along columns (numpy transposes,
+    as the reference's DilateY transposes; any image shape)"""
+    t = oracle.line_x(img, r, dil)
+    return np.ascontiguousarray(oracle.line_x(np.ascontiguousarray(t.T), r, dil).T)
+
+
+def cpu_baseline(torch, dev):
+    """Single-thread oracle (the restatement of the reference's CPU algorithms, compiled -O3
+    -mavx2 as the reference builds) on bounded samples of each workload, timed on this host:
+      headline  one 8192^2 uint8 DilateX+DilateY per side 3..63 (ero_dil_line_se_t.hpp:49-280)
+      C1        Dilate / Erode(SESquare2D, 1) on 1024^2 (ero_dil_static_se_t.hpp:375-472), 5 runs each
+      C3        Reconstruction (hybrid scans + FIFO, mreconstruct_t.hpp:75-195) and leveling
+                (priority queue, leveling_t.hpp:34-202) on the 4096^2 h = 32 workload, once each
+      C4        the per-image loop (dilation + opening) over the first 96 of the 512 batch images,
+                and Reconstruction over the first 64 of the 256 batch masks"""
     import numpy as np
     import oracle
     from poutre2_amd import synth
+    legs = {}
     img = synth.uniform_u8((8192, 8192), 2)
     tot, npx = 0.0, 0
     for s in SIDES:
@@ -462,10 +508,58 @@ def cpu_baseline_8k():
         oracle.line_y(tmp, r, True)
         tot += time.perf_counter() - t0
         npx += img.size
-    del np
-    return {"value": round(npx / tot / 1e9, 4), "unit": "Gpixel/s", "cores": 1, "kind": "port",
+    head = round(npx / tot / 1e9, 4)
+    secs = tot
+    a = synth.uniform_u8((1024, 1024), 1)
+    for name, dil in (("dilate", True), ("erode", False)):
+        t0 = time.perf_counter()
+        for _ in range(5):
+            oracle.static(a, "SESquare2D", dil)
+        t = (time.perf_counter() - t0) / 5
+        secs += 5 * t
+        legs[f"C1_{name}_1024"] = {"ms": round(t * 1e3, 2), "gpx_s": round(a.size / t / 1e9, 4)}
+    mask = synth.blurred_noise_mask(4096, 4096, 4, device=dev).cpu().numpy()
+    marker = synth.hmax_marker(mask, 32)
+    t0 = time.perf_counter()
+    oracle.reconstruct("dilate", marker, mask, "SESquare2D")
+    t = time.perf_counter() - t0
+    legs["C3_reconstruct_hmax32_4096"] = {"ms": round(t * 1e3, 1), "gpx_s": round(mask.size / t / 1e9, 5)}
+    g = _cpu_square(oracle, np, _cpu_square(oracle, np, mask, 7, False), 7, True)
+    t0 = time.perf_counter()
+    oracle.leveling(mask, g, "SESquare2D")
+    t2 = time.perf_counter() - t0
+    legs["C3_leveling_open15_4096"] = {"ms": round(t2 * 1e3, 1), "gpx_s": round(mask.size / t2 / 1e9, 5)}
+    secs += t + t2
+    shapes, radii, imgs = synth.batch_dilate_open_workload()
+    t0 = time.perf_counter()
+    n = 0
+    for a, r in list(zip(imgs, radii))[:96]:
+        _cpu_square(oracle, np, a, r, True)
+        _cpu_square(oracle, np, _cpu_square(oracle, np, a, r, False), r, True)
+        n += a.size
+    t = time.perf_counter() - t0
+    secs += t
+    legs["C4_dilate_open_per_image"] = {"images": 96, "mpx": round(n / 1e6, 1), "ms": round(t * 1e3, 1),
+                                        "gpx_s": round(n / t / 1e9, 4)}
+    del imgs
+    g6 = synth.rng(6)
+    n, t = 0, 0.0
+    for i in range(64):
+        H, W = int(g6.integers(128, 1025)), int(g6.integers(128, 1025))
+        m = synth.blurred_noise_mask(H, W, 1000 + i, device=dev).cpu().numpy()
+        mk = synth.hmax_marker(m, 16)
+        t0 = time.perf_counter()
+        oracle.reconstruct("dilate", mk, m, "SESquare2D")
+        t += time.perf_counter() - t0
+        n += m.size
+    secs += t
+    legs["C4_reconstruct_per_image"] = {"images": 64, "mpx": round(n / 1e6, 1), "ms": round(t * 1e3, 1),
+                                        "gpx_s": round(n / t / 1e9, 4)}
+    return {"value": head, "unit": "Gpixel/s", "cores": 1, "kind": "port",
             "sample": "one 8192x8192 uint8 DilateX+DilateY per side 3,7,15,31,63 (van Herk rows + transposes), "
-                      "oracle compiled -O3 -mavx2, 1 thread", "seconds": round(tot, 2)}
+                      "oracle compiled -O3 -mavx2, 1 thread; legs: C1 1024^2 x5, C3 4096^2 once, "
+                      "C4 first 96 images / 64 masks",
+            "seconds": round(secs, 2), "cpu_model": cpu_model(), "threads": 1, "legs": legs}
 
 
 def main():
@@ -526,8 +620,8 @@ def main():
         extra["autograd"] = bench_autograd(pm, torch, dev)
     cpu = None
     if not args.no_cpu:
-        log("bench: cpu baseline")
-        cpu = cpu_baseline_8k()
+        log("bench: cpu baseline (single-thread oracle)")
+        cpu = cpu_baseline(torch, dev)
     props = torch.cuda.get_device_properties(dev)
     extra["tile_width_table"] = tuned
     line = {
