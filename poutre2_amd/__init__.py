@@ -31,7 +31,7 @@ __all__ = ["static_nl_se", "static_compound_se", "RecType", "erode", "dilate", "
            "h_minima", "h_maxima", "h_concave", "h_convex", "dynamic_pseudo_closing", "dynamic_pseudo_opening",
            "high_leveling", "low_leveling", "leveling", "arith_sup", "arith_inf", "arith_ssub_image",
            "arith_ssub_const", "arith_sadd_const", "batch_dilate_open", "batch_reconstruction", "PoutreError",
-           "version", "PType", "CompoundType", "factory_image", "from_string", "to_string", "as_sizes_compatible",
+           "version", "PType", "CompoundType", "image_ctype", "factory_image", "from_string", "to_string", "as_sizes_compatible",
            "as_types_compatible", "are_images_different", "CompOpType", "compare", "arith_invert", "arith_sadd_image",
            "arith_invert_image", "arith_inf_image", "arith_sup_image", "arith_saturated_add_image",
            "arith_saturated_sub_image", "arith_saturated_add_constant", "arith_saturated_sub_constant", "clone",
@@ -80,14 +80,21 @@ class SE:
     def __repr__(self):
         return f"SE({self._c})"
 
+    def _centre(self):
+        """the first offset, taken as the centre (neighbor_list_se_t.hpp:96-125)"""
+        return self._c[0] if self._c else ()
+
     def transpose(self):
-        return SE([tuple(-v for v in c) for c in self._c])
+        """offsets mirrored about the centre: 2 * centre - offset"""
+        ctr = self._centre()
+        return SE([tuple(2 * a - v for a, v in zip(ctr, c)) for c in self._c])
 
     def type(self):
         return se_type.neighbor_list
 
     def remove_center(self):
-        return SE([c for c in self._c if any(c)])
+        ctr = self._centre()
+        return SE([c for c in self._c if c != ctr])
 
     def size(self):
         return len(self._c)
@@ -152,6 +159,8 @@ def _img(x):
 
 
 def _desc(t):
+    if image_ctype(t) != CompoundType.Scalar:
+        raise PoutreError(-2, "Unsupported compound type")
     t = _img(t)
     if not t.is_cuda:
         raise PoutreError(-6, "images must be device tensors (HBM-resident)")
@@ -171,6 +180,8 @@ def _desc(t):
 def _descs(ts):
     """pm_image array for a list of device tensors, built as one numpy record array (the
     per-image Python work of a 512-image batch would otherwise rival its kernel time)."""
+    if any(image_ctype(t) != CompoundType.Scalar for t in ts):
+        raise PoutreError(-2, "Unsupported compound type")
     ts = [_img(t) for t in ts]
     n = len(ts)
     rec = np.zeros(n, dtype=_PM_IMAGE_NP)
@@ -240,50 +251,116 @@ def _torch_dtype(np_dtype):
             np.int64: torch.int64, np.float32: torch.float32, np.float64: torch.float64}[np_dtype]
 
 
+_PLANES = {CompoundType.Scalar: 1, CompoundType["3Planes"]: 3, CompoundType["4Planes"]: 4}
+
+
+def _tag(t, ctype):
+    """compound images are tensors with a trailing planes axis, tagged with their CompoundType
+    (the operators take Scalar images only, like the reference's dispatch)"""
+    if ctype != CompoundType.Scalar:
+        t._poutre_ctype = ctype
+    return t
+
+
+def image_ctype(img):
+    """the CompoundType of an image made by factory_image / from_string (Scalar otherwise)"""
+    return getattr(img, "_poutre_ctype", CompoundType.Scalar)
+
+
 def factory_image(dims, ctype, ptype, device="cuda"):
     """poutre::Create(dims, ctype, ptype) (python/base/image_interface.cpp:27): a zero-filled
-    image of that shape and type, as a device tensor.  Only Scalar images exist here."""
-    if CompoundType(ctype) != CompoundType.Scalar:
-        raise PoutreError(-2, "Create: only scalar images are supported")
+    image of that shape and type, as a device tensor; 3Planes / 4Planes images carry a
+    trailing axis of 3 / 4 planes."""
+    ctype = CompoundType(ctype)
     dims = tuple(int(d) for d in dims)
-    if len(dims) < 1 or len(dims) > 3:
+    if len(dims) < 1 or len(dims) > 4:
         raise PoutreError(-4, f"Unsupported number of dims {len(dims)}")
-    return _torch().zeros(dims, dtype=_torch_dtype(_PT_INFO[PType(ptype)][1]), device=device)
+    shape = dims + ((_PLANES[ctype],) if ctype != CompoundType.Scalar else ())
+    return _tag(_torch().zeros(shape, dtype=_torch_dtype(_PT_INFO[PType(ptype)][1]), device=device), ctype)
+
+
+_INT_RANGE = {np.int32: (-2 ** 31, 2 ** 31 - 1), np.int64: (-2 ** 63, 2 ** 63 - 1)}
+
+
+def _parse_values(vals, np_dtype):
+    """each value read through its PType's text type (types_traits.hpp:118-243): uint8 /
+    uint16 as unsigned int (the C++ stream wraps a minus sign modulo 2^32), then cast; int32 /
+    int64 as themselves (out of range is a read failure); float32 / float64 as floats"""
+    try:
+        if np_dtype in (np.uint8, np.uint16):
+            ints = []
+            for v in vals:
+                x = int(v, 10)
+                if abs(x) > 2 ** 32 - 1:
+                    raise ValueError(v)
+                ints.append(x % 2 ** 32)
+            return np.array(ints, dtype=np.uint64).astype(np_dtype)
+        if np_dtype in (np.int32, np.int64):
+            lo, hi = _INT_RANGE[np_dtype]
+            ints = [int(v, 10) for v in vals]
+            if any(x < lo or x > hi for x in ints):
+                raise ValueError("out of range")
+            return np.array(ints, dtype=np_dtype)
+        return np.array([float(v) for v in vals], dtype=np.float64).astype(np_dtype)
+    except ValueError as e:
+        raise PoutreError(-1, f"ImageFromString: unable to read current pixel ({e})") from None
 
 
 def from_string(text, device="cuda"):
-    """poutre::ImageFromString (python/base/image_interface.cpp:29): "Scalar <PType> <rank>
-    <dims...> <values...>", values read as numbers and converted to the pixel type (trailing
-    values are ignored)."""
+    """poutre::ImageFromString (python/base/image_interface.cpp:29; image_interface.cpp:616-690):
+    "<Scalar|3Planes|4Planes> <PType> <rank> <dims...> <values...>", rank 1..4, the values of a
+    compound pixel consecutive; trailing values are ignored."""
     tok = text.split()
-    if len(tok) < 3 or tok[0] != "Scalar":
-        raise PoutreError(-1, "ImageFromString: only Scalar images are supported")
+    if len(tok) < 3:
+        raise PoutreError(-1, "ImageFromString: unable to read the header")
+    ctypes_by_name = {"Scalar": CompoundType.Scalar, "3Planes": CompoundType["3Planes"],
+                      "4Planes": CompoundType["4Planes"]}
+    if tok[0] not in ctypes_by_name:
+        raise PoutreError(-1, "Unable to read CompoundType from stream")
+    ctype = ctypes_by_name[tok[0]]
     names = {v[0]: k for k, v in _PT_INFO.items()}
     if tok[1] not in names:
-        raise PoutreError(-2, f"ImageFromString: unknown PType {tok[1]}")
-    pt, rank = names[tok[1]], int(tok[2])
-    if rank < 1 or rank > 3:
-        raise PoutreError(-4, f"Unsupported number of dims {rank}")
+        raise PoutreError(-2, "Unable to read PType from stream")
+    pt = names[tok[1]]
+    try:
+        rank = int(tok[2])
+    except ValueError:
+        raise PoutreError(-1, "ImageFromString: unable to read NumDims") from None
+    if rank < 1:
+        raise PoutreError(-4, "ImageFromString: wrong value from NumDims")
+    if rank > 4:
+        raise PoutreError(-4, "Unsupported number of dims")
     dims = [int(v) for v in tok[3:3 + rank]]
-    n = int(np.prod(dims))
+    if len(dims) != rank:
+        raise PoutreError(-1, "ImageFromString: unable to read dim")
+    planes = _PLANES[ctype]
+    n = int(np.prod(dims)) * planes
     vals = tok[3 + rank:3 + rank + n]
-    if len(dims) != rank or len(vals) < n:
-        raise PoutreError(-1, "ImageFromString: not enough values")
-    arr = np.array([float(v) for v in vals], dtype=np.float64).astype(_PT_INFO[pt][1]).reshape(dims)
-    return _torch().from_numpy(arr).to(device)
+    if len(vals) < n:
+        raise PoutreError(-1, "ImageFromString: unable to read current pixel")
+    arr = _parse_values(vals, _PT_INFO[pt][1]).reshape(dims + ([planes] if planes > 1 else []))
+    return _tag(_torch().from_numpy(arr).to(device), ctype)
 
 
-def to_string(img):
-    """poutre::ImageToString (python/base/image_interface.cpp:30): the from_string format;
-    floats printed with 6 significant digits (the C++ stream default)."""
+def to_string(img, ctype=None):
+    """poutre::ImageToString (python/base/image_interface.cpp:30; image_interface.cpp:879-909):
+    the from_string format with single spaces, integers as integers, floats in the C++ stream
+    default (6 significant digits).  `ctype` defaults to the image's own (Scalar for a plain
+    tensor); for 3Planes / 4Planes the last axis holds the planes."""
     t = _img(img)
+    ctype = CompoundType(ctype if ctype is not None else image_ctype(img))
     pt = PType(_ptype(t))
+    shape = list(t.shape)
+    if ctype != CompoundType.Scalar:
+        if not shape or shape[-1] != _PLANES[ctype]:
+            raise PoutreError(-1, f"to_string: a {ctype.name} image needs a last axis of {_PLANES[ctype]} planes")
+        shape = shape[:-1]
     a = t.detach().cpu().numpy().reshape(-1)
     if a.dtype.kind == "f":
         vals = ["%g" % float(v) for v in a]
     else:
         vals = [str(int(v)) for v in a]
-    return " ".join(["Scalar", _PT_INFO[pt][0], str(t.dim())] + [str(d) for d in t.shape] + vals)
+    return " ".join([ctype.name, _PT_INFO[pt][0], str(len(shape))] + [str(d) for d in shape] + vals)
 
 
 def convert_into(img, out):

@@ -20,6 +20,9 @@ LIB = os.path.join(HERE, "libpoutre_amd.so")
 # C++ API test program (tests/cpp/api_kats.cpp), linked against the library
 API_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "api_kats.cpp")
 API_TEST_BIN = os.path.join(HERE, "api_kats")
+# host-only C++ API checks (tests/cpp/host_kats.cpp): fixture format and SE model, no GPU needed
+HOST_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "host_kats.cpp")
+HOST_TEST_BIN = os.path.join(HERE, "host_kats")
 ARCH = "gfx950"
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -55,12 +58,14 @@ def build(verbose=False, jobs=None):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
-    if os.path.exists(API_TEST_SRC):
-        cmd = [HIPCC, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), API_TEST_SRC, "-o", API_TEST_BIN,
+    for src, out in ((API_TEST_SRC, API_TEST_BIN), (HOST_TEST_SRC, HOST_TEST_BIN)):
+        if not os.path.exists(src):
+            continue
+        cmd = [HIPCC, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), src, "-o", out,
                "-L", HERE, "-lpoutre_amd", "-Wl,-rpath,$ORIGIN"]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
-            raise RuntimeError(f"api test build failed:\n{r.stdout}\n{r.stderr}")
+            raise RuntimeError(f"C++ test program build failed ({os.path.basename(src)}):\n{r.stdout}\n{r.stderr}")
     if verbose:
         print(f"built {LIB} from {len(srcs)} sources")
     return LIB

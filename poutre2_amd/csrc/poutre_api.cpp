@@ -92,13 +92,25 @@ static std::string describe(const char *kind, const std::vector<std::size_t> &s,
   return os.str();
 }
 
-HostImage::HostImage(std::vector<std::size_t> shape, PType ptype)
-    : m_shape(std::move(shape)), m_ptype(ptype), m_data(count(m_shape) * SizeOf(ptype))
+static std::size_t planes_of(CompoundType c)
 {
-  if (m_shape.empty() || m_shape.size() > 3) fail("Unsupported number of dims");
+  switch (c) {
+  case CompoundType::CompoundType_Scalar: return 1;
+  case CompoundType::CompoundType_3Planes: return 3;
+  case CompoundType::CompoundType_4Planes: return 4;
+  default: fail("Unsupported compound type");
+  }
+}
+
+HostImage::HostImage(std::vector<std::size_t> shape, PType ptype, CompoundType ctype)
+    : m_shape(std::move(shape)), m_ptype(ptype), m_ctype(ctype),
+      m_data(count(m_shape) * planes_of(ctype) * SizeOf(ptype))
+{
+  if (m_shape.empty() || m_shape.size() > 4) fail("Unsupported number of dims");
 }
 std::string HostImage::str() const { return describe("HostImage", m_shape, m_ptype); }
 std::size_t HostImage::size() const { return count(m_shape); }
+std::size_t HostImage::planes() const { return planes_of(m_ctype); }
 
 DeviceImage::DeviceImage(std::vector<std::size_t> shape, PType ptype) : m_shape(std::move(shape)), m_ptype(ptype)
 {
@@ -116,7 +128,7 @@ static void copy_any(const IInterface &src, const IInterface &dst)
 {
   AssertSizesCompatible(src, dst, "CopyInto images have not compatible sizes");
   AssertAsTypesCompatible(src, dst, "CopyInto images must have compatible types");
-  const std::size_t bytes = count(src.GetShape()) * SizeOf(src.GetPType());
+  const std::size_t bytes = count(src.GetShape()) * planes_of(src.GetCType()) * SizeOf(src.GetPType());
   if (bytes == 0) return;
   hipMemcpyKind kind = src.OnDevice() ? (dst.OnDevice() ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost)
                                       : (dst.OnDevice() ? hipMemcpyHostToDevice : hipMemcpyHostToHost);
@@ -128,8 +140,9 @@ void CopyInto(const IInterface &i_src, IInterface &o_dst) { copy_any(i_src, o_ds
 
 std::unique_ptr<IInterface> Create(const std::vector<std::size_t> &dims, CompoundType ctype, PType ptype)
 {
-  if (ctype != CompoundType::CompoundType_Scalar) fail("Create: only scalar images are supported");
-  return std::make_unique<HostImage>(dims, ptype);
+  (void)planes_of(ctype);  // Scalar, 3Planes or 4Planes
+  (void)SizeOf(ptype);
+  return std::make_unique<HostImage>(dims, ptype, ctype);
 }
 std::unique_ptr<IInterface> CreateDevice(const std::vector<std::size_t> &dims, PType ptype)
 {
@@ -142,6 +155,9 @@ std::unique_ptr<IInterface> CloneGeometry(const IInterface &i_img)
 }
 
 // ------------------------------------------------------------------ fixture format
+// ImageFromString / ImageToString (src/base/image_interface.cpp:616-909, type names
+// src/base/types.cpp:201-345): values go through each PType's text type (TypeTraits::str_type,
+// types_traits.hpp:118-243) and the default stream format.
 static PType ptype_from(const std::string &s)
 {
   if (s == "GUINT8") return PType::PType_GrayUINT8;
@@ -150,51 +166,78 @@ static PType ptype_from(const std::string &s)
   if (s == "GINT64") return PType::PType_GrayINT64;
   if (s == "F32") return PType::PType_F32;
   if (s == "D64") return PType::PType_D64;
-  fail("ImageFromString: unknown PType " + s);
+  fail("Unable to read PType from stream");
+}
+static CompoundType ctype_from(const std::string &s)
+{
+  if (s == "Scalar") return CompoundType::CompoundType_Scalar;
+  if (s == "3Planes") return CompoundType::CompoundType_3Planes;
+  if (s == "4Planes") return CompoundType::CompoundType_4Planes;
+  fail("Unable to read CompoundType from stream");
+}
+static const char *ctype_name(CompoundType c)
+{
+  switch (c) {
+  case CompoundType::CompoundType_Scalar: return "Scalar";
+  case CompoundType::CompoundType_3Planes: return "3Planes";
+  case CompoundType::CompoundType_4Planes: return "4Planes";
+  default: return "Unknown PType";
+  }
 }
 
-template <typename T> static void fill_from(std::istringstream &is, void *data, std::size_t n)
+// T: storage type, S: the text type values are read as and written as
+template <typename T, typename S> static void read_values(std::istringstream &is, void *data, std::size_t n,
+                                                          const std::string &src)
 {
   T *p = static_cast<T *>(data);
   for (std::size_t i = 0; i < n; i++) {
-    double v;
-    if (!(is >> v)) fail("ImageFromString: not enough values");
+    S v{};
+    is >> v;
+    if (is.fail()) fail("ImageFromStringDenseScalarDispatchPType: unable to read current pixel from " + src);
     p[i] = static_cast<T>(v);
   }
+}
+template <typename T, typename S> static void write_values(std::ostringstream &os, const void *data, std::size_t n)
+{
+  const T *p = static_cast<const T *>(data);
+  for (std::size_t i = 0; i < n; i++) os << static_cast<S>(p[i]) << ' ';
 }
 
 std::unique_ptr<IInterface> ImageFromString(const std::string &str)
 {
   std::istringstream is(str);
-  std::string ctype, pt;
-  std::size_t rank = 0;
-  is >> ctype >> pt >> rank;
-  if (ctype != "Scalar") fail("ImageFromString: only Scalar images are supported");
-  std::vector<std::size_t> shape(rank);
-  for (auto &d : shape) is >> d;
-  const PType p = ptype_from(pt);
-  auto img = Create(shape, CompoundType::CompoundType_Scalar, p);
-  const std::size_t n = count(shape);
+  if (!is.good()) fail("ImageFromString: unable to read input string");
+  std::string cs, ps;
+  is >> cs;
+  if (is.fail()) fail("ImageFromString: unable to read CompoundType from " + str);
+  const CompoundType c = ctype_from(cs);
+  is >> ps;
+  if (is.fail()) fail("ImageFromString: unable to read PType from " + str);
+  const PType p = ptype_from(ps);
+  std::ptrdiff_t rank = -1;
+  is >> rank;
+  if (is.fail()) fail("ImageFromString: unable to read NumDims from " + str);
+  if (!(rank > 0)) fail("ImageFromString: wrong value from NumDims " + str);
+  std::vector<std::size_t> shape;
+  for (std::ptrdiff_t i = 0; i < rank; i++) {
+    std::ptrdiff_t d = 0;
+    is >> d;
+    if (is.fail()) fail("ImageFromString: unable to read dim from " + str);
+    shape.push_back(static_cast<std::size_t>(d));
+  }
+  if (rank > 4) fail("Unsupported number of dims");
+  auto img = Create(shape, c, p);
+  const std::size_t n = count(shape) * planes_of(c);
   switch (p) {
-  case PType::PType_GrayUINT8: fill_from<pUINT8>(is, img->Data(), n); break;
-  case PType::PType_GrayUINT16: fill_from<pUINT16>(is, img->Data(), n); break;
-  case PType::PType_GrayINT32: fill_from<pINT32>(is, img->Data(), n); break;
-  case PType::PType_GrayINT64: fill_from<pINT64>(is, img->Data(), n); break;
-  case PType::PType_F32: fill_from<pFLOAT>(is, img->Data(), n); break;
-  case PType::PType_D64: fill_from<pDOUBLE>(is, img->Data(), n); break;
-  default: break;
+  case PType::PType_GrayUINT8: read_values<pUINT8, unsigned int>(is, img->Data(), n, str); break;
+  case PType::PType_GrayUINT16: read_values<pUINT16, unsigned int>(is, img->Data(), n, str); break;
+  case PType::PType_GrayINT32: read_values<pINT32, pINT32>(is, img->Data(), n, str); break;
+  case PType::PType_GrayINT64: read_values<pINT64, pINT64>(is, img->Data(), n, str); break;
+  case PType::PType_F32: read_values<pFLOAT, pFLOAT>(is, img->Data(), n, str); break;
+  case PType::PType_D64: read_values<pDOUBLE, pDOUBLE>(is, img->Data(), n, str); break;
+  default: fail("ImageFromStringDenseScalarDispatchPType::Unsupported ptype");
   }
   return img;  // trailing values, if any, are ignored as by the reference's parser
-}
-
-template <typename T> static void print_to(std::ostringstream &os, const void *data, std::size_t n)
-{
-  const T *p = static_cast<const T *>(data);
-  for (std::size_t i = 0; i < n; i++) {
-    os << ' ';
-    if constexpr (std::is_floating_point<T>::value) os << p[i];
-    else os << static_cast<long long>(p[i]);
-  }
 }
 
 std::string ImageToString(const IInterface &i_img)
@@ -202,125 +245,50 @@ std::string ImageToString(const IInterface &i_img)
   std::unique_ptr<IInterface> host;
   const IInterface *src = &i_img;
   if (i_img.OnDevice()) {
-    host = Create(i_img.GetShape(), CompoundType::CompoundType_Scalar, i_img.GetPType());
+    host = Create(i_img.GetShape(), i_img.GetCType(), i_img.GetPType());
     CopyInto(i_img, *host);
     src = host.get();
   }
+  if (src->GetRank() == 0) fail("Unsupported number of dims:0");
   std::ostringstream os;
-  os << "Scalar " << ptype_name(src->GetPType()) << ' ' << src->GetRank();
-  for (auto d : src->GetShape()) os << ' ' << d;
-  const std::size_t n = count(src->GetShape());
+  os << ctype_name(src->GetCType()) << ' ' << ptype_name(src->GetPType()) << ' ' << src->GetRank() << ' ';
+  for (auto d : src->GetShape()) os << d << ' ';
+  const std::size_t n = count(src->GetShape()) * planes_of(src->GetCType());
   switch (src->GetPType()) {
-  case PType::PType_GrayUINT8: print_to<pUINT8>(os, src->Data(), n); break;
-  case PType::PType_GrayUINT16: print_to<pUINT16>(os, src->Data(), n); break;
-  case PType::PType_GrayINT32: print_to<pINT32>(os, src->Data(), n); break;
-  case PType::PType_GrayINT64: print_to<pINT64>(os, src->Data(), n); break;
-  case PType::PType_F32: print_to<pFLOAT>(os, src->Data(), n); break;
-  case PType::PType_D64: print_to<pDOUBLE>(os, src->Data(), n); break;
-  default: break;
+  case PType::PType_GrayUINT8: write_values<pUINT8, unsigned int>(os, src->Data(), n); break;
+  case PType::PType_GrayUINT16: write_values<pUINT16, unsigned int>(os, src->Data(), n); break;
+  case PType::PType_GrayINT32: write_values<pINT32, pINT32>(os, src->Data(), n); break;
+  case PType::PType_GrayINT64: write_values<pINT64, pINT64>(os, src->Data(), n); break;
+  case PType::PType_F32: write_values<pFLOAT, pFLOAT>(os, src->Data(), n); break;
+  case PType::PType_D64: write_values<pDOUBLE, pDOUBLE>(os, src->Data(), n); break;
+  default: fail("ImageToStringDenseScalarDispatchPType::Unsupported pixel type");
   }
-  return os.str();
+  std::string out = os.str();
+  out.pop_back();  // the separator after the last field
+  return out;
 }
 
 // ------------------------------------------------------------------ structuring elements
 namespace se {
 
-neighbor_list::neighbor_list(std::size_t rank, std::vector<coord> coords) : m_rank(rank), m_coords(std::move(coords))
-{
-  for (const auto &c : m_coords)
-    if (c.size() != rank) fail("neighbor_list: coordinate of wrong rank");
-}
-neighbor_list neighbor_list::transpose() const
-{
-  std::vector<coord> t = m_coords;
-  for (auto &c : t)
-    for (auto &v : c) v = -v;
-  return neighbor_list(m_rank, t);
-}
-neighbor_list neighbor_list::remove_center() const
-{
-  std::vector<coord> t;
-  for (const auto &c : m_coords)
-    if (std::any_of(c.begin(), c.end(), [](std::ptrdiff_t v) { return v != 0; })) t.push_back(c);
-  return neighbor_list(m_rank, t);
-}
-bool neighbor_list::is_symmetric() const { return transpose().is_equal_unordered(*this); }
-std::pair<neighbor_list, neighbor_list> neighbor_list::split_upper_lower() const
-{
-  std::vector<coord> up, lo;
-  for (const auto &c : m_coords) {
-    int s = 0;
-    for (auto v : c)
-      if (v != 0) { s = v < 0 ? -1 : 1; break; }
-    if (s < 0) up.push_back(c);
-    else if (s > 0) lo.push_back(c);
-  }
-  return {neighbor_list(m_rank, up), neighbor_list(m_rank, lo)};
-}
-std::ptrdiff_t neighbor_list::maximum_extension(std::size_t dim, bool positive) const
-{
-  std::ptrdiff_t e = 0;
-  for (const auto &c : m_coords) e = std::max(e, positive ? c.at(dim) : -c.at(dim));
-  return e;
-}
-std::unique_ptr<IStructuringElement> neighbor_list::Transpose() const
-{ return std::make_unique<neighbor_list>(transpose()); }
-std::unique_ptr<IStructuringElement> neighbor_list::RemoveCenter() const
-{ return std::make_unique<neighbor_list>(remove_center()); }
-std::unique_ptr<IStructuringElement> neighbor_list::Clone() const { return std::make_unique<neighbor_list>(*this); }
-bool neighbor_list::is_equal(const IStructuringElement &o) const noexcept
-{
-  auto p = dynamic_cast<const neighbor_list *>(&o);
-  return p && p->m_rank == m_rank && p->m_coords == m_coords;
-}
-// the same SETS of offsets (neighbor_list_se_t.hpp:177-195: sorted, repeats removed)
-bool neighbor_list::is_equal_unordered(const IStructuringElement &o) const noexcept
-{
-  auto p = dynamic_cast<const neighbor_list *>(&o);
-  if (!p || p->m_rank != m_rank) return false;
-  auto a = m_coords, b = p->m_coords;
-  std::sort(a.begin(), a.end());
-  std::sort(b.begin(), b.end());
-  a.erase(std::unique(a.begin(), a.end()), a.end());
-  b.erase(std::unique(b.begin(), b.end()), b.end());
-  return a == b;
-}
-
-static neighbor_list make_box(std::size_t rank, bool cross)
-{
-  std::vector<neighbor_list::coord> c;
-  c.push_back(neighbor_list::coord(rank, 0));
-  const int n = rank == 2 ? 9 : 27;
-  for (int i = 0; i < n; i++) {
-    neighbor_list::coord p(rank);
-    int k = i, l1 = 0;
-    for (std::size_t a = 0; a < rank; a++) {
-      p[rank - 1 - a] = (k % 3) - 1;
-      l1 += std::abs(p[rank - 1 - a]);
-      k /= 3;
-    }
-    if (l1 == 0 || (cross && l1 > 1)) continue;
-    c.push_back(p);
-  }
-  return neighbor_list(rank, c);
-}
-static neighbor_list make_seg(std::size_t rank, std::size_t axis)
-{
-  std::vector<neighbor_list::coord> c(3, neighbor_list::coord(rank, 0));
-  c[1][axis] = -1;
-  c[2][axis] = 1;
-  return neighbor_list(rank, c);
-}
-const neighbor_list SESegmentX1D = make_seg(1, 0);
-const neighbor_list SESquare2D = make_box(2, false);
-const neighbor_list SECross2D = make_box(2, true);
-const neighbor_list SESegmentX2D = make_seg(2, 1);
-const neighbor_list SESegmentY2D = make_seg(2, 0);
-const neighbor_list SESegmentX3D = make_seg(3, 2);
-const neighbor_list SESegmentY3D = make_seg(3, 1);
-const neighbor_list SESegmentZ3D = make_seg(3, 0);
-const neighbor_list SECross3D = make_box(3, true);
-const neighbor_list SESquare3D = make_box(3, false);
+// predefined runtime SEs: offsets and their order as the reference lists them
+// (src/structuring_element/predefined_nl_se.cpp:19-91), {dy, dx} / {dz, dy, dx}
+const details::neighbor_list_t<1> SESegmentX1D{{0}, {-1}, {+1}};
+const details::neighbor_list_t<2> SESegmentX2D{{0, 0}, {0, -1}, {0, +1}};
+const details::neighbor_list_t<2> SESegmentY2D{{0, 0}, {-1, 0}, {+1, 0}};
+const details::neighbor_list_t<2> SESquare2D{{0, 0},   {-1, -1}, {+0, -1}, {+1, -1}, {-1, +0},
+                                             {+1, +0}, {-1, +1}, {+0, +1}, {+1, +1}};
+const details::neighbor_list_t<2> SECross2D{{0, 0}, {+0, -1}, {-1, +0}, {+1, +0}, {+0, +1}};
+const details::neighbor_list_t<3> SECross3D{{0, 0, 0},   {+0, +0, -1}, {+0, -1, +0}, {-1, +0, +0},
+                                            {+1, +0, +0}, {+0, +1, +0}, {+0, +0, +1}};
+const details::neighbor_list_t<3> SESquare3D{
+    {0, 0, 0},    {-1, -1, -1}, {+0, -1, -1}, {+1, -1, -1}, {-1, +0, -1}, {+0, +0, -1}, {+1, +0, -1},
+    {-1, +1, -1}, {+0, +1, -1}, {+1, +1, -1}, {-1, -1, +0}, {+0, -1, +0}, {+1, -1, +0}, {-1, +0, +0},
+    {+1, +0, +0}, {-1, +1, +0}, {+0, +1, +0}, {+1, +1, +0}, {-1, -1, +1}, {+0, -1, +1}, {+1, -1, +1},
+    {-1, +0, +1}, {+0, +0, +1}, {+1, +0, +1}, {-1, +1, +1}, {+0, +1, +1}, {+1, +1, +1}};
+const details::neighbor_list_t<3> SESegmentX3D{{0, 0, 0}, {0, 0, -1}, {0, 0, +1}};
+const details::neighbor_list_t<3> SESegmentY3D{{0, 0, 0}, {0, -1, 0}, {0, +1, 0}};
+const details::neighbor_list_t<3> SESegmentZ3D{{0, 0, 0}, {-1, 0, 0}, {+1, 0, 0}};
 
 }  // namespace se
 
@@ -333,6 +301,8 @@ class Staged
 public:
   Staged(const IInterface &img, bool upload) : m_img(img)
   {
+    if (img.GetCType() != CompoundType::CompoundType_Scalar) fail("Unsupported compound type");
+    if (img.GetRank() == 0 || img.GetRank() > 3) fail("Unsupported number of dims");
     m_desc.ptype = (int32_t)img.GetPType();
     m_desc.rank = (int32_t)img.GetRank();
     const auto s = img.GetShape();
@@ -547,27 +517,43 @@ void Dilate(const IInterface &i, se::Compound_NL_SE s, const int size, IInterfac
   unary(i, o, "Dilate compound", [&](pm_image *a, pm_image *b) { return pm_dilate_compound(a, (int)s, size, b, nullptr); });
 }
 
-static std::vector<int64_t> nl_offsets(const se::IStructuringElement &s, std::size_t rank, int &n)
+// the offsets of a runtime SE whose rank is the image's (ero_dil.cpp:59-243: a
+// neighbor_list_t of the image's rank, anything else is refused)
+template <std::ptrdiff_t R> static bool nl_of_rank(const se::IStructuringElement &s, std::vector<int64_t> &offs, int &n)
 {
-  auto nl = dynamic_cast<const se::neighbor_list *>(&s);
-  if (!nl) fail("Unsupported structuring element type (neighbor_list only)");
-  if (nl->rank() != rank) fail("SE and image have not the same rank");
-  std::vector<int64_t> offs;
+  const auto *nl = dynamic_cast<const se::details::neighbor_list_t<R> *>(&s);
+  if (!nl) return false;
   for (const auto &c : nl->get_coordinates())
-    for (auto v : c) offs.push_back((int64_t)v);
-  n = (int)nl->GetSize();
+    for (std::size_t k = 0; k < (std::size_t)R; k++) offs.push_back((int64_t)c[k]);
+  n = (int)nl->size();
+  return true;
+}
+static std::vector<int64_t> nl_offsets(const se::IStructuringElement &s, std::size_t rank, int &n, const char *what)
+{
+  std::vector<int64_t> offs;
+  bool ok = false;
+  switch (rank) {
+  case 0: fail(std::string(what) + " Unsupported number of dims:0");
+  case 1: ok = nl_of_rank<1>(s, offs, n); break;
+  case 2: ok = nl_of_rank<2>(s, offs, n); break;
+  case 3: ok = nl_of_rank<3>(s, offs, n); break;
+  default: fail(std::string(what) + " Unsupported number of dims");
+  }
+  if (!ok) fail(std::string(what) + " Unsupported IStructuringElement type");
   return offs;
 }
 void Erode(const IInterface &i, const se::IStructuringElement &s, IInterface &o)
 {
+  checks(i, o, "Erode");  // the reference's checks come before the rank / SE dispatch
   int n = 0;
-  auto offs = nl_offsets(s, i.GetRank(), n);
+  auto offs = nl_offsets(s, i.GetRank(), n, "Erode");
   unary(i, o, "Erode", [&](pm_image *a, pm_image *b) { return pm_erode_nl(a, offs.data(), n, b, nullptr); });
 }
 void Dilate(const IInterface &i, const se::IStructuringElement &s, IInterface &o)
 {
+  checks(i, o, "Dilate");  // the reference's checks come before the rank / SE dispatch
   int n = 0;
-  auto offs = nl_offsets(s, i.GetRank(), n);
+  auto offs = nl_offsets(s, i.GetRank(), n, "Dilate");
   unary(i, o, "Dilate", [&](pm_image *a, pm_image *b) { return pm_dilate_nl(a, offs.data(), n, b, nullptr); });
 }
 

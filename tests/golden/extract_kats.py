@@ -23,6 +23,12 @@ Sources read (relative to the reference root):
 and, into ``compare_kats.json``, the whole-image cases of
   test/pixel_processing/compare.cpp                (CompareImage, every operand form)
 
+and, into ``iostring_kats.json``, the fixture-format cases of
+  test/base/image_iostring.cpp                     (ImageFromString / ImageToString: the input
+                                                    or expected string, the pixel type, compound
+                                                    type, shape, and the pixel values or the
+                                                    fill value)
+
 Usage: python tests/golden/extract_kats.py [/root/reference]
 """
 import json
@@ -121,7 +127,7 @@ def extract(ref_root):
                 for d in exp["shape"]:
                     n *= d
                 case = {"file": rel, "name": name, "sub": sub, "line": line,
-                        "inputs": inputs, "expected": exp,
+                        "inputs": inputs, "expected": exp, "expected_text": txt,
                         "expected_complete": exp["n_listed"] >= n}
                 case.update(call or {})
                 if h is not None:
@@ -177,6 +183,46 @@ def extract_compare(ref_root):
     return cases
 
 
+_PT = {"GrayUINT8": "GUINT8", "GrayINT32": "GINT32", "GrayINT64": "GINT64", "F32": "F32", "D64": "D64"}
+_CT_OF_STORAGE = {"pUINT8": "GUINT8", "pINT32": "GINT32", "pINT64": "GINT64", "pFLOAT": "F32", "pDOUBLE": "D64"}
+_NUM = re.compile(r"-?\d+(?:\.\d*)?(?:[eE][-+]?\d+)?")
+
+
+def _nums(text):
+    return [float(t) if ("." in t or "e" in t or "E" in t) else int(t) for t in _NUM.findall(text)]
+
+
+def extract_iostring(ref_root):
+    rel = "test/base/image_iostring.cpp"
+    src = open(os.path.join(ref_root, rel)).read()
+    heads = list(re.finditer(r'TEST_CASE\("([^"]*)"[^)]*\)', src))
+    cases = []
+    for i, h in enumerate(heads):
+        body = src[h.end():heads[i + 1].start() if i + 1 < len(heads) else len(src)]
+        line = src[:h.start()].count("\n") + 1
+        strings = [_cooked(m.group(0)) for m in re.finditer(r'"(?:[^"\\]|\\.|\\\n)*"', body)]
+        ptype = _PT[re.search(r"PType::PType_(\w+)", body).group(1)]
+        ctype = re.search(r"CompoundType::CompoundType_(\w+)", body).group(1)
+        case = {"file": rel, "line": line, "name": h.group(1), "ptype": ptype, "compound": ctype}
+        if "ImageFromString" in body:
+            case["kind"] = "read"
+            case["text"] = strings[0]
+            case["rank"] = int(re.search(r"GetRank\(\)\s*==\s*(\d+)", body).group(1))
+            case["shape"] = _nums(re.search(r"expectedcoords\s*=\s*\{([^}]*)\}", body).group(1))
+            case["values"] = _nums(re.search(r"expectedvalues\s*=\s*\{(.*?)\};", body, re.S).group(1))
+        else:
+            case["kind"] = "write"
+            m = re.search(r"image_t<(.*?)>\s+img\(\{([^}]*)\}\)", body)
+            case["shape"] = _nums(m.group(2))
+            fill = re.search(r"img\.fill\((.*?)\);", body).group(1)
+            if re.fullmatch(r"\s*\w+\s*", fill) and not _NUM.fullmatch(fill.strip()):
+                fill = re.search(r"\b" + fill.strip() + r"\s*=\s*([^;]+);", body).group(1)
+            case["fill"] = _nums(fill)
+            case["expected"] = strings[0]
+        cases.append(case)
+    return cases
+
+
 def main():
     ref_root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     cases = extract(ref_root)
@@ -186,7 +232,10 @@ def main():
     cmp = extract_compare(ref_root)
     with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "compare_kats.json"), "w") as f:
         json.dump({"source": "ThomasRetornaz/poutre2 inline KATs (compare.cpp)", "cases": cmp}, f, indent=1)
-    print(f"wrote {len(cases)} cases to {out}")
+    ios = extract_iostring(ref_root)
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "iostring_kats.json"), "w") as f:
+        json.dump({"source": "ThomasRetornaz/poutre2 inline KATs (image_iostring.cpp)", "cases": ios}, f, indent=1)
+    print(f"wrote {len(cases)} cases to {out}, {len(cmp)} compare cases, {len(ios)} iostring cases")
 
 
 if __name__ == "__main__":

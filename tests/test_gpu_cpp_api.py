@@ -35,9 +35,11 @@ def test_cpp_api_kats(tmp_path):
         if op in ("t_Dilate", "t_Erode") and c["runtime_se"]:
             op = "rt_" + op[2:]
         ins = c["inputs"]
+        # the 9th field is the reference's expected string, compared raw as its Catch2 test does
+        # (REQUIRE_THAT(ImageToString(out), Equals(expected)))
         lines.append("|".join([op, c.get("se") or "", c.get("rec_type") or "", str(c.get("r") or ""),
                                str(c.get("h") or ""), _img_str(ins[0]), _img_str(ins[1]) if len(ins) > 1 else "",
-                               _img_str(c["expected"])]))
+                               _img_str(c["expected"]), c["expected_text"]]))
     # CompareImage: every whole-image KAT of the reference (test/pixel_processing/compare.cpp)
     for c in COMPARE_KATS:
         vals = ",".join(str(o.get("value", 0)) for o in (c["comp"], c["true"], c["false"]))

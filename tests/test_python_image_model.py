@@ -49,19 +49,54 @@ def test_to_string_float_format():
 
 
 def test_from_string_conversions_and_errors():
-    # values are read as numbers, then converted to the pixel type (truncation toward zero)
-    t = pm.from_string("Scalar GINT32 1 3 1.9 -1.9 7", device="cpu")
-    assert t.tolist() == [1, -1, 7]
+    # values go through the PType's text type: "1.9" is not an int (the reference's stream read
+    # stops at '.', and the next value read fails), uint8 reads an unsigned int and casts
+    with pytest.raises(pm.PoutreError):
+        pm.from_string("Scalar GINT32 1 3 1.9 -1.9 7", device="cpu")
+    assert pm.from_string("Scalar GUINT8 1 3 -1 300 255", device="cpu").tolist() == [255, 44, 255]
+    with pytest.raises(pm.PoutreError):
+        pm.from_string("Scalar GINT32 1 1 2147483648", device="cpu")
+    big = "Scalar GINT64 1 3 9223372036854775807 -9223372036854775808 -1"
+    assert pm.to_string(pm.from_string(big, device="cpu")) == big
     # trailing values are ignored
     assert pm.from_string("Scalar GUINT8 1 2 3 4 5", device="cpu").tolist() == [3, 4]
     with pytest.raises(pm.PoutreError):
-        pm.from_string("3Planes GUINT8 1 2 3 4", device="cpu")
+        pm.from_string("Vector GUINT8 1 2 3 4", device="cpu")
     with pytest.raises(pm.PoutreError):
         pm.from_string("Scalar GFOO 1 2 3 4", device="cpu")
     with pytest.raises(pm.PoutreError):
         pm.from_string("Scalar GUINT8 2 2 2 1 2 3", device="cpu")
     with pytest.raises(pm.PoutreError):
-        pm.from_string("Scalar GUINT8 4 1 1 1 1 0", device="cpu")
+        pm.from_string("Scalar GUINT8 0", device="cpu")
+    # rank 4 images exist (the operators take ranks 1..3)
+    r4 = pm.from_string("Scalar GUINT8 4 1 2 1 2 0 255 7 8", device="cpu")
+    assert r4.shape == (1, 2, 1, 2) and pm.to_string(r4) == "Scalar GUINT8 4 1 2 1 2 0 255 7 8"
+
+
+def _iostring_kats():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "iostring_kats.json")) as f:
+        return json.load(f)["cases"]
+
+
+@pytest.mark.parametrize("c", _iostring_kats(), ids=lambda c: c["name"])
+def test_iostring_kats(c):
+    """test/base/image_iostring.cpp:28-242 through the Python names (from_string / to_string)"""
+    if c["kind"] == "read":
+        t = pm.from_string(c["text"], device="cpu")
+        assert pm.image_ctype(t).name == c["compound"]
+        planes = {"Scalar": 1, "3Planes": 3, "4Planes": 4}[c["compound"]]
+        shape = list(t.shape[:-1]) if planes > 1 else list(t.shape)
+        assert shape == c["shape"] and len(shape) == c["rank"]
+        assert t.reshape(-1).tolist() == c["values"]
+    else:
+        dt = {"GUINT8": torch.uint8, "F32": torch.float32, "D64": torch.float64}[c["ptype"]]
+        planes = {"Scalar": 1, "3Planes": 3, "4Planes": 4}[c["compound"]]
+        shape = tuple(c["shape"]) + ((planes,) if planes > 1 else ())
+        t = torch.empty(shape, dtype=dt)
+        t[...] = torch.tensor(c["fill"], dtype=dt)
+        assert pm.to_string(t, pm.CompoundType[c["compound"]]) == c["expected"]
 
 
 def test_factory_image_and_predicates():
@@ -72,10 +107,13 @@ def test_factory_image_and_predicates():
     assert pm.as_sizes_compatible(a, b) and not pm.as_sizes_compatible(a, c)
     assert pm.as_types_compatible(a, c) and not pm.as_types_compatible(a, b)
     assert pm.are_images_different(a, c) and not pm.are_images_different(a, a)
+    c3 = pm.factory_image((3, 4), pm.CompoundType["3Planes"], pm.PType.GrayUINT8, device="cpu")
+    assert c3.shape == (3, 4, 3) and pm.image_ctype(c3) == pm.CompoundType["3Planes"]
+    assert pm.to_string(c3).startswith("3Planes GUINT8 2 3 4 0 0 0")
+    with pytest.raises(pm.PoutreError):  # the operators take Scalar images only
+        pm.dilate(c3, pm.static_nl_se.square2d, 1)
     with pytest.raises(pm.PoutreError):
-        pm.factory_image((3, 4), pm.CompoundType["3Planes"], pm.PType.GrayUINT8, device="cpu")
-    with pytest.raises(pm.PoutreError):
-        pm.factory_image((1, 2, 3, 4), pm.CompoundType.Scalar, pm.PType.GrayUINT8, device="cpu")
+        pm.factory_image((1, 2, 3, 4, 5), pm.CompoundType.Scalar, pm.PType.GrayUINT8, device="cpu")
     assert [int(p) for p in pm.PType] == [4, 8, 16, 32, 64, 128]
 
 
@@ -92,13 +130,16 @@ def test_dlpack_producers_are_images():
 def test_se_object():
     """pypoutre's SE methods (python/structuring_element/se_interface.cpp:15-23) on a neighbour
     list; unordered equality compares sets (neighbor_list_se_t.hpp:177-195)"""
-    sq = pm.SE([(dy, dx) for dy in (-1, 0, 1) for dx in (-1, 0, 1)])
+    # the centre is the first offset, as in every predefined SE (predefined_nl_se.cpp:19-91)
+    sq = pm.SE([(0, 0)] + [(dy, dx) for dy in (-1, 0, 1) for dx in (-1, 0, 1) if (dy, dx) != (0, 0)])
     assert sq.size() == 9 and sq.type() == pm.se_type.neighbor_list
     rc = sq.remove_center()
     assert rc.size() == 8 and (0, 0) not in list(rc)
-    assert rc.is_equal_unordered(rc.transpose()) and not rc.is_equal(rc.transpose())
-    seg = pm.SE([(0, -1), (0, 0), (0, 1), (0, 2)])
-    assert list(seg.transpose()) == [(0, 1), (0, 0), (0, -1), (0, -2)]
+    assert sq.is_equal_unordered(sq.transpose()) and not sq.is_equal(sq.transpose())
+    seg = pm.SE([(0, 0), (0, -1), (0, 1), (0, 2)])
+    assert list(seg.transpose()) == [(0, 0), (0, 1), (0, -1), (0, -2)]
+    # mirrored about the first offset when that is not the origin (neighbor_list_se_t.hpp:96-107)
+    assert list(pm.SE([(1, 1), (0, 1)]).transpose()) == [(1, 1), (2, 1)]
     assert not seg.is_equal_unordered(seg.transpose())
     assert pm.SE([(0, 0), (0, 0), (-1, 0)]).is_equal_unordered(pm.SE([(-1, 0), (0, 0)]))
     assert seg.clone().is_equal(seg) and not seg.is_equal([(0, -1), (0, 0), (0, 1), (0, 2)])
