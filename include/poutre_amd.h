@@ -127,6 +127,16 @@ int pm_gradient(const pm_image *in, int se, int iter, pm_image *out, void *strea
  * src/geodesy/mreconstruct.cpp:48-132 (mreconstruct_t.hpp:75-512). */
 int pm_reconstruct(int rec_type, const pm_image *marker, const pm_image *mask, int se, pm_image *out,
                    void *stream);
+/* ---- testing hooks (no reference counterpart): force the error paths from the host, so that
+ * tests reach them with every kernel exiting normally.
+ * pm_debug_set_max_rounds(n): reconstructions (and the geodesy built on them) called afterwards
+ *   stop with PM_E_NOCONV after n global rounds when not converged; 0 restores the default caps.
+ * pm_debug_fail_allocations(n): the next n scratch allocations fail with PM_E_HIP, as on an
+ *   out-of-memory device, without calling the allocator; 0 clears.
+ * Both settings are process-wide. */
+int pm_debug_set_max_rounds(int n);
+int pm_debug_fail_allocations(int n);
+
 /* pm_reconstruct for uint8 1D/2D images and 3x3-class SEs (square, cross, segments) that
  * allocates nothing.  No counterpart in the reference: it is for callers that cannot allocate
  * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Its

@@ -543,10 +543,18 @@ def leveling(ref, marker, se, out=None):
     return _lev("pm_leveling", ref, marker, se, out)
 
 
+def _scalar(v):
+    """a scalar operand as the double the C ABI carries: integers beyond 2^53 would be rounded,
+    so they are refused (the library then checks that the value fits the pixel type)"""
+    if isinstance(v, (int, np.integer)) and abs(int(v)) > 2 ** 53:
+        raise PoutreError(-6, f"scalar operand {int(v)} is not exactly representable (|value| > 2^53)")
+    return float(v)
+
+
 def _ext(name, img, value, se, out):
     out = _out(img, out)
     di, do = _desc(img), _desc(out)
-    _call(name, ctypes.byref(di), float(value), int(se), ctypes.byref(do), _stream(img))
+    _call(name, ctypes.byref(di), _scalar(value), int(se), ctypes.byref(do), _stream(img))
     return out
 
 
@@ -695,7 +703,7 @@ def arith_ssub_image(a, b, out=None):
 def _ewc(name, a, c, out):
     out = _out(a, out)
     da, do = _desc(a), _desc(out)
-    _call(name, ctypes.byref(da), float(c), ctypes.byref(do), _stream(a))
+    _call(name, ctypes.byref(da), _scalar(c), ctypes.byref(do), _stream(a))
     return out
 
 

@@ -61,6 +61,8 @@ SIGNATURES = {
     "pm_gradient": [_P, _I, _I, _P, _VP],
     "pm_reconstruct": [_I, _P, _P, _I, _P, _VP],
     "pm_reconstruct_noalloc": [_I, _P, _P, _I, _P, _VP],
+    "pm_debug_set_max_rounds": [_I],
+    "pm_debug_fail_allocations": [_I],
     "pm_low_leveling": [_P, _P, _I, _P, _VP],
     "pm_high_leveling": [_P, _P, _I, _P, _VP],
     "pm_leveling": [_P, _P, _I, _P, _VP],

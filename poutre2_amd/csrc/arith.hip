@@ -174,8 +174,11 @@ template <int OP> static int arithc(const pm_image *a, double c, pm_image *out, 
   PM_TRY(check_image(a, what));
   PM_TRY(check_pair(a, out, what));
   const int64_t n = numel(*a);
-  auto f = [a = a, c = c, o = out, n = n, s = as_stream(stream)](auto tag) -> int {
-      using T = typename decltype(tag)::type; return ewc<T, OP>((const T *)a->data, (T)c, (T *)o->data, n, s); };
+  auto f = [a = a, c = c, o = out, n = n, s = as_stream(stream), what = what](auto tag) -> int {
+      using T = typename decltype(tag)::type;
+      T cv{};
+      PM_TRY(scalar_as<T>(c, &cv, what));
+      return ewc<T, OP>((const T *)a->data, cv, (T *)o->data, n, s); };
   return dispatch_ptype(a->ptype, f);
 }
 

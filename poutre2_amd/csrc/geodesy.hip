@@ -161,7 +161,8 @@ int reconstruct_dev(const T *marker, const T *mask, T *out, Dims d, int se, hipS
   if (!tiled) PM_TRY(se_nl(se, a));
   const int tiles_x = (int)((d.W + RT - 1) / RT), tiles_y = (int)((d.H + RT - 1) / RT);
   const int xblocks = (int)((d.W + 255) / 256);
-  for (int round = 0; round < MAX_ROUNDS; round++) {
+  const int cap = round_cap(MAX_ROUNDS);
+  for (int round = 0; round < cap; round++) {
     hipLaunchKernelGGL(zero_flag_kernel, dim3(1), dim3(64), 0, s, dflag);
     if (tiled) {
       const dim3 g((unsigned)(tiles_x * tiles_y)), b(RT);
@@ -338,7 +339,8 @@ template <int K> static int extrema_entry(const pm_image *in, double hval, int s
       using T = typename decltype(tag)::type;
       const T *src = (const T *)in->data;
       T *dst = (T *)o->data;
-      const T hv = (T)h;
+      T hv{};
+      PM_TRY(scalar_as<T>(h, &hv, names[K]));
       const int64_t n = d.D * d.H * d.W;
       if (K == X_HMAX) return h_extrema<T, true>(src, hv, dst, d, se, s);
       if (K == X_HMIN) return h_extrema<T, false>(src, hv, dst, d, se, s);

@@ -1,4 +1,5 @@
 // pm_common.hip -- error state, argument checks and scratch allocation shared by all entry points.
+#include <atomic>
 #include <mutex>
 
 #include "pm_common.hpp"
@@ -103,10 +104,29 @@ static hipMemPool_t scratch_pool(int dev, hipError_t &err)
   return pools[dev];
 }
 
+static std::atomic<int> g_max_rounds{0}, g_fail_allocs{0};
+
+int round_cap(int own_cap)
+{
+  const int c = g_max_rounds.load(std::memory_order_relaxed);
+  return (c > 0 && c < own_cap) ? c : own_cap;
+}
+
+bool take_forced_alloc_failure()
+{
+  int n = g_fail_allocs.load(std::memory_order_relaxed);
+  while (n > 0)
+    if (g_fail_allocs.compare_exchange_weak(n, n - 1)) return true;
+  return false;
+}
+
 int Scratch::alloc(size_t bytes, hipStream_t s)
 {
   stream = s;
   if (bytes == 0) bytes = 256;
+  if (take_forced_alloc_failure())
+    PM_FAIL(PM_E_HIP, "scratch allocation of %zu bytes failed: out of device memory (forced by the testing hook)",
+            bytes);
   int dev = 0;
   PM_HIP(hipGetDevice(&dev));
   hipError_t e = hipSuccess;
@@ -157,3 +177,17 @@ int fill_bytes(void *p, uint8_t v, size_t n, hipStream_t s)
 extern "C" const char *pm_last_error(void) { return pm::g_err; }
 
 extern "C" const char *pm_version(void) { return "poutre2_amd 0.1 (gfx950)"; }
+
+namespace pm {
+static int set_hook(std::atomic<int> &h, int n, const char *what)
+{
+  clear_error();
+  if (n < 0) PM_FAIL(PM_E_ARG, "%s: negative count", what);
+  h.store(n);
+  return PM_OK;
+}
+}  // namespace pm
+
+extern "C" int pm_debug_set_max_rounds(int n) { return pm::set_hook(pm::g_max_rounds, n, "pm_debug_set_max_rounds"); }
+extern "C" int pm_debug_fail_allocations(int n)
+{ return pm::set_hook(pm::g_fail_allocs, n, "pm_debug_fail_allocations"); }

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <limits>
@@ -163,7 +164,33 @@ __device__ __forceinline__ void store12_any(uint8_t *p, uint32_t a, uint32_t b, 
 }
 __device__ __forceinline__ void store2_any(uint8_t *p, uint32_t a) { *reinterpret_cast<u16_a1 *>(p) = (unsigned short)a; }
 
+// testing hooks (pm_debug_set_max_rounds / pm_debug_fail_allocations): the round cap a
+// reconstruction uses (its own cap when no hook is set), and whether the next scratch
+// allocation must fail
+int round_cap(int own_cap);
+bool take_forced_alloc_failure();
+
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// A scalar operand (h of the extrema, the constant of the saturated arithmetic) converted to
+// the pixel type T.  The reference takes it as the ScalarTypeVariant alternative of the
+// image's type (std::get, arith.cpp:38,55), so it is always representable there; here an
+// integer T needs a whole number inside T's range and a float T a value inside its range
+// (or an infinity / NaN), anything else is PM_E_ARG rather than a silent wrap.
+template <typename T> int scalar_as(double v, T *out, const char *what)
+{
+  if constexpr (std::numeric_limits<T>::is_integer) {
+    const double lo = (double)std::numeric_limits<T>::lowest();
+    const double hi_excl = std::ldexp(1.0, std::numeric_limits<T>::digits);  // max + 1, exact
+    if (!(v == std::floor(v)) || v < lo || v >= hi_excl)
+      PM_FAIL(PM_E_ARG, "%s: value %.17g is not a value of the image's pixel type", what, v);
+  } else {
+    if (std::isfinite(v) && std::fabs(v) > (double)std::numeric_limits<T>::max())
+      PM_FAIL(PM_E_ARG, "%s: value %.17g is outside the image's pixel type", what, v);
+  }
+  *out = static_cast<T>(v);
+  return PM_OK;
+}
 
 constexpr int PM_MAX_DEVICES = 64;
 

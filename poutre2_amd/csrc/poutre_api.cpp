@@ -360,8 +360,13 @@ void binary(const IInterface &a, const IInterface &b, IInterface &out, const std
   so.download();
 }
 
+// the C ABI carries scalars as double: an int64 alternative beyond 2^53 would be rounded, so
+// it is refused instead
 double scalar(const ScalarTypeVariant &v)
 {
+  if (const auto *i = std::get_if<pINT64>(&v))
+    if (*i > (pINT64(1) << 53) || *i < -(pINT64(1) << 53))
+      fail("scalar operand " + std::to_string(*i) + " is not exactly representable (|value| > 2^53)");
   return std::visit([](auto x) { return static_cast<double>(x); }, v);
 }
 

@@ -402,8 +402,9 @@ static int rec_rounds(const RecImage &one, const RecImage *many, int nimg, int n
   else if (se == PM_SE_SEGMENT_Y_2D) nbk = NBK_SEGY;
   const unsigned grid = 8u * (unsigned)((nt + 7) / 8);  // every tile index below 8 * ceil(nt / 8)
   int round = 0, batch = 8;
-  while (round < REC_MAX_ROUNDS) {
-    const int nb = (round + batch <= REC_MAX_ROUNDS) ? batch : REC_MAX_ROUNDS - round;
+  const int cap = round_cap(REC_MAX_ROUNDS);
+  while (round < cap) {
+    const int nb = (round + batch <= cap) ? batch : cap - round;
     for (int i = 0; i < nb; i++) {
       const int R = round + i;
       switch (nbk) {

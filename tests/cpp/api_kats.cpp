@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "poutre_amd.h"
 #include "poutre_amd.hpp"
 
 using namespace poutre;
@@ -202,6 +203,23 @@ int main(int argc, char **argv)
   fails += expect_throw("rank mismatch", [&] { Erode(*a, se::Common_NL_SE::SESquare3D, 1, *CloneGeometry(*a)); });
   auto ns = Create({4, 6}, CompoundType::CompoundType_Scalar, PType::PType_GrayUINT8);
   fails += expect_throw("non-square DilateY", [&] { DilateY(*ns, 1, *CloneGeometry(*ns)); });
+  // a failing scratch allocation surfaces as the reference's runtime_error (config.hpp:203-215)
+  {
+    auto f = CreateDevice({64, 80}, PType::PType_GrayUINT8);
+    auto m = CreateDevice({64, 80}, PType::PType_GrayUINT8);
+    auto o = CreateDevice({64, 80}, PType::PType_GrayUINT8);
+    pm_debug_fail_allocations(1);
+    fails += expect_throw("forced allocation failure", [&] { geo::leveling(*f, *m, se::Common_NL_SE::SESquare2D, *o); });
+    pm_debug_fail_allocations(0);
+    geo::leveling(*f, *m, se::Common_NL_SE::SESquare2D, *o);  // and works again
+    // a scalar the pixel type cannot hold is refused, not wrapped; int64 beyond 2^53 too
+    fails += expect_throw("h out of range", [&] { geo::h_maxima(*f, ScalarTypeVariant(pINT32(300)), se::Common_NL_SE::SESquare2D, *o); });
+    auto f64 = CreateDevice({8, 8}, PType::PType_GrayINT64);
+    auto o64 = CreateDevice({8, 8}, PType::PType_GrayINT64);
+    fails += expect_throw("int64 h beyond 2^53", [&] {
+      geo::h_maxima(*f64, ScalarTypeVariant(pINT64((pINT64(1) << 53) + 1)), se::Common_NL_SE::SESquare2D, *o64);
+    });
+  }
   // a runtime SE of another rank than the image is refused (ero_dil.cpp:71-72)
   fails += expect_throw("SE rank", [&] { Dilate(*a, se::SESquare3D, *CloneGeometry(*a)); });
   std::printf("api_kats: %d cases run (%d skipped), %d failures\n", n, skipped, fails);
