@@ -334,28 +334,28 @@ using namespace pm;
 
 extern "C" {
 int pm_arith_sadd_image(const pm_image *a, const pm_image *b, pm_image *out, void *stream)
-{ return arith2<EW_SADD>(a, b, out, stream, "ArithSaturatedAddImage"); }
-int pm_arith_invert(const pm_image *in, pm_image *out, void *stream) { return invert_entry(in, out, as_stream(stream)); }
-int pm_convert_into(const pm_image *in, pm_image *out, void *stream) { return convert_entry(in, out, as_stream(stream)); }
+{ PM_TRACE(); return arith2<EW_SADD>(a, b, out, stream, "ArithSaturatedAddImage"); }
+int pm_arith_invert(const pm_image *in, pm_image *out, void *stream) { PM_TRACE(); return invert_entry(in, out, as_stream(stream)); }
+int pm_convert_into(const pm_image *in, pm_image *out, void *stream) { PM_TRACE(); return convert_entry(in, out, as_stream(stream)); }
 int pm_compare(const pm_image *in, int op, const pm_operand *comp, const pm_operand *vtrue, const pm_operand *vfalse,
                pm_image *out, void *stream)
-{
+{ PM_TRACE();
   return compare_entry(in, op, comp, vtrue, vfalse, out, as_stream(stream));
 }
 int pm_arith_sup(const pm_image *a, const pm_image *b, pm_image *out, void *stream)
-{ return arith2<EW_SUP>(a, b, out, stream, "ArithSupImage"); }
+{ PM_TRACE(); return arith2<EW_SUP>(a, b, out, stream, "ArithSupImage"); }
 int pm_arith_inf(const pm_image *a, const pm_image *b, pm_image *out, void *stream)
-{ return arith2<EW_INF>(a, b, out, stream, "ArithInfImage"); }
+{ PM_TRACE(); return arith2<EW_INF>(a, b, out, stream, "ArithInfImage"); }
 int pm_arith_ssub_image(const pm_image *a, const pm_image *b, pm_image *out, void *stream)
-{ return arith2<EW_SSUB>(a, b, out, stream, "ArithSaturatedSubImage"); }
+{ PM_TRACE(); return arith2<EW_SSUB>(a, b, out, stream, "ArithSaturatedSubImage"); }
 int pm_arith_ssub_const(const pm_image *in, double c, pm_image *out, void *stream)
-{ return arithc<EWC_SSUB>(in, c, out, stream, "ArithSaturatedSubConstant"); }
+{ PM_TRACE(); return arithc<EWC_SSUB>(in, c, out, stream, "ArithSaturatedSubConstant"); }
 int pm_arith_sadd_const(const pm_image *in, double c, pm_image *out, void *stream)
-{ return arithc<EWC_SADD>(in, c, out, stream, "ArithSaturatedAddConstant"); }
+{ PM_TRACE(); return arithc<EWC_SADD>(in, c, out, stream, "ArithSaturatedAddConstant"); }
 int pm_opening(const pm_image *in, int se, int iter, pm_image *out, void *stream)
-{ return compose<C_OPEN>(in, se, iter, out, stream); }
+{ PM_TRACE(); return compose<C_OPEN>(in, se, iter, out, stream); }
 int pm_closing(const pm_image *in, int se, int iter, pm_image *out, void *stream)
-{ return compose<C_CLOSE>(in, se, iter, out, stream); }
+{ PM_TRACE(); return compose<C_CLOSE>(in, se, iter, out, stream); }
 int pm_gradient(const pm_image *in, int se, int iter, pm_image *out, void *stream)
-{ return compose<C_GRAD>(in, se, iter, out, stream); }
+{ PM_TRACE(); return compose<C_GRAD>(in, se, iter, out, stream); }
 }

@@ -166,7 +166,7 @@ using namespace pm;
 
 extern "C" int pm_batch_dilate_open(int n, const pm_image *ins, const int *radii, pm_image *dil_outs,
                                     pm_image *open_outs, void *stream)
-{
+{ PM_TRACE();
   clear_error();
   BatchLayout L;
   PM_TRY(batch_layout(n, ins, radii, dil_outs, open_outs, L));
@@ -201,7 +201,7 @@ static void plan_free(pm_batch_plan *p)
 
 extern "C" int pm_batch_plan_create(int n, const pm_image *ins, const int *radii, pm_image *dil_outs,
                                     pm_image *open_outs, pm_batch_plan **plan)
-{
+{ PM_TRACE();
   clear_error();
   if (!plan) PM_FAIL(PM_E_ARG, "batch plan: null plan pointer");
   *plan = nullptr;
@@ -272,7 +272,7 @@ extern "C" int pm_batch_plan_create(int n, const pm_image *ins, const int *radii
 }
 
 extern "C" int pm_batch_plan_launch(pm_batch_plan *plan, void *stream)
-{
+{ PM_TRACE();
   clear_error();
   if (!plan || !plan->exec) PM_FAIL(PM_E_ARG, "batch plan: null plan");
   int dev = 0;
@@ -283,7 +283,7 @@ extern "C" int pm_batch_plan_launch(pm_batch_plan *plan, void *stream)
 }
 
 extern "C" int pm_batch_plan_destroy(pm_batch_plan *plan)
-{
+{ PM_TRACE();
   clear_error();
   plan_free(plan);
   return PM_OK;

@@ -538,27 +538,27 @@ using namespace pm;
 
 extern "C" {
 int pm_erode_static(const pm_image *in, int se, int iter, pm_image *out, void *stream)
-{ return erodil_static<false>(in, se, iter, out, stream); }
+{ PM_TRACE(); return erodil_static<false>(in, se, iter, out, stream); }
 int pm_dilate_static(const pm_image *in, int se, int iter, pm_image *out, void *stream)
-{ return erodil_static<true>(in, se, iter, out, stream); }
+{ PM_TRACE(); return erodil_static<true>(in, se, iter, out, stream); }
 int pm_erode_compound(const pm_image *in, int compound, int size, pm_image *out, void *stream)
-{ return erodil_compound<false>(in, compound, size, out, stream); }
+{ PM_TRACE(); return erodil_compound<false>(in, compound, size, out, stream); }
 int pm_dilate_compound(const pm_image *in, int compound, int size, pm_image *out, void *stream)
-{ return erodil_compound<true>(in, compound, size, out, stream); }
+{ PM_TRACE(); return erodil_compound<true>(in, compound, size, out, stream); }
 int pm_erode_nl(const pm_image *in, const int64_t *offsets, int n, pm_image *out, void *stream)
-{ return erodil_nl<false>(in, offsets, n, out, stream); }
+{ PM_TRACE(); return erodil_nl<false>(in, offsets, n, out, stream); }
 int pm_dilate_nl(const pm_image *in, const int64_t *offsets, int n, pm_image *out, void *stream)
-{ return erodil_nl<true>(in, offsets, n, out, stream); }
+{ PM_TRACE(); return erodil_nl<true>(in, offsets, n, out, stream); }
 int pm_erode_x(const pm_image *in, int64_t half, pm_image *out, void *stream)
-{ return erodil_line<false>(in, half, out, stream, 0); }
+{ PM_TRACE(); return erodil_line<false>(in, half, out, stream, 0); }
 int pm_dilate_x(const pm_image *in, int64_t half, pm_image *out, void *stream)
-{ return erodil_line<true>(in, half, out, stream, 0); }
+{ PM_TRACE(); return erodil_line<true>(in, half, out, stream, 0); }
 int pm_erode_y(const pm_image *in, int64_t half, pm_image *out, void *stream)
-{ return erodil_line<false>(in, half, out, stream, 1); }
+{ PM_TRACE(); return erodil_line<false>(in, half, out, stream, 1); }
 int pm_dilate_y(const pm_image *in, int64_t half, pm_image *out, void *stream)
-{ return erodil_line<true>(in, half, out, stream, 1); }
+{ PM_TRACE(); return erodil_line<true>(in, half, out, stream, 1); }
 int pm_erode_rect(const pm_image *in, int64_t ry, int64_t rx, pm_image *out, void *stream)
-{ return erodil_rect<false>(in, ry, rx, out, stream); }
+{ PM_TRACE(); return erodil_rect<false>(in, ry, rx, out, stream); }
 int pm_dilate_rect(const pm_image *in, int64_t ry, int64_t rx, pm_image *out, void *stream)
-{ return erodil_rect<true>(in, ry, rx, out, stream); }
+{ PM_TRACE(); return erodil_rect<true>(in, ry, rx, out, stream); }
 }

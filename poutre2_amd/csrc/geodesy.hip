@@ -378,23 +378,23 @@ using namespace pm;
 
 extern "C" {
 int pm_reconstruct(int rec_type, const pm_image *marker, const pm_image *mask, int se, pm_image *out, void *stream)
-{ return reconstruct_entry(rec_type, marker, mask, se, out, stream); }
+{ PM_TRACE(); return reconstruct_entry(rec_type, marker, mask, se, out, stream); }
 int pm_low_leveling(const pm_image *ref, const pm_image *marker, int se, pm_image *out, void *stream)
-{ return leveling_entry<LEV_LOW>(ref, marker, se, out, stream); }
+{ PM_TRACE(); return leveling_entry<LEV_LOW>(ref, marker, se, out, stream); }
 int pm_high_leveling(const pm_image *ref, const pm_image *marker, int se, pm_image *out, void *stream)
-{ return leveling_entry<LEV_HIGH>(ref, marker, se, out, stream); }
+{ PM_TRACE(); return leveling_entry<LEV_HIGH>(ref, marker, se, out, stream); }
 int pm_leveling(const pm_image *ref, const pm_image *marker, int se, pm_image *out, void *stream)
-{ return leveling_entry<LEV_BOTH>(ref, marker, se, out, stream); }
+{ PM_TRACE(); return leveling_entry<LEV_BOTH>(ref, marker, se, out, stream); }
 int pm_h_maxima(const pm_image *in, double h, int se, pm_image *out, void *stream)
-{ return extrema_entry<X_HMAX>(in, h, se, out, stream); }
+{ PM_TRACE(); return extrema_entry<X_HMAX>(in, h, se, out, stream); }
 int pm_h_minima(const pm_image *in, double h, int se, pm_image *out, void *stream)
-{ return extrema_entry<X_HMIN>(in, h, se, out, stream); }
+{ PM_TRACE(); return extrema_entry<X_HMIN>(in, h, se, out, stream); }
 int pm_h_convex(const pm_image *in, double h, int se, pm_image *out, void *stream)
-{ return extrema_entry<X_HCONVEX>(in, h, se, out, stream); }
+{ PM_TRACE(); return extrema_entry<X_HCONVEX>(in, h, se, out, stream); }
 int pm_h_concave(const pm_image *in, double h, int se, pm_image *out, void *stream)
-{ return extrema_entry<X_HCONCAVE>(in, h, se, out, stream); }
+{ PM_TRACE(); return extrema_entry<X_HCONCAVE>(in, h, se, out, stream); }
 int pm_dynamic_pseudo_opening(const pm_image *in, double h, int se, pm_image *out, void *stream)
-{ return extrema_entry<X_DPO>(in, h, se, out, stream); }
+{ PM_TRACE(); return extrema_entry<X_DPO>(in, h, se, out, stream); }
 int pm_dynamic_pseudo_closing(const pm_image *in, double h, int se, pm_image *out, void *stream)
-{ return extrema_entry<X_DPC>(in, h, se, out, stream); }
+{ PM_TRACE(); return extrema_entry<X_DPC>(in, h, se, out, stream); }
 }

@@ -726,12 +726,12 @@ using namespace pm;
 
 extern "C" {
 int pm_label_binary(const pm_image *in, int se, pm_image *out, int64_t *count, void *stream)
-{
+{ PM_TRACE();
   return lab_run<false>(in, se, out, count, as_stream(stream), "label_binary");
 }
 
 int pm_label_flat_zones(const pm_image *in, int se, pm_image *out, int64_t *count, void *stream)
-{
+{ PM_TRACE();
   return lab_run<true>(in, se, out, count, as_stream(stream), "label_flat_zones");
 }
 
@@ -754,6 +754,6 @@ static int lab_extrema(bool minima, const pm_image *in, int se, pm_image *out, v
   return lab_run<false>(&t, se, out, nullptr, s, what);
 }
 
-int pm_label_minima(const pm_image *in, int se, pm_image *out, void *stream) { return lab_extrema(true, in, se, out, stream); }
-int pm_label_maxima(const pm_image *in, int se, pm_image *out, void *stream) { return lab_extrema(false, in, se, out, stream); }
+int pm_label_minima(const pm_image *in, int se, pm_image *out, void *stream) { PM_TRACE(); return lab_extrema(true, in, se, out, stream); }
+int pm_label_maxima(const pm_image *in, int se, pm_image *out, void *stream) { PM_TRACE(); return lab_extrema(false, in, se, out, stream); }
 }

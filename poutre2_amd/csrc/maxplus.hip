@@ -513,20 +513,20 @@ static int maxplus_fwd(const float *x, int64_t B, int64_t H, int64_t W, const fl
 
 extern "C" int pm_maxplus_fwd(const float *x, int64_t B, int64_t H, int64_t W, const float *w, int K, int dilate,
                               float *y, void *stream)
-{
+{ PM_TRACE();
   return maxplus_fwd(x, B, H, W, w, K, dilate, y, nullptr, stream);
 }
 
 extern "C" int pm_maxplus_fwd_sel(const float *x, int64_t B, int64_t H, int64_t W, const float *w, int K, int dilate,
                                   float *y, uint8_t *sel, void *stream)
-{
+{ PM_TRACE();
   if (!sel && B * H * W > 0) PM_FAIL(PM_E_ARG, "maxplus: null tap buffer");
   return maxplus_fwd(x, B, H, W, w, K, dilate, y, sel, stream);
 }
 
 extern "C" int pm_maxplus_bwd_sel(const uint8_t *sel, const float *gy, int64_t B, int64_t H, int64_t W, int K,
                                   int dilate, int with_w, float *gx, float *gw, void *stream)
-{
+{ PM_TRACE();
   PM_TRY(mp_check(B, H, W, K, gy, gx));
   if (!sel && B * H * W > 0) PM_FAIL(PM_E_ARG, "maxplus: null tap buffer");
   if (gx == gy && B * H * W > 0) PM_FAIL(PM_E_ALIAS, "maxplus: gx must be a distinct buffer");
@@ -560,7 +560,7 @@ extern "C" int pm_maxplus_bwd_sel(const uint8_t *sel, const float *gy, int64_t B
 
 extern "C" int pm_maxplus_bwd(const float *x, const float *gy, int64_t B, int64_t H, int64_t W, const float *w, int K,
                               int dilate, float *gx, float *gw, void *stream)
-{
+{ PM_TRACE();
   PM_TRY(mp_check(B, H, W, K, x, gx));
   if (!gy && B * H * W > 0) PM_FAIL(PM_E_ARG, "maxplus: null gradient");
   if ((gx == x || gx == gy) && B * H * W > 0) PM_FAIL(PM_E_ALIAS, "maxplus: gx must be a distinct buffer");

@@ -5,6 +5,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <cfloat>
 #include <cmath>
@@ -169,6 +170,17 @@ __device__ __forceinline__ void store2_any(uint8_t *p, uint32_t a) { *reinterpre
 // allocation must fail
 int round_cap(int own_cap);
 bool take_forced_alloc_failure();
+
+// A roctx range around each C ABI call, named after the entry point, so that kernel traces
+// (rocprofv3 --marker-trace) attribute kernels to API calls; without a tracer attached a
+// push / pop is a call into an empty table.
+struct TraceRange {
+  explicit TraceRange(const char *name) { roctxRangePushA(name); }
+  ~TraceRange() { roctxRangePop(); }
+  TraceRange(const TraceRange &) = delete;
+  TraceRange &operator=(const TraceRange &) = delete;
+};
+#define PM_TRACE() ::pm::TraceRange pm_trace_range_(__func__)
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 

@@ -511,7 +511,7 @@ using namespace pm;
 // Batched reconstruction: the tiles of all images go through the same rounds.
 extern "C" int pm_batch_reconstruct(int n, int rec_type, const pm_image *markers, const pm_image *masks, int se,
                                     pm_image *outs, void *stream)
-{
+{ PM_TRACE();
   clear_error();
   if (n < 0 || (n > 0 && (!markers || !masks || !outs))) PM_FAIL(PM_E_ARG, "batch reconstruct: bad arguments");
   if (rec_type != PM_REC_DILATE && rec_type != PM_REC_ERODE) PM_FAIL(PM_E_ARG, "Unsupported reconstruction type");
@@ -541,7 +541,7 @@ extern "C" int pm_batch_reconstruct(int n, int rec_type, const pm_image *markers
 // are read into host stack memory).  Calls on one GPU share that array: they must not overlap.
 extern "C" int pm_reconstruct_noalloc(int rec_type, const pm_image *marker, const pm_image *mask, int se,
                                       pm_image *out, void *stream)
-{
+{ PM_TRACE();
   clear_error();
   PM_TRY(check_binary(marker, mask, out, "Reconstruction"));
   if (rec_type != PM_REC_DILATE && rec_type != PM_REC_ERODE) PM_FAIL(PM_E_ARG, "Unsupported reconstruction type");

@@ -124,7 +124,7 @@ template int sq_u8<false>(const uint8_t *, uint8_t *, int64_t, int64_t, int64_t,
 using namespace pm;
 
 extern "C" int pm_autotune(void *stream)
-{
+{ PM_TRACE();
   clear_error();
   hipStream_t s = as_stream(stream);
   int dev = 0, cus = 0;
