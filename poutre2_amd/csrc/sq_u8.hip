@@ -4,9 +4,10 @@
 // uint8 (ero_dil_static.cpp:229-305, ero_dil_line_se_t.hpp:49-280): one launch reads
 // every input pixel once from HBM and writes every output once.
 //
-// One workgroup (256 threads) owns an output tile of 128 rows and TW = 128, 192 or 256 columns
+// One workgroup (256 threads) owns an output tile of TH = 64 (r <= 3) or 128 rows and TW = 128,
+// 192 or 256 columns
 // (per device and radius: the table pm_autotune fills, else the rule sq_tw).
-//  1. The input region (128+2r rows, TW+2*R16 columns, R16 = 16 for r <= 15 and 32 above,
+//  1. The input region (TH+2r rows, TW+2*R16 columns, R16 = 16 for r <= 15 and 32 above,
 //     so that global loads of aligned rows are 16-byte aligned) is copied to LDS with 128-bit
 //     loads (at byte addresses when rows are not 16-byte aligned); pixels
 //     outside the image get the neutral value (0 for dilation, 255 for erosion).
@@ -51,11 +52,12 @@ static int tuned_tw(int dev, bool aligned, int r)
 static int sq_launch(bool dil, bool aligned, int tw, const uint8_t *in, uint8_t *out, int64_t B, int64_t H, int64_t W,
                      int r, int cus, hipStream_t s)
 {
-  const int tiles_x = (int)((W + tw - 1) / tw), tiles_y = (int)((H + FT - 1) / FT);
+  const int th = sq_th(r);
+  const int tiles_x = (int)((W + tw - 1) / tw), tiles_y = (int)((H + th - 1) / th);
   const int64_t per = (int64_t)tiles_x * tiles_y;
   const int64_t ntiles = per * B;
   if (ntiles > INT32_MAX) PM_FAIL(PM_E_ARG, "image too large for the fused square kernel");
-  const size_t lds = sq_tile_lds_bytes(r, tw);
+  const size_t lds = sq_tile_lds_bytes(r, tw, th);
   const sq_kernel_t k = dil ? (aligned ? sq_kernel_for<true, true>(r, tw) : sq_kernel_for<true, false>(r, tw))
                             : (aligned ? sq_kernel_for<false, true>(r, tw) : sq_kernel_for<false, false>(r, tw));
   int occ = 0;

@@ -26,6 +26,8 @@ __global__ __launch_bounds__(FTHREADS) void sq_u8_kernel(const uint8_t *__restri
                                                          int ntiles)
 {
   extern __shared__ __align__(16) uint8_t smem[];
+  static_assert(RC > 0, "the tile height follows the compile-time radius");
+  constexpr int TH = sq_th(RC);  // tile height (the host sizes the grid and LDS with sq_th)
   // A tile's region is read with all its loads in flight, then copied to LDS.  The
   // workgroups resident on a CU overlap each other's loads and compute; prefetching the
   // next tile into registers instead was slower (r = 1: 34.9 vs 33.5 us, r = 7: 40.2 vs
@@ -35,11 +37,11 @@ __global__ __launch_bounds__(FTHREADS) void sq_u8_kernel(const uint8_t *__restri
     const int img = tile / tiles_per_img;
     const int t_in = tile - img * tiles_per_img;
     const int ty = t_in / tiles_x, tx = t_in - ty * tiles_x;
-    const int y0 = ty * FT, x0 = tx * TW;
+    const int y0 = ty * TH, x0 = tx * TW;
     const int64_t ibase = (int64_t)img * H * W;
-    sq_tile_load<K, DIL, ALIGNED, RC, TW>(smem, in + ibase, H, W, r_rt, y0, x0);
+    sq_tile_load<K, DIL, ALIGNED, RC, TW, sq_r16<K>(), false, TH>(smem, in + ibase, H, W, r_rt, y0, x0);
     __syncthreads();
-    sq_tile_passes<K, DIL, ALIGNED, RC, TW>(smem, out + ibase, H, W, r_rt, y0, x0);
+    sq_tile_passes<K, DIL, ALIGNED, RC, TW, TH>(smem, out + ibase, H, W, r_rt, y0, x0);
     __syncthreads();  // A/T are rewritten by the next tile
   }
 }

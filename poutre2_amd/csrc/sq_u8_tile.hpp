@@ -8,13 +8,23 @@ namespace pm {
 
 typedef unsigned short u16x2_t __attribute__((ext_vector_type(2)));
 
-constexpr int FT = 128;       // output tile height (and default width)
+constexpr int FT = 128;       // default output tile height (and default width)
 constexpr int FTHREADS = 256;
-// Transposed buffer T: one 128-byte row per region column, 32-bit word w of row R stored
-// at word (w ^ ((R >> 2) & 31)).  Pass 1 writes rows 4q..4q+3 from lanes of consecutive q
-// and pass 2 reads rows k2*K+t at word q2; with the swizzle both land on distinct banks
-// (a plain pitch maps them onto 8 banks).
-__device__ __forceinline__ int t_off(int R, int w) { return R * FT + ((w ^ ((R >> 2) & 31)) << 2); }
+// Output tile height of the single-image square kernel for radius r.  Small radii read
+// little halo, and 64-row tiles need half the LDS and fewer registers (6 resident
+// workgroups per CU instead of 3); from r = 4 on the halo of the shorter tile costs more
+// than the occupancy gains.  Measured on 8192x8192 dilation (us, 128 vs 64 rows):
+// r = 1 33.6 / 31.3, r = 2 35.7 / 31.6, r = 3 36.9 / 32.4, r = 4 34.6 / 34.2,
+// r = 5 35.3 / 35.4, r = 7 36.4 / 36.3, r = 8 33.5 / 36.7.
+constexpr int sq_th(int r) { return r <= 3 ? 64 : FT; }
+// Transposed buffer T of a tile TH rows high: one TH-byte row per region column, 32-bit
+// word w of row R stored at word (w ^ ((R >> 2) & (TH/4 - 1))).  Pass 1 writes rows
+// 4q..4q+3 from lanes of consecutive q and pass 2 reads rows k2*K+t at word q2; with the
+// swizzle both land on distinct banks (a plain pitch maps them onto 8 banks).
+template <int TH = FT> __device__ __forceinline__ int t_off(int R, int w)
+{
+  return R * TH + ((w ^ ((R >> 2) & (TH / 4 - 1))) << 2);
+}
 
 template <bool DIL> __device__ __forceinline__ uint32_t pk(uint32_t a, uint32_t b)
 {
@@ -129,17 +139,17 @@ constexpr int sq_tw(int K, int RC) { return (RC > 0 ? RC <= 16 : K < 32) ? 256 :
 constexpr int sq_rmax_of(int K) { return K == 4 ? 3 : K == 8 ? 7 : K == 16 ? 15 : 32; }
 // vertical-pass tasks (column quad x run of K rows) per thread for tiles TW wide: the
 // quads cover columns [R16 - r, R16 + TW + r) of the region
-constexpr int sq_maxt(int K, int TW)
+constexpr int sq_maxt(int K, int TW, int TH = FT)
 {
   const int r = sq_rmax_of(K), R16 = K == 32 ? 32 : 16;
   const int nq = ((R16 + TW + r + 3) >> 2) - ((R16 - r) >> 2);  // largest at the largest radius
-  return (nq * (FT / K) + FTHREADS - 1) / FTHREADS;
+  return (nq * (TH / K) + FTHREADS - 1) / FTHREADS;
 }
-// 16-byte region chunks per thread (region of FT + 2r rows, TW + 2*R16 columns)
-constexpr int sq_maxc(int K, int TW)
+// 16-byte region chunks per thread (region of TH + 2r rows, TW + 2*R16 columns)
+constexpr int sq_maxc(int K, int TW, int TH = FT)
 {
   const int r = sq_rmax_of(K), R16 = K == 32 ? 32 : 16;
-  return ((FT + 2 * r) * ((TW + 2 * R16) >> 4) + FTHREADS - 1) / FTHREADS;
+  return ((TH + 2 * r) * ((TW + 2 * R16) >> 4) + FTHREADS - 1) / FTHREADS;
 }
 
 // Windowed max/min of 2r+1 rows for K consecutive output rows of one 4-byte column quad.
@@ -215,7 +225,7 @@ __device__ __forceinline__ void vh_run(LD &&row, int r_rt, uint32_t (&e)[K], uin
 // Windowed max/min along the rows of the transposed buffer T for the task (word q2, output
 // rows k2*K .. k2*K+K-1 of T's line order, whose row c0 holds line 0's window start): the
 // high-byte pairs e / o of vh_run.
-template <int K, bool DIL, int RC>
+template <int K, bool DIL, int RC, int TH = FT>
 __device__ __forceinline__ void sq_tline(const uint8_t *T, int r, int c0, int q2, int k2, uint32_t (&e)[K],
                                          uint32_t (&o)[K])
 {
@@ -224,20 +234,20 @@ __device__ __forceinline__ void sq_tline(const uint8_t *T, int r, int c0, int q2
   // group share a base address and differ by immediate offsets.
   const int R0 = k2 * K + c0;
   const int m = k2 * (K / 4) + (c0 >> 2), s0 = c0 & 3;
-  const uint8_t *Tb = T + R0 * FT;
+  const uint8_t *Tb = T + R0 * TH;
   vh_run<K, DIL, RC>(
       [&](int t) {
         const int u = (s0 + t) >> 2;
-        return lds_word((Tb + ((q2 ^ ((m + u) & 31)) << 2)) + t * FT);
+        return lds_word((Tb + ((q2 ^ ((m + u) & (TH / 4 - 1))) << 2)) + t * TH);
       },
       r, e, o);
 }
 
-template <int K, bool DIL, int RC>
+template <int K, bool DIL, int RC, int TH = FT>
 __device__ __forceinline__ void sq_pass3_rows(const uint8_t *T, int r, int c0, int q2, int k2, uint32_t (&rows)[4][K / 4])
 {
   uint32_t e[K], o[K];
-  sq_tline<K, DIL, RC>(T, r, c0, q2, k2, e, o);
+  sq_tline<K, DIL, RC, TH>(T, r, c0, q2, k2, e, o);
 #pragma unroll
   for (int g = 0; g < K / 4; g++) {
     uint32_t w[4];
@@ -309,11 +319,11 @@ __device__ __forceinline__ void sq_store_rows(const uint32_t (&rows)[4][K / 4], 
   }
 }
 
-// Passes 2 and 3 of one tile of FT rows and TW columns whose input region (rows
-// y0-r .. y0+FT+r-1, columns x0-R16 .. x0+TW+R16-1 of the image) is in smem (row pitch
+// Passes 2 and 3 of one tile of TH rows and TW columns whose input region (rows
+// y0-r .. y0+TH+r-1, columns x0-R16 .. x0+TW+R16-1 of the image) is in smem (row pitch
 // AW = TW + 2*R16).  dst is the output image (row pitch W).  ALIGNED: W % 16 == 0 and dst
 // 16-byte aligned.  RC > 0: the radius is the compile-time constant RC (r_rt is ignored).
-template <int K, bool DIL, bool ALIGNED, int RC = 0, int TW = FT>
+template <int K, bool DIL, bool ALIGNED, int RC = 0, int TW = FT, int TH = FT>
 __device__ __forceinline__ void sq_tile_passes(uint8_t *smem, uint8_t *__restrict__ dst_img, int H, int W, int r_rt,
                                                int y0, int x0)
 {
@@ -328,10 +338,10 @@ __device__ __forceinline__ void sq_tile_passes(uint8_t *smem, uint8_t *__restric
   // storage of A.  Aliasing the two buffers halves the LDS footprint and doubles the
   // number of resident workgroups.
   {
-    constexpr int MAXT = sq_maxt(K, TW);
+    constexpr int MAXT = sq_maxt(K, TW, TH);
     const int q0 = (R16 - r) >> 2, q1 = (R16 + TW + r + 3) >> 2;
     const int nq = q1 - q0;
-    const int ntask = nq * (FT / K);
+    const int ntask = nq * (TH / K);
     uint32_t res[MAXT][K];
 #pragma unroll
     for (int sl = 0; sl < MAXT; sl++) {
@@ -355,7 +365,7 @@ __device__ __forceinline__ void sq_tile_passes(uint8_t *smem, uint8_t *__restric
         for (int g = 0; g < K / 4; g++)
 #pragma unroll
           for (int c = 0; c < 4; c++)
-            *reinterpret_cast<uint32_t *>(T + t_off(4 * q + c, (k * K >> 2) + g)) = res[sl][4 * g + c];
+            *reinterpret_cast<uint32_t *>(T + t_off<TH>(4 * q + c, (k * K >> 2) + g)) = res[sl][4 * g + c];
       }
     }
   }
@@ -364,36 +374,37 @@ __device__ __forceinline__ void sq_tile_passes(uint8_t *smem, uint8_t *__restric
   // ---- 3. horizontal pass over T, transposed back in registers, stored to HBM
   {
     constexpr int NR = TW / K;
-    constexpr int ntask = (FT / 4) * NR;
+    constexpr int ntask = (TH / 4) * NR;
     // tile wholly inside the image: no bounds checks on the stores
-    const bool full = (y0 + FT <= H) && (x0 + TW <= W);
+    const bool full = (y0 + TH <= H) && (x0 + TW <= W);
     for (int task = tid; task < ntask; task += FTHREADS) {
       const int q2 = task / NR, k2 = task - q2 * NR;
       uint32_t rows[4][K / 4];
-      sq_pass3_rows<K, DIL, RC>(T, r, R16 - r, q2, k2, rows);
+      sq_pass3_rows<K, DIL, RC, TH>(T, r, R16 - r, q2, k2, rows);
       sq_store_rows<K, ALIGNED, TW>(rows, dst_img, H, W, y0, x0, q2, k2, full);
     }
   }
 }
 
-// Copy the input region of the tile at (y0, x0) -- rows y0-r .. y0+FT+r-1, columns
+// Copy the input region of the tile at (y0, x0) -- rows y0-r .. y0+TH+r-1, columns
 // x0-R16 .. x0+TW+R16-1 -- into smem (row pitch TW + 2*R16) with all of a thread's
 // 16-byte loads in flight before the LDS writes.  Pixels outside the image get the neutral
 // value.  ALIGNED: W % 16 == 0 and src 16-byte aligned (a chunk is wholly inside or
 // outside the image); otherwise load16_any, and the chunk that straddles the right edge
 // of the image comes from load_edge_chunk.
-// PAD (a multiple of 16, default R16) replaces R16 as the column pad.
-template <int K, bool DIL, bool ALIGNED, int RC = 0, int TW = FT, int PAD = sq_r16<K>(), bool OPQ = false>
+// PAD (a multiple of 16, default R16) replaces R16 as the column pad; TH is the tile height.
+template <int K, bool DIL, bool ALIGNED, int RC = 0, int TW = FT, int PAD = sq_r16<K>(), bool OPQ = false,
+          int TH = FT>
 __device__ __forceinline__ void sq_tile_load(uint8_t *smem, const uint8_t *__restrict__ src, int H, int W, int r_rt,
                                              int y0, int x0)
 {
-  constexpr int MAXC = PAD == sq_r16<K>() ? sq_maxc(K, TW)
-                                          : ((FT + 2 * (RC > 0 ? RC : sq_rmax_of(K))) * ((TW + 2 * PAD) >> 4) +
+  constexpr int MAXC = PAD == sq_r16<K>() ? sq_maxc(K, TW, TH)
+                                          : ((TH + 2 * (RC > 0 ? RC : sq_rmax_of(K))) * ((TW + 2 * PAD) >> 4) +
                                              FTHREADS - 1) / FTHREADS;
   constexpr int R16 = PAD;
   constexpr int AW = TW + 2 * R16, cpr = AW >> 4;
   const int r = RC > 0 ? RC : r_rt;
-  const int nchunk = (FT + 2 * r) * cpr;
+  const int nchunk = (TH + 2 * r) * cpr;
   const uint32_t nv = DIL ? 0u : 0xFFFFFFFFu;
   const int gx0 = x0 - R16;  // image column of region column 0 (a multiple of 16)
   const int tid = tile_tid<OPQ>();
@@ -425,7 +436,7 @@ __device__ __forceinline__ void sq_tile_load(uint8_t *smem, const uint8_t *__res
     }
   }
   if (!ALIGNED && ce >= 0 && ce < cpr) {
-    for (int ri = tid; ri < FT + 2 * r; ri += FTHREADS) {
+    for (int ri = tid; ri < TH + 2 * r; ri += FTHREADS) {
       const int gy = y0 - r + ri;
       uint4 e = make_uint4(nv, nv, nv, nv);
       if ((unsigned)gy < (unsigned)H) e = load_edge_chunk(src + (int64_t)gy * W, W, gx0 + (ce << 4), DIL ? 0 : 255);
@@ -589,12 +600,12 @@ __device__ __forceinline__ void sq_tile_grad_passes(uint8_t *smem, uint8_t *__re
 }
 
 constexpr int SQ_TILE_RMAX = 32;
-// LDS bytes of one workgroup for radius r and tile width TW (region A and transposed
-// buffer T alias)
-inline size_t sq_tile_lds_bytes(int r, int TW = FT)
+// LDS bytes of one workgroup for radius r, tile width TW and height TH (region A and
+// transposed buffer T alias)
+inline size_t sq_tile_lds_bytes(int r, int TW = FT, int TH = FT)
 {
   const int R16 = sq_k_of(r) == 32 ? 32 : 16;
-  const size_t a = (size_t)(FT + 2 * r) * (TW + 2 * R16), t = (size_t)(TW + 2 * R16) * FT;
+  const size_t a = (size_t)(TH + 2 * r) * (TW + 2 * R16), t = (size_t)(TW + 2 * R16) * TH;
   return a > t ? a : t;
 }
 
