@@ -29,9 +29,12 @@
  *   extrema chain              KATs test/geodesy/extrema.cpp
  *   3D SESquare3D / SECross3D  or_static computes the mathematical definition (= the runtime
  *                              neighbour-list path, ero_dil_runtime_nl_se_t.hpp:48-97), pinned by
- *                              the z-constant 3D KATs.  What the reference's static slice
- *                              composites (ero_dil_static_se_t.hpp:761-915) return on z-varying
- *                              input is parity unpinned: only running the reference could pin it.
+ *                              the z-constant 3D KATs and the brute-force 3D window.  The
+ *                              reference's static slice composites (ero_dil_static_se_t.hpp:
+ *                              761-915) return something else on z-varying input; a literal
+ *                              restatement of them (tests/test_static3d_composite.py) reproduces
+ *                              the reference's 3D KATs and shows where they diverge.  The
+ *                              library keeps the definition, as or_static does.
  */
 #ifndef POUTRE_ORACLE_H
 #define POUTRE_ORACLE_H
