@@ -6,9 +6,10 @@
 // ever raises a pixel to a value it is entitled to reaches the same bits.  Here:
 //  * 2D (3x3 neighbourhoods): one wavefront per 64x64 tile keeps the tile plus a 1-pixel
 //    halo in LDS and sweeps it in the four raster directions until nothing changes,
-//    then writes it back; a device-wide flag records whether any tile border moved,
-//    and the host repeats global rounds until a round moves no border.
-//  * any rank (3D SEs): in-place neighbour-list relaxation rounds with the same flag.
+//    then writes it back; a per-round device flag records whether any tile border moved.
+//  * any rank (3D SEs): in-place neighbour-list relaxation rounds with the same flags.
+// Rounds are enqueued in batches without host synchronisation: round R exits at once when
+// round R-1 moved nothing, and the host reads the batch's flags once, at its end.
 // Reconstruction by erosion is the dual (min/max exchanged).
 // Leveling uses the closed forms (verified against the priority-queue algorithm of
 // leveling_t.hpp:34-202 by the oracle tests):
@@ -45,11 +46,17 @@ static unsigned nb_bits(int se)
 }
 
 __global__ void zero_flag_kernel(int *f) { if (threadIdx.x == 0) f[0] = 0; }
+__global__ void zero_flags_kernel(int *f, int n)
+{
+  if ((int)threadIdx.x < n) f[threadIdx.x] = 0;
+}
 
 template <typename T, bool DIL, unsigned NB>
 __global__ __launch_bounds__(RT) void rec_tile_kernel(T *__restrict__ cur, const T *__restrict__ mask, int H, int W,
-                                                      int tiles_x, int *__restrict__ border_moved)
+                                                      int tiles_x, const int *__restrict__ prev,
+                                                      int *__restrict__ border_moved)
 {
+  if (prev && *prev == 0) return;  // the previous round moved nothing: converged
   using L = Lat<T, DIL>;
   __shared__ T c[RP * RP];
   __shared__ T m[RT * RT];
@@ -112,8 +119,10 @@ __global__ __launch_bounds__(RT) void rec_tile_kernel(T *__restrict__ cur, const
 
 template <typename T, bool DIL>
 __global__ __launch_bounds__(256) void rec_nl_kernel(T *__restrict__ cur, const T *__restrict__ mask, int D, int H,
-                                                     int W, int xblocks, NLArgs a, int *__restrict__ moved)
+                                                     int W, int xblocks, NLArgs a, const int *__restrict__ prev,
+                                                     int *__restrict__ moved)
 {
+  if (prev && *prev == 0) return;  // the previous round moved nothing: converged
   using L = Lat<T, DIL>;
   const int row = blockIdx.x / xblocks;
   const int x = (blockIdx.x - row * xblocks) * 256 + threadIdx.x;
@@ -150,11 +159,10 @@ int reconstruct_dev(const T *marker, const T *mask, T *out, Dims d, int se, hipS
   // select_marker: marker ^ mask (dilate) / marker v mask (erode), mreconstruct_t.hpp:51,69
   PM_TRY((ew2<T, DIL ? EW_INF : EW_SUP>(marker, mask, out, n, s)));
   Scratch flag;
-  PM_TRY(flag.alloc(sizeof(int), s));
-  int *dflag = (int *)flag.ptr;
+  PM_TRY(flag.alloc(sizeof(int) * PM_COUNTERS, s));
+  int *dflag = (int *)flag.ptr;  // one change flag per round of a batch
   int *hflag = nullptr;
-  PM_HIP(hipHostMalloc((void **)&hflag, sizeof(int), hipHostMallocDefault));
-  struct HostFree { int *p; ~HostFree() { if (p) (void)hipHostFree(p); } } hf{hflag};
+  PM_TRY(pinned_counters(&hflag));
   const unsigned nb = nb_bits(se);
   const bool tiled = nb != 0 && d.D == 1;
   NLArgs a;
@@ -162,25 +170,36 @@ int reconstruct_dev(const T *marker, const T *mask, T *out, Dims d, int se, hipS
   const int tiles_x = (int)((d.W + RT - 1) / RT), tiles_y = (int)((d.H + RT - 1) / RT);
   const int xblocks = (int)((d.W + 255) / 256);
   const int cap = round_cap(MAX_ROUNDS);
-  for (int round = 0; round < cap; round++) {
-    hipLaunchKernelGGL(zero_flag_kernel, dim3(1), dim3(64), 0, s, dflag);
-    if (tiled) {
-      const dim3 g((unsigned)(tiles_x * tiles_y)), b(RT);
-      if (nb == NB_SQUARE)
-        hipLaunchKernelGGL((rec_tile_kernel<T, DIL, NB_SQUARE>), g, b, 0, s, out, mask, (int)d.H, (int)d.W, tiles_x, dflag);
-      else if (nb == NB_CROSS)
-        hipLaunchKernelGGL((rec_tile_kernel<T, DIL, NB_CROSS>), g, b, 0, s, out, mask, (int)d.H, (int)d.W, tiles_x, dflag);
-      else if (nb == NB_SEGX)
-        hipLaunchKernelGGL((rec_tile_kernel<T, DIL, NB_SEGX>), g, b, 0, s, out, mask, (int)d.H, (int)d.W, tiles_x, dflag);
-      else
-        hipLaunchKernelGGL((rec_tile_kernel<T, DIL, NB_SEGY>), g, b, 0, s, out, mask, (int)d.H, (int)d.W, tiles_x, dflag);
-    } else
-      hipLaunchKernelGGL((rec_nl_kernel<T, DIL>), dim3((unsigned)(xblocks * d.D * d.H)), dim3(256), 0, s, out, mask,
-                         (int)d.D, (int)d.H, (int)d.W, xblocks, a, dflag);
+  int round = 0, batch = 4;
+  while (round < cap) {
+    const int n = (round + batch <= cap) ? batch : cap - round;
+    hipLaunchKernelGGL(zero_flags_kernel, dim3(1), dim3(PM_COUNTERS), 0, s, dflag, n);
+    for (int i = 0; i < n; i++) {
+      // the first round of a batch runs unconditionally: the host saw its predecessor move
+      const int *prev = i > 0 ? dflag + i - 1 : nullptr;
+      int *mv = dflag + i;
+      if (tiled) {
+        const dim3 g((unsigned)(tiles_x * tiles_y)), b(RT);
+        const int H = (int)d.H, W = (int)d.W;
+        if (nb == NB_SQUARE)
+          hipLaunchKernelGGL((rec_tile_kernel<T, DIL, NB_SQUARE>), g, b, 0, s, out, mask, H, W, tiles_x, prev, mv);
+        else if (nb == NB_CROSS)
+          hipLaunchKernelGGL((rec_tile_kernel<T, DIL, NB_CROSS>), g, b, 0, s, out, mask, H, W, tiles_x, prev, mv);
+        else if (nb == NB_SEGX)
+          hipLaunchKernelGGL((rec_tile_kernel<T, DIL, NB_SEGX>), g, b, 0, s, out, mask, H, W, tiles_x, prev, mv);
+        else
+          hipLaunchKernelGGL((rec_tile_kernel<T, DIL, NB_SEGY>), g, b, 0, s, out, mask, H, W, tiles_x, prev, mv);
+      } else
+        hipLaunchKernelGGL((rec_nl_kernel<T, DIL>), dim3((unsigned)(xblocks * d.D * d.H)), dim3(256), 0, s, out, mask,
+                           (int)d.D, (int)d.H, (int)d.W, xblocks, a, prev, mv);
+    }
     PM_HIP(hipGetLastError());
-    PM_HIP(hipMemcpyAsync(hflag, dflag, sizeof(int), hipMemcpyDeviceToHost, s));
+    PM_HIP(hipMemcpyAsync(hflag, dflag, sizeof(int) * n, hipMemcpyDeviceToHost, s));
     PM_HIP(hipStreamSynchronize(s));
-    if (*hflag == 0) return PM_OK;
+    for (int i = 0; i < n; i++)
+      if (hflag[i] == 0) return PM_OK;
+    round += n;
+    batch = batch * 2 > PM_COUNTERS ? PM_COUNTERS : batch * 2;
   }
   PM_FAIL(PM_E_NOCONV, "reconstruction did not converge");
 }

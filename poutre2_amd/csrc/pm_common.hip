@@ -120,6 +120,14 @@ bool take_forced_alloc_failure()
   return false;
 }
 
+int pinned_counters(int **h)
+{
+  static thread_local int *counters = nullptr;
+  if (!counters) PM_HIP(hipHostMalloc((void **)&counters, sizeof(int) * PM_COUNTERS, hipHostMallocDefault));
+  *h = counters;
+  return PM_OK;
+}
+
 int Scratch::alloc(size_t bytes, hipStream_t s)
 {
   stream = s;

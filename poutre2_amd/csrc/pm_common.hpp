@@ -171,6 +171,11 @@ __device__ __forceinline__ void store2_any(uint8_t *p, uint32_t a) { *reinterpre
 int round_cap(int own_cap);
 bool take_forced_alloc_failure();
 
+// Pinned host memory for PM_COUNTERS round counters, allocated once per host thread (the
+// reconstructions read their convergence counters into it once per batch of rounds).
+constexpr int PM_COUNTERS = 64;
+int pinned_counters(int **h);
+
 // A roctx range around each C ABI call, named after the entry point, so that kernel traces
 // (rocprofv3 --marker-trace) attribute kernels to API calls; without a tracer attached a
 // push / pop is a call into an empty table.

@@ -386,7 +386,7 @@ static size_t rec_scratch_ints(int64_t nt) { return (size_t)nt + REC_MAX_ROUNDS;
 
 // Rounds driver shared by the single-image, batched and scratch-free entry points.  `sc` is
 // device scratch of rec_scratch_ints(nt) ints (cleared here by a fill kernel), `hmoved` host
-// memory for 64 counters (pinned, or any host memory when the caller cannot allocate).
+// memory for PM_COUNTERS counters (pinned, or any host memory when the caller cannot allocate).
 template <bool DIL>
 static int rec_rounds(const RecImage &one, const RecImage *many, int nimg, int nt, int se, int *sc, int *hmoved,
                       hipStream_t s)
@@ -428,7 +428,7 @@ static int rec_rounds(const RecImage &one, const RecImage *many, int nimg, int n
     for (int i = 0; i < nb; i++)
       if (hmoved[i] == 0) return PM_OK;
     round += nb;
-    batch = batch * 2 > 64 ? 64 : batch * 2;
+    batch = batch * 2 > PM_COUNTERS ? PM_COUNTERS : batch * 2;
   }
   PM_FAIL(PM_E_NOCONV, "reconstruction did not converge");
 }
@@ -445,15 +445,6 @@ static int rec_describe(const uint8_t *marker, const uint8_t *mask, uint8_t *out
   r.tiles_x = (int)tx;
   r.tiles_y = (int)ty;
   r.tile0 = (int)tile0;
-  return PM_OK;
-}
-
-// pinned host buffer for the convergence counters, allocated once per host thread
-static int pinned_counters(int **h)
-{
-  static thread_local int *hmoved = nullptr;
-  if (!hmoved) PM_HIP(hipHostMalloc((void **)&hmoved, sizeof(int) * 64, hipHostMallocDefault));
-  *h = hmoved;
   return PM_OK;
 }
 
@@ -560,7 +551,7 @@ extern "C" int pm_reconstruct_noalloc(int rec_type, const pm_image *marker, cons
             RW, NOALLOC_SCRATCH_INTS - REC_MAX_ROUNDS);
   int *sc = nullptr;
   PM_TRY(noalloc_scratch(&sc));
-  int hmoved[64];
+  int hmoved[PM_COUNTERS];
   hipStream_t s = as_stream(stream);
   if (rec_type == PM_REC_DILATE) return rec_rounds<true>(one, nullptr, 1, (int)nt, se, sc, hmoved, s);
   return rec_rounds<false>(one, nullptr, 1, (int)nt, se, sc, hmoved, s);
