@@ -68,32 +68,106 @@ __device__ __forceinline__ int select_tap(const float *p, int pitch, const float
   return sel;
 }
 
-// SEL: also store the selected tap of every output (without it the tap index is dead code and
-// the window reduces to a plain max / min chain)
+// Max-plus forward over LDS tiles.  A thread computes four horizontally adjacent outputs from
+// row segments read with 16-byte LDS loads (NV float4 per tap row instead of 4K scalar reads).
+// SEL (a call autograd records): every tap is compared in raster order with a strict
+// comparison, so the first maximiser wins, and its index is stored.  Without SEL the window
+// reduces to an IEEE max / min chain (v_max3 / v_min3): it ignores NaN like the strict
+// comparison, and differs from it only in which zero it returns when +0 and -0 tie for the
+// extremum, so an output whose extremum is zero is recomputed with the strict comparison.
 template <int K, bool DIL, bool FLAT, bool SEL>
 __global__ __launch_bounds__(MTHREADS) void mp_fwd_kernel(const float *__restrict__ x, const float *__restrict__ wg,
                                                           float *__restrict__ y, uint8_t *__restrict__ selo, int H,
                                                           int W, int tiles_x, int tiles_per_img)
 {
-  constexpr int R = (K - 1) / 2, RC = MTW + K - 1, RR = MTH_F + K - 1;
-  __shared__ float X[RR * RC];
-  __shared__ float w[K * K];
+  constexpr int R = (K - 1) / 2, RR = MTH_F + K - 1;
+  constexpr int RCP = (MTW + K - 1 + 3) & ~3;  // region pitch, a multiple of 4 floats
+  constexpr int NV = (4 + K - 1 + 3) / 4;      // float4 per row segment of 4 outputs
+  static_assert(MTW % 4 == 0 && MTW - 4 + 4 * NV <= RCP, "row segments stay inside the region row");
+  __shared__ __align__(16) float X[RR * RCP];
   const int img = blockIdx.x / tiles_per_img, t = blockIdx.x - img * tiles_per_img;
   const int ty = t / tiles_x, tx = t - ty * tiles_x;
   const int y0 = ty * MTH_F, x0 = tx * MTW;
-  const float *xi = x + (int64_t)img * H * W;
-  if (!FLAT && threadIdx.x < K * K) w[threadIdx.x] = wg[threadIdx.x];
-  load_region(X, RR, RC, xi, H, W, y0 - R, x0 - R, pad_value<DIL>());
+  const int64_t ib = (int64_t)img * H * W;
+  load_region(X, RR, RCP, x + ib, H, W, y0 - R, x0 - R, pad_value<DIL>());
+  float wv[K * K];  // uniform: scalar loads
+#pragma unroll
+  for (int k = 0; k < K * K; k++) wv[k] = FLAT ? 0.f : wg[k];
   __syncthreads();
-  float *yi = y + (int64_t)img * H * W;
-  for (int e = threadIdx.x; e < MTH_F * MTW; e += MTHREADS) {
-    const int i = e / MTW, j = e - i * MTW;
+  auto tap = [&](float xv, int k) { return FLAT ? xv : (DIL ? xv + wv[k] : xv - wv[k]); };
+  for (int e = threadIdx.x; e < MTH_F * (MTW / 4); e += MTHREADS) {
+    const int i = e / (MTW / 4), j = (e - i * (MTW / 4)) * 4;
     const int gy = y0 + i, gx = x0 + j;
     if (gy >= H || gx >= W) continue;
-    float best;
-    const int sel = select_tap<K, DIL, FLAT>(&X[i * RC + j], RC, w, best);
-    yi[(int64_t)gy * W + gx] = best;
-    if (SEL) selo[(int64_t)img * H * W + (int64_t)gy * W + gx] = (uint8_t)sel;
+    const float *base = &X[i * RCP + j];
+    float best[4];
+    int sel[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      best[q] = pad_value<DIL>();
+      sel[q] = 0;
+    }
+#pragma unroll
+    for (int dy = 0; dy < K; dy++) {
+      float r[4 * NV];
+#pragma unroll
+      for (int u = 0; u < NV; u++) {
+        const float4 f = *reinterpret_cast<const float4 *>(base + dy * RCP + 4 * u);
+        r[4 * u] = f.x;
+        r[4 * u + 1] = f.y;
+        r[4 * u + 2] = f.z;
+        r[4 * u + 3] = f.w;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        if constexpr (SEL) {
+#pragma unroll
+          for (int dx = 0; dx < K; dx++) {
+            const float v = tap(r[q + dx], dy * K + dx);
+            const bool better = DIL ? (v > best[q]) : (v < best[q]);
+            best[q] = better ? v : best[q];
+            sel[q] = better ? dy * K + dx : sel[q];
+          }
+        } else {
+          float m = tap(r[q], dy * K);
+#pragma unroll
+          for (int dx = 1; dx < K; dx++) m = DIL ? fmaxf(m, tap(r[q + dx], dy * K + dx)) : fminf(m, tap(r[q + dx], dy * K + dx));
+          best[q] = DIL ? fmaxf(best[q], m) : fminf(best[q], m);
+        }
+      }
+    }
+    if constexpr (!SEL) {
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if (best[q] == 0.0f) {  // +0 / -0 tie: the strict comparison decides the sign
+          float b = pad_value<DIL>();
+#pragma unroll
+          for (int dy = 0; dy < K; dy++)
+#pragma unroll
+            for (int dx = 0; dx < K; dx++) {
+              const float v = tap(base[dy * RCP + q + dx], dy * K + dx);
+              b = (DIL ? (v > b) : (v < b)) ? v : b;
+            }
+          best[q] = b;
+        }
+    }
+    float *yo = y + ib + (int64_t)gy * W + gx;
+    uint8_t *so = SEL ? selo + ib + (int64_t)gy * W + gx : nullptr;
+    if (gx + 4 <= W && ((uintptr_t)yo & 15) == 0) {
+      *reinterpret_cast<float4 *>(yo) = make_float4(best[0], best[1], best[2], best[3]);
+      if (SEL) {
+        const uint32_t packed = (uint32_t)sel[0] | ((uint32_t)sel[1] << 8) | ((uint32_t)sel[2] << 16) | ((uint32_t)sel[3] << 24);
+        if (((uintptr_t)so & 3) == 0) *reinterpret_cast<uint32_t *>(so) = packed;
+        else store4_any(so, packed);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if (gx + q < W) {
+          yo[q] = best[q];
+          if (SEL) so[q] = (uint8_t)sel[q];
+        }
+    }
   }
 }
 
