@@ -34,9 +34,11 @@ __device__ __forceinline__ void batch_tile(uint8_t *smem, const uint8_t *src, ui
                                            int y0, int x0)
 {
   constexpr int K = sq_k_of(R), TW = sq_tw(K, R);
-  sq_tile_load<K, DIL, false, R, TW>(smem, src, b.H, b.W, R, y0, x0);
+  // thread-index-derived offsets recomputed per item (tile_tid): the batched kernels hold
+  // two or three tile variants, whose hoisted offsets would all stay live
+  sq_tile_load<K, DIL, false, R, TW, sq_r16<K>(), true>(smem, src, b.H, b.W, R, y0, x0);
   __syncthreads();
-  sq_tile_passes<K, DIL, false, R, TW>(smem, dst, b.H, b.W, R, y0, x0);
+  sq_tile_passes<K, DIL, false, R, TW, FT, true>(smem, dst, b.H, b.W, R, y0, x0);
   __syncthreads();  // LDS is rewritten by the next item
 }
 
@@ -98,7 +100,7 @@ __global__ __launch_bounds__(FTHREADS) void batch_fused_kernel(const BatchImage 
       const BatchImage b = im[batch_find<true>(im, nimg, t)];
       const int tl = t - b.otile0, ty = tl / b.tiles_x, tx = tl - ty * b.tiles_x;
       const int y0 = ty * FO, x0 = tx * OPEN_TW;
-      sq_tile_load<K, false, false, R, OPEN_TW, open_pad(R)>(smem, b.in, b.H, b.W, R, y0 - R, x0);
+      sq_tile_load<K, false, false, R, OPEN_TW, open_pad(R), true>(smem, b.in, b.H, b.W, R, y0 - R, x0);
       __syncthreads();
       sq_open_tile<K, true, false, R, OPEN_TW>(smem, b.open, b.H, b.W, y0, x0);
       __syncthreads();  // LDS is rewritten by the next item

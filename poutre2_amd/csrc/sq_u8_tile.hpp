@@ -323,7 +323,8 @@ __device__ __forceinline__ void sq_store_rows(const uint32_t (&rows)[4][K / 4], 
 // y0-r .. y0+TH+r-1, columns x0-R16 .. x0+TW+R16-1 of the image) is in smem (row pitch
 // AW = TW + 2*R16).  dst is the output image (row pitch W).  ALIGNED: W % 16 == 0 and dst
 // 16-byte aligned.  RC > 0: the radius is the compile-time constant RC (r_rt is ignored).
-template <int K, bool DIL, bool ALIGNED, int RC = 0, int TW = FT, int TH = FT>
+// OPQ: thread-index-derived offsets recomputed per tile (see tile_tid).
+template <int K, bool DIL, bool ALIGNED, int RC = 0, int TW = FT, int TH = FT, bool OPQ = false>
 __device__ __forceinline__ void sq_tile_passes(uint8_t *smem, uint8_t *__restrict__ dst_img, int H, int W, int r_rt,
                                                int y0, int x0)
 {
@@ -332,7 +333,7 @@ __device__ __forceinline__ void sq_tile_passes(uint8_t *smem, uint8_t *__restric
   const int r = RC > 0 ? RC : r_rt;
   uint8_t *A = smem;
   uint8_t *T = smem;
-  const int tid = threadIdx.x;
+  const int tid = tile_tid<OPQ>();
   // ---- 2. vertical pass.  Every thread keeps the (transposed) results of its at most
   // MAXT tasks in registers; after a barrier they are written into T, which reuses the
   // storage of A.  Aliasing the two buffers halves the LDS footprint and doubles the
