@@ -79,7 +79,7 @@ template <int R>
 __global__ __launch_bounds__(FTHREADS) void batch_fused_kernel(const BatchImage *__restrict__ im, int nimg, int ndil,
                                                                int nitems, int *queue)
 {
-  constexpr int K = sq_k_of(R), TW = sq_tw(K, R), FO = open_fo(R);
+  constexpr int K = sq_k_open(R), TW = sq_tw(K, R), FO = open_fo(R);  // K: the opening tile's runs
   static_assert(R <= BATCH_FUSED_RMAX && TW == OPEN_TW, "fused batch geometry");
   extern __shared__ __align__(16) uint8_t smem[];
   __shared__ int s_item;

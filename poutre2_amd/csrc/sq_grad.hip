@@ -49,8 +49,8 @@ typedef void (*grad_kernel_t)(const uint8_t *, uint8_t *, int, int, int, int, in
 
 template <int... RS> static grad_kernel_t grad_pick(int r, bool aligned, std::integer_sequence<int, RS...>)
 {
-  static constexpr grad_kernel_t al[] = {sq_grad_kernel<sq_k_of(RS + 1), true, RS + 1>...};
-  static constexpr grad_kernel_t any[] = {sq_grad_kernel<sq_k_of(RS + 1), false, RS + 1>...};
+  static constexpr grad_kernel_t al[] = {sq_grad_kernel<sq_k_open(RS + 1), true, RS + 1>...};
+  static constexpr grad_kernel_t any[] = {sq_grad_kernel<sq_k_open(RS + 1), false, RS + 1>...};
   return aligned ? al[r - 1] : any[r - 1];
 }
 

@@ -37,7 +37,7 @@ constexpr int64_t SQ_U8_OPEN_RMAX = 7;
 constexpr int OPEN_TW = 256;
 
 constexpr int open_pad(int r) { return 2 * r <= 16 ? 16 : ((2 * r + 15) / 16) * 16; }
-constexpr int open_fo(int r) { return (FT - 2 * r) & ~(sq_k_of(r) - 1); }
+constexpr int open_fo(int r) { return (FT - 2 * r) & ~(sq_k_open(r) - 1); }
 inline size_t sq_open_lds_bytes(int r) { return (size_t)(FT + 2 * r) * (OPEN_TW + 2 * open_pad(r)); }
 
 // N: natural-layout band (128 rows, row pitch TW).  Word w of row R is stored at

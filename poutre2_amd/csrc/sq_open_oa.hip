@@ -8,7 +8,7 @@ namespace pm {
 
 template <int... RS> static open_kernel_t pick(int r, std::integer_sequence<int, RS...>)
 {
-  static constexpr open_kernel_t tab[] = {sq_open_kernel<sq_k_of(RS + 1), true, true, RS + 1, OPEN_TW>...};
+  static constexpr open_kernel_t tab[] = {sq_open_kernel<sq_k_open(RS + 1), true, true, RS + 1, OPEN_TW>...};
   return tab[r - 1];
 }
 

@@ -20,8 +20,9 @@
 //  3. Horizontal pass = the same vertical code over the transposed buffer; the 4x4 byte
 //     transpose back to row order is done in registers (v_perm_b32) and the tile is
 //     stored with 128-bit (or 32-bit for K < 16) global stores.
-// K (run length) is the largest power of two <= min(2r, 32): the suffix covers rows
-// [j, K) and the prefix rows [K, j+2r] of output j, which needs K <= 2r + 1.  For r = 1
+// K (run length, sq_k_of) is 4 for r <= 3, 16 for r <= 15 and 32 above.  With K <= 2r + 1
+// the suffix covers rows [j, K) and the prefix rows [K, j+2r] of output j; for 4 <= r <= 7
+// the 16-row run is two blocks of 8 rows split at rows 8 and 16 (needs 2r >= 8).  For r = 1
 // a direct 3-tap form with K = 4 is used.
 // The transposed buffer T reuses the LDS of the region A (results of the vertical pass
 // are held in registers across a barrier), so a workgroup needs max(|A|, |T|) bytes.

@@ -125,10 +125,20 @@ __device__ __forceinline__ uint4 load_edge_chunk(const uint8_t *__restrict__ row
   return make_uint4(wd[0], wd[1], wd[2], wd[3]);
 }
 
-// Radius r and run length K of the fused square kernel: K is the largest power of two
-// <= min(2r, 32) (4 for r = 1), and the region is padded by R16 = 16 columns on each side
-// for K <= 16 (r <= 15) and by 32 for K = 32 (16 <= r <= 32).
-constexpr int sq_k_of(int r) { return r >= 16 ? 32 : r >= 8 ? 16 : r >= 4 ? 8 : 4; }
+// Radius r and run length K (output rows per vertical-pass task) of the fused square
+// kernels: 4 for r <= 3, 16 for 4 <= r <= 15 (for r < 8 as two blocks of 8 rows sharing
+// their loads, see vh_run), 32 above; the region is padded by R16 = 16 columns on each side
+// for K <= 16 (r <= 15) and by 32 for K = 32 (16 <= r <= 32).  Runs of 16 rather than 8
+// rows for r = 4..7 read 8 + 4r rows per 16 outputs instead of 16 + 4r and halve the tasks:
+// 8192^2 dilation, us, 8-row vs 16-row runs: r = 4 34.9 / 32.4, r = 5 35.7 / 32.9,
+// r = 6 36.5 / 33.1, r = 7 36.7 / 33.9.  Every kernel instantiates K with a compile-time
+// radius RC, which the two-block form needs.
+constexpr int sq_k_of(int r) { return r >= 16 ? 32 : r >= 4 ? 16 : 4; }
+// Run length of the four-pass fused opening / closing and of the fused gradient: 8 for
+// r = 4..7, whose results they hold in more registers (8192^2, us, 8-row vs 16-row runs:
+// opening r = 4 54.3-55.8 / 57.1-57.8, r = 7 60.8-63.5 / 61.6-64.8; gradient r = 7
+// 58.6-60.7 / 60.4-63.0).
+constexpr int sq_k_open(int r) { return r >= 16 ? 32 : r >= 8 ? 16 : r >= 4 ? 8 : 4; }
 template <int K> constexpr int sq_r16() { return K == 32 ? 32 : 16; }
 // Tile width (output columns): 256 up to r = 16, 128 above (RC = 0: run-time radius of
 // run length K).  Wider tiles make longer contiguous row segments for HBM; above r = 16
@@ -161,7 +171,67 @@ __device__ __forceinline__ void vh_run(LD &&row, int r_rt, uint32_t (&e)[K], uin
 {
   const uint32_t NEU = DIL ? 0u : 0xFFFFFFFFu;
   const int r = RC > 0 ? RC : r_rt;
-  if (K > 4 || 2 * r >= K) {
+  if constexpr (RC > 0 && K == 16 && 2 * RC < K) {
+    // 16-row run for 4 <= r <= 7 as two blocks of 8 sharing their loads: outputs 0..7 take
+    // the suffix over rows j..7 and the prefix over rows 8..j+2r, outputs 8..15 the suffix
+    // over rows j..15 and the prefix over rows 16..j+2r (both splits need 2r >= 8)
+    static_assert(2 * RC >= 8, "two-block runs need 2r >= 8");
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      const uint32_t w = row(j);
+      e[j] = even_hi(w);
+      o[j] = odd_hi(w);
+    }
+    uint32_t pe = NEU, po = NEU;
+#pragma unroll
+    for (int t = 8; t < 2 * RC; t++) {  // prefix rows every output of block 1 shares
+      pe = pk<DIL>(pe, e[t]);
+      po = pk<DIL>(po, o[t]);
+    }
+#pragma unroll
+    for (int j = 6; j >= 0; j--) {
+      e[j] = pk<DIL>(e[j], e[j + 1]);
+      o[j] = pk<DIL>(o[j], o[j + 1]);
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int t = 2 * RC + j;  // rows 8..15 are still the raw rows here
+      uint32_t we, wo;
+      if (t < 16) {
+        we = e[t];
+        wo = o[t];
+      } else {
+        const uint32_t w = row(t);
+        we = even_hi(w);
+        wo = odd_hi(w);
+      }
+      pe = pk<DIL>(pe, we);
+      po = pk<DIL>(po, wo);
+      e[j] = pk<DIL>(e[j], pe);
+      o[j] = pk<DIL>(o[j], po);
+    }
+#pragma unroll
+    for (int j = 14; j >= 8; j--) {
+      e[j] = pk<DIL>(e[j], e[j + 1]);
+      o[j] = pk<DIL>(o[j], o[j + 1]);
+    }
+    pe = NEU;
+    po = NEU;
+#pragma unroll
+    for (int t = 16; t < 8 + 2 * RC; t++) {  // prefix rows every output of block 2 shares
+      const uint32_t w = row(t);
+      pe = pk<DIL>(pe, even_hi(w));
+      po = pk<DIL>(po, odd_hi(w));
+    }
+#pragma unroll
+    for (int j = 8; j < 16; j++) {
+      const uint32_t w = row(2 * RC + j);
+      pe = pk<DIL>(pe, even_hi(w));
+      po = pk<DIL>(po, odd_hi(w));
+      e[j] = pk<DIL>(e[j], pe);
+      o[j] = pk<DIL>(o[j], po);
+    }
+  } else if (K > 4 || 2 * r >= K) {
 #pragma unroll
     for (int j = 0; j < K; j++) {
       const uint32_t w = row(j);
