@@ -11,7 +11,8 @@
 //   gx[p] = sum over taps t of [sel(p - t) == t] * gy[p - t]
 //   gw[t] = +-sum over q of [sel(q) == t] * gy[q]      (+ for dilation, - for erosion)
 // gw is reduced per workgroup (re-selecting backward) or per wave (stored-tap backward) into a
-// partials buffer, then summed over those in a fixed order by a second kernel.
+// partials buffer, then summed over those in a fixed order (gw_reduce: many partials first in
+// contiguous ranges, one workgroup each, then per tap).
 // Memory per pixel: forward reads x, writes y (8 B); the re-selecting backward (pm_maxplus_bwd,
 // LDS tiles, taps in raster order) reads x and gy and writes gx (12 B).  The _sel forms store
 // the selected tap of every output in the forward (9 B) and read it in the backward instead
@@ -488,6 +489,33 @@ __global__ __launch_bounds__(MTHREADS) void mp_bwd_flat_kernel(const uint8_t *__
   }
 }
 
+// First stage of the gw reduction over many partials: workgroup j sums the partials of the
+// contiguous range of blocks [j*n/G, (j+1)*n/G) for every tap into part2[j][k].  Thread t
+// owns tap t % KK and every (256 / KK)-th block of the range, so consecutive threads read
+// consecutive floats; the per-thread sums are added in thread order.  Fixed order
+// throughout: the same bits on every run.
+__global__ __launch_bounds__(256) void gw_reduce1_kernel(const float *__restrict__ part, int nblocks, int KK,
+                                                         float *__restrict__ part2)
+{
+  __shared__ float red[256];
+  const int j = blockIdx.x, G = gridDim.x;
+  const int b0 = (int)((int64_t)j * nblocks / G), b1 = (int)((int64_t)(j + 1) * nblocks / G);
+  const int per = 256 / KK, t = threadIdx.x, k = t % KK, g = t / KK;
+  float acc = 0.0f;
+  if (g < per)
+    for (int b = b0 + g; b < b1; b += per) acc += part[(int64_t)b * KK + k];
+  red[t] = acc;
+  __syncthreads();
+  if (t < KK) {
+    float s = 0.0f;
+    for (int u = 0; u < per; u++) s += red[u * KK + t];
+    part2[(int64_t)j * KK + t] = s;
+  }
+}
+
+// partials per first-stage workgroup; above GW_DIRECT partials the reduction runs in two stages
+constexpr int GW_STAGE1 = 256, GW_DIRECT = 2048;
+
 // gw[k] = sum over blocks b (in order) of part[b][k]: one workgroup per tap, fixed-order tree
 __global__ __launch_bounds__(256) void gw_reduce_kernel(const float *__restrict__ part, int nblocks, int KK,
                                                         float *__restrict__ gw)
@@ -503,6 +531,28 @@ __global__ __launch_bounds__(256) void gw_reduce_kernel(const float *__restrict_
     __syncthreads();
   }
   if (threadIdx.x == 0) gw[k] = red[0];
+}
+
+// floats of scratch the gw reduction of n partials of KK taps needs (partials + first stage)
+static size_t gw_scratch_floats(int64_t n, int KK)
+{
+  return (size_t)n * KK + (n > GW_DIRECT ? (size_t)GW_STAGE1 * KK : 0);
+}
+
+// gw from n per-block partials at part (scratch of gw_scratch_floats(n, KK) floats)
+static int gw_reduce(float *part, int n, int KK, float *gw, hipStream_t s)
+{
+  const float *src = part;
+  int m = n;
+  if (n > GW_DIRECT) {
+    float *part2 = part + (size_t)n * KK;
+    hipLaunchKernelGGL(gw_reduce1_kernel, dim3(GW_STAGE1), dim3(256), 0, s, part, n, KK, part2);
+    src = part2;
+    m = GW_STAGE1;
+  }
+  hipLaunchKernelGGL(gw_reduce_kernel, dim3(KK), dim3(256), 0, s, src, m, KK, gw);
+  PM_HIP(hipGetLastError());
+  return PM_OK;
 }
 
 static int mp_check(int64_t B, int64_t H, int64_t W, int K, const void *a, const void *b)
@@ -614,7 +664,7 @@ extern "C" int pm_maxplus_bwd_sel(const uint8_t *sel, const float *gy, int64_t B
     return PM_OK;
   }
   Scratch part;
-  if (with_w) PM_TRY(part.alloc(sizeof(float) * (size_t)nwaves * K * K, s));
+  if (with_w) PM_TRY(part.alloc(sizeof(float) * gw_scratch_floats(nwaves, K * K), s));
   float *pp = (float *)part.ptr;
 #define L(KK)                                                                                                \
   do {                                                                                                       \
@@ -625,10 +675,7 @@ extern "C" int pm_maxplus_bwd_sel(const uint8_t *sel, const float *gy, int64_t B
   PM_MP_DISPATCH(L)
 #undef L
   PM_HIP(hipGetLastError());
-  if (with_w) {
-    hipLaunchKernelGGL(gw_reduce_kernel, dim3(K * K), dim3(256), 0, s, pp, nwaves, K * K, gw);
-    PM_HIP(hipGetLastError());
-  }
+  if (with_w) return gw_reduce(pp, nwaves, K * K, gw, s);
   return PM_OK;
 }
 
@@ -649,7 +696,7 @@ extern "C" int pm_maxplus_bwd(const float *x, const float *gy, int64_t B, int64_
   }
   if (nb > INT32_MAX) PM_FAIL(PM_E_SIZE, "maxplus: batch too large");
   Scratch part;
-  if (!flat) PM_TRY(part.alloc(sizeof(float) * (size_t)nb * K * K, s));
+  if (!flat) PM_TRY(part.alloc(sizeof(float) * gw_scratch_floats(nb, K * K), s));
   float *pp = (float *)part.ptr;
 #define L(KK)                                                                                                \
   do {                                                                                                       \
@@ -664,9 +711,6 @@ extern "C" int pm_maxplus_bwd(const float *x, const float *gy, int64_t B, int64_
   PM_MP_DISPATCH(L)
 #undef L
   PM_HIP(hipGetLastError());
-  if (!flat) {
-    hipLaunchKernelGGL(gw_reduce_kernel, dim3(K * K), dim3(256), 0, s, pp, (int)nb, K * K, gw);
-    PM_HIP(hipGetLastError());
-  }
+  if (!flat) return gw_reduce(pp, (int)nb, K * K, gw, s);
   return PM_OK;
 }
