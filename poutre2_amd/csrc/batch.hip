@@ -3,14 +3,16 @@
 // pm_batch_dilate_open computes, for every image i of a batch,
 //   dil[i]  = Dilate(in[i], SESquare2D, r_i)                      (ero_dil_static.cpp:268-305)
 //   open[i] = Dilate(Erode(in[i], SESquare2D, r_i), SESquare2D, r_i)
-// The images are grouped by radius.  Per group up to r = BATCH_FUSED_RMAX (15), one persistent
-// launch walks a work list spanning all its images: the dilation tiles, then the fused
-// opening tiles (sq_open.hpp: erosion and dilation in LDS, nothing in between goes through
-// memory).  Larger radii take two launches: items for the dilation and for the erosion into
-// a scratch image, then the dilation of the eroded tiles (stream order separates them).
-// Every item runs the tile code of the single-image fused kernels specialised for the
-// group's radius, with loads and stores for any width and alignment.
+// The images are grouped by radius.  All groups up to r = BATCH_FUSED_RMAX (15) run in ONE
+// persistent launch (batch_one_kernel) whose work list spans all their images: per group the
+// fused opening tiles (sq_open.hpp: erosion and dilation in LDS, nothing in between goes
+// through memory), then the dilation tiles.  Larger radii take two launches per group: items
+// for the dilation and for the erosion into a scratch image, then the dilation of the eroded
+// tiles (stream order separates them).  Every item runs the tile code of the single-image
+// fused kernels specialised for its group's radius, with loads and stores for any width and
+// alignment.
 // The reference runs the two Erode/Dilate calls per image, one image at a time.
+#include <cstring>
 #include <vector>
 
 #include "batch_tile.hpp"
@@ -38,7 +40,11 @@ struct BatchLayout {
   int64_t tmp_bytes = 0;
   size_t first[NG] = {};
   std::vector<BatchImage> all;
-  int *queue = nullptr;  // per group: the next item of the fused kernel (scratch)
+  std::vector<BatchGroup> one;  // groups of the one-launch kernel, in work-list order
+  int64_t one_items = 0;
+  size_t one_lds = 0;
+  BatchGroup *dgrp = nullptr;  // device copy of `one` (scratch)
+  int *queue = nullptr;        // the one-launch kernel's queue counters (scratch, 2 ints)
 };
 
 static int batch_layout(int n, const pm_image *ins, const int *radii, pm_image *dil_outs, pm_image *open_outs,
@@ -77,24 +83,27 @@ static int batch_layout(int n, const pm_image *ins, const int *radii, pm_image *
       L.tmp_bytes += (H * W + 255) & ~(int64_t)255;
     }
     if (L.tiles[g] + L.otiles[g] > INT32_MAX / 4) PM_FAIL(PM_E_SIZE, "batch: too many tiles");
+    int64_t fused_items = 0;  // the one-launch kernel's work list spans all groups up to BATCH_FUSED_RMAX
+    for (int k = 0; k < BATCH_FUSED_RMAX; k++) fused_items += L.tiles[k] + L.otiles[k];
+    if (fused_items > INT32_MAX / 4) PM_FAIL(PM_E_SIZE, "batch: too many tiles");
     L.pixels += H * W;
     L.grp[g].push_back(b);
   }
   return PM_OK;
 }
 
-// Scratch layout: eroded images (256-byte aligned each), the descriptor table, then one queue
-// counter per group (256-byte aligned).
+// Scratch layout: eroded images (256-byte aligned each), then the upload area: the
+// descriptor table, the group table of the one-launch kernel and its two queue counters
+// (each part 256-byte aligned).
 static size_t batch_desc_bytes(const BatchLayout &L)
 {
   size_t d = 0;
   for (int g = 0; g < BatchLayout::NG; g++) d += L.grp[g].size() * sizeof(BatchImage);
   return (d + 255) & ~(size_t)255;
 }
-static size_t batch_scratch_bytes(const BatchLayout &L)
-{
-  return (size_t)L.tmp_bytes + batch_desc_bytes(L) + BatchLayout::NG * sizeof(int);
-}
+constexpr size_t BATCH_GRP_BYTES = (BatchLayout::NG * sizeof(BatchGroup) + 255) & ~(size_t)255;
+static size_t batch_upload_bytes(const BatchLayout &L) { return batch_desc_bytes(L) + BATCH_GRP_BYTES + 2 * sizeof(int); }
+static size_t batch_scratch_bytes(const BatchLayout &L) { return (size_t)L.tmp_bytes + batch_upload_bytes(L); }
 
 // point the descriptors at scratch `base` and build the flat host table
 static BatchImage *batch_bind(BatchLayout &L, uint8_t *base)
@@ -111,39 +120,64 @@ static BatchImage *batch_bind(BatchLayout &L, uint8_t *base)
     L.first[g] = L.all.size();
     L.all.insert(L.all.end(), L.grp[g].begin(), L.grp[g].end());
   }
-  L.queue = (int *)(base + L.tmp_bytes + batch_desc_bytes(L));
-  return (BatchImage *)(base + L.tmp_bytes);
+  // the one-launch work list: groups of decreasing radius (heaviest tiles first)
+  L.one.clear();
+  L.one_items = 0;
+  L.one_lds = 0;
+  for (int g = BATCH_FUSED_RMAX - 1; g >= 0; g--) {
+    if (L.grp[g].empty()) continue;
+    BatchGroup G;
+    G.r = g + 1;
+    G.img0 = (int)L.first[g];
+    G.nimg = (int)L.grp[g].size();
+    G.item0 = (int)L.one_items;
+    G.nopen = (int)L.otiles[g];
+    G.nitems = (int)(L.otiles[g] + L.tiles[g]);
+    L.one.push_back(G);
+    L.one_items += G.nitems;
+    const size_t lds = batch_one_lds_bytes(G.r);
+    if (lds > L.one_lds) L.one_lds = lds;
+  }
+  uint8_t *up = base + L.tmp_bytes;
+  L.dgrp = (BatchGroup *)(up + batch_desc_bytes(L));
+  L.queue = (int *)(up + batch_desc_bytes(L) + BATCH_GRP_BYTES);
+  return (BatchImage *)up;
 }
 
-// enqueue every radius group (kernel launches only, so that the sequence can be captured
-// into a graph): one fused launch for r <= BATCH_FUSED_RMAX, two phases above.  Group j goes
-// to stream ss[j % ns]; the groups are independent, so on several streams their launches
-// overlap (the tail of one fills with the work of another).
+// host image of the upload area (descriptor table, group table, zeroed queue counters)
+static std::vector<uint8_t> batch_upload(const BatchLayout &L)
+{
+  std::vector<uint8_t> h(batch_upload_bytes(L), 0);
+  if (!L.all.empty()) memcpy(h.data(), L.all.data(), L.all.size() * sizeof(BatchImage));
+  if (!L.one.empty()) memcpy(h.data() + batch_desc_bytes(L), L.one.data(), L.one.size() * sizeof(BatchGroup));
+  return h;
+}
+
+// Enqueue the batch (kernel launches only, so that the sequence can be captured into a
+// graph): the one-launch kernel over every group up to BATCH_FUSED_RMAX, whose queue counters
+// must be zero (they are uploaded as zeros and the kernel resets them when it ends), and two
+// launches per larger radius.  Launches go to the streams ss[0 .. ns-1] round robin: the
+// two-launch groups are independent of the rest, so on several streams they overlap.
 static int batch_enqueue(const BatchLayout &L, const BatchImage *dimg, const hipStream_t *ss, int ns)
 {
   int dev = 0, cus = 0;
   PM_HIP(hipGetDevice(&dev));
   PM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   int j = 0;
-  for (int g = 0; g < BatchLayout::NG; g++) {
+  if (L.one_items > 0) {
+    const hipStream_t s = ss[j++ % ns];
+    int occ = 0;
+    PM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, batch_one_kernel, FTHREADS, L.one_lds));
+    int64_t G = (int64_t)(occ > 0 ? occ : 1) * cus;
+    if (G > L.one_items) G = L.one_items;
+    hipLaunchKernelGGL(batch_one_kernel, dim3((unsigned)G), dim3(FTHREADS), L.one_lds, s, dimg, L.dgrp,
+                       (int)L.one.size(), (int)L.one_items, L.queue);
+    PM_HIP(hipGetLastError());
+  }
+  for (int g = BATCH_FUSED_RMAX; g < BatchLayout::NG; g++) {
     if (L.grp[g].empty()) continue;
     const hipStream_t s = ss[j++ % ns];
     const int r = g + 1;
-    if (r <= BATCH_FUSED_RMAX) {
-      const batch_fused_t k = batch_fused_for(r);
-      const size_t l1 = sq_tile_lds_bytes(r, sq_tw(sq_k_of(r), r)), l2 = sq_open_lds_bytes(r);
-      const size_t lds = l1 > l2 ? l1 : l2;
-      int occ = 0;
-      PM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, FTHREADS, lds));
-      const int64_t items = L.tiles[g] + L.otiles[g];
-      int64_t G = (int64_t)(occ > 0 ? occ : 1) * cus;
-      if (G > items) G = items;
-      PM_TRY(fill_bytes(L.queue + g, 0, sizeof(int), s));  // a kernel: safe under capture
-      hipLaunchKernelGGL(k, dim3((unsigned)G), dim3(FTHREADS), lds, s, dimg + L.first[g], (int)L.grp[g].size(),
-                         (int)L.tiles[g], (int)items, L.queue + g);
-      PM_HIP(hipGetLastError());
-      continue;
-    }
     for (int phase = 0; phase < 2; phase++) {
       const batch_kernel_t k = batch_kernel(r, phase);
       const size_t lds = sq_tile_lds_bytes(r, sq_tw(sq_k_of(r), r));
@@ -175,7 +209,8 @@ extern "C" int pm_batch_dilate_open(int n, const pm_image *ins, const int *radii
   Scratch sc;
   PM_TRY(sc.alloc(batch_scratch_bytes(L), s));
   BatchImage *dimg = batch_bind(L, (uint8_t *)sc.ptr);
-  PM_HIP(hipMemcpyAsync(dimg, L.all.data(), L.all.size() * sizeof(BatchImage), hipMemcpyHostToDevice, s));
+  const std::vector<uint8_t> up = batch_upload(L);
+  PM_HIP(hipMemcpyAsync(dimg, up.data(), up.size(), hipMemcpyHostToDevice, s));
   return batch_enqueue(L, dimg, &s, 1);
 }
 
@@ -222,16 +257,18 @@ extern "C" int pm_batch_plan_create(int n, const pm_image *ins, const int *radii
     PM_FAIL(PM_E_HIP, "batch plan: scratch allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
   }
   BatchImage *dimg = batch_bind(L, (uint8_t *)p->scratch);
-  if (!L.all.empty()) {
-    e = hipMemcpy(dimg, L.all.data(), L.all.size() * sizeof(BatchImage), hipMemcpyHostToDevice);
+  {
+    const std::vector<uint8_t> up = batch_upload(L);
+    e = hipMemcpy(dimg, up.data(), up.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
       plan_free(p);
       PM_FAIL(PM_E_HIP, "batch plan: descriptor upload failed: %s", hipGetErrorString(e));
     }
   }
   // capture the launches on private streams (default priority): the capture stream forks
-  // into NB branches (one per group, round robin) that join back before the capture ends,
-  // so the graph's radius groups are independent nodes that may run concurrently
+  // into NB branches (the one-launch kernel, then one per two-launch group, round robin)
+  // that join back before the capture ends, so they are independent graph nodes that may run
+  // concurrently
   constexpr int NB = 4;
   hipStream_t st[NB] = {};
   hipEvent_t fork = nullptr, join[NB] = {};

@@ -63,55 +63,62 @@ __global__ __launch_bounds__(FTHREADS) void batch_sq_kernel(const BatchImage *__
   }
 }
 
-// Radius groups up to this radius run the one-launch form (batch_fused_kernel).  The fused
+// Radius groups up to this radius run in the one-launch kernel (batch_one_kernel).  The fused
 // opening tile is used up to r = 15 here although single images switch to the two-launch
-// composition above r = 7 (sq_open.hpp): in a batch the composition's second launch per
-// radius group costs more than the fused tile's larger halo.
+// composition above r = 7 (sq_open.hpp): in a batch the composition's second launch costs
+// more than the fused tile's larger halo.
 constexpr int BATCH_FUSED_RMAX = 15;
 
-// Radius groups up to BATCH_FUSED_RMAX: one launch over ndil dilation tiles and nitems - ndil
-// fused-opening tiles (sq_open.hpp) of the same images, so no eroded image goes through
-// memory and no second launch is needed.  Workgroups take items from a queue (*queue: the
-// next item, zero at launch): a workgroup that becomes resident late, when other groups'
-// kernels share the GPU, takes fewer items instead of a fixed share.  The opening tiles, the
-// heavier items, come first.
+// A radius group of the one-launch kernel (batch_one_kernel): its images are im[img0 ..
+// img0 + nimg - 1] of the descriptor table, its items are [item0, item0 + nitems) of the
+// launch's work list: first its nopen fused-opening tiles, then its dilation tiles.
+struct BatchGroup {
+  int r, img0, nimg, item0, nopen, nitems;
+};
+
+// Item t of a radius group up to BATCH_FUSED_RMAX: t < nopen is fused-opening tile t
+// (sq_open.hpp: erosion and dilation in LDS, no eroded image in memory), otherwise
+// dilation tile t - nopen.  im / nimg: the group's images.
 template <int R>
-__global__ __launch_bounds__(FTHREADS) void batch_fused_kernel(const BatchImage *__restrict__ im, int nimg, int ndil,
-                                                               int nitems, int *queue)
+__device__ __forceinline__ void batch_fused_item(uint8_t *smem, const BatchImage *__restrict__ im, int nimg, int nopen,
+                                                 int t)
 {
   constexpr int K = sq_k_open(R), TW = sq_tw(K, R), FO = open_fo(R);  // K: the opening tile's runs
   static_assert(R <= BATCH_FUSED_RMAX && TW == OPEN_TW, "fused batch geometry");
-  extern __shared__ __align__(16) uint8_t smem[];
-  __shared__ int s_item;
-  const int nopen = nitems - ndil;
-  for (;;) {
-    __syncthreads();  // every thread has read the previous item
-    if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
+  if (t >= nopen) {
+    const int d = t - nopen;
+    const BatchImage b = im[batch_find(im, nimg, d)];
+    const int tl = d - b.tile0, ty = tl / b.tiles_x, tx = tl - ty * b.tiles_x;
+    batch_tile<R, true>(smem, b.in, b.dil, b, ty * FT, tx * TW);
+  } else {
+    const BatchImage b = im[batch_find<true>(im, nimg, t)];
+    const int tl = t - b.otile0, ty = tl / b.tiles_x, tx = tl - ty * b.tiles_x;
+    const int y0 = ty * FO, x0 = tx * OPEN_TW;
+    sq_tile_load<K, false, false, R, OPEN_TW, open_pad(R), true>(smem, b.in, b.H, b.W, R, y0 - R, x0);
     __syncthreads();
-    const int item = s_item;
-    if (item >= nitems) break;
-    if (item >= nopen) {
-      const int d = item - nopen;
-      const BatchImage b = im[batch_find(im, nimg, d)];
-      const int tl = d - b.tile0, ty = tl / b.tiles_x, tx = tl - ty * b.tiles_x;
-      batch_tile<R, true>(smem, b.in, b.dil, b, ty * FT, tx * TW);
-    } else {
-      const int t = item;
-      const BatchImage b = im[batch_find<true>(im, nimg, t)];
-      const int tl = t - b.otile0, ty = tl / b.tiles_x, tx = tl - ty * b.tiles_x;
-      const int y0 = ty * FO, x0 = tx * OPEN_TW;
-      sq_tile_load<K, false, false, R, OPEN_TW, open_pad(R), true>(smem, b.in, b.H, b.W, R, y0 - R, x0);
-      __syncthreads();
-      sq_open_tile<K, true, false, R, OPEN_TW>(smem, b.open, b.H, b.W, y0, x0);
-      __syncthreads();  // LDS is rewritten by the next item
-    }
+    sq_open_tile<K, true, false, R, OPEN_TW>(smem, b.open, b.H, b.W, y0, x0);
+    __syncthreads();  // LDS is rewritten by the next item
   }
 }
 
+// One persistent launch over every radius group up to BATCH_FUSED_RMAX (batch_open.hip).
+// Workgroups take items from a queue: ctr[0] is the next item, ctr[1] counts the workgroups
+// that found the queue empty; both are zero at launch, and the last workgroup to finish
+// resets them, so the launch can be replayed (graph plans) without clearing them first.
+// Groups come in order of decreasing radius (the heavier tiles first, the cheapest last, so
+// the launch's tail is short), each group's opening tiles before its dilation tiles; at any
+// time the resident workgroups mostly run one radius, i.e. one specialisation of the code.
+// Dynamic LDS: the largest tile of the launch's radii (batch_one_lds_bytes).
+__global__ __launch_bounds__(FTHREADS) void batch_one_kernel(const BatchImage *__restrict__ im,
+                                                             const BatchGroup *__restrict__ grp, int ngrp,
+                                                             int nitems, int *ctr);
+inline size_t batch_one_lds_bytes(int r)
+{
+  const size_t a = sq_tile_lds_bytes(r, sq_tw(sq_k_of(r), r)), b = sq_open_lds_bytes(r);
+  return a > b ? a : b;
+}
+
 typedef void (*batch_kernel_t)(const BatchImage *, int, int);
-typedef void (*batch_fused_t)(const BatchImage *, int, int, int, int *);
-// fused kernel of radius r in [1, BATCH_FUSED_RMAX] (batch_open.hip)
-batch_fused_t batch_fused_for(int r);
 
 // kernel of radius r in [LO, LO + 7] and phase 0 / 1 (batch_r*.hip)
 template <int LO> batch_kernel_t batch_kernel_range(int r, int phase);
