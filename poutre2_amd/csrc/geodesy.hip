@@ -241,41 +241,75 @@ static int leveling_half(const T *ref, const T *marker, T *out, Dims d, int se, 
 }
 
 // Order of the marker g against the reference f: bit 0 set when some g > f, bit 1 when some
-// g < f (both set when a comparison fails, e.g. NaN).  One atomicOr per wave.
+// g < f (both set when a comparison fails, e.g. NaN).  Grid-stride over 16-byte chunks with
+// four chunks of each image in flight per thread; one atomicOr per workgroup.
 template <typename T>
 __global__ __launch_bounds__(256) void order_kernel(const T *__restrict__ f, const T *__restrict__ g, int64_t n,
                                                     int *__restrict__ bits)
 {
   constexpr int E = 16 / (int)sizeof(T);  // elements per 16-byte chunk
+  constexpr int U = 4;                     // chunks in flight per thread and image
   const int64_t nchunks = n / E;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int b = 0;
-  for (int64_t c = tid; c < nchunks; c += stride) {
-    const uint4 fa = load16_any(reinterpret_cast<const uint8_t *>(f + c * E));
-    const uint4 ga = load16_any(reinterpret_cast<const uint8_t *>(g + c * E));
-    T fv[E], gv[E];
-    __builtin_memcpy(fv, &fa, 16);
-    __builtin_memcpy(gv, &ga, 16);
+  auto cmp = [&](T gv, T fv) {
+    if (!(gv <= fv)) b |= 1;
+    if (!(gv >= fv)) b |= 2;
+  };
+  for (int64_t c0 = tid; c0 < nchunks; c0 += U * stride) {
+    uint4 fa[U], ga[U];
 #pragma unroll
-    for (int e = 0; e < E; e++) {
-      if (!(gv[e] <= fv[e])) b |= 1;
-      if (!(gv[e] >= fv[e])) b |= 2;
+    for (int u = 0; u < U; u++) {
+      const int64_t c = c0 + u * stride;
+      if (c < nchunks) {
+        fa[u] = load16_any(reinterpret_cast<const uint8_t *>(f + c * E));
+        ga[u] = load16_any(reinterpret_cast<const uint8_t *>(g + c * E));
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (c0 + u * stride < nchunks) {
+        T fv[E], gv[E];
+        __builtin_memcpy(fv, &fa[u], 16);
+        __builtin_memcpy(gv, &ga[u], 16);
+#pragma unroll
+        for (int e = 0; e < E; e++) cmp(gv[e], fv[e]);
+      }
     }
   }
-  for (int64_t i = nchunks * E + tid; i < n; i += stride) {
-    const T a = f[i], c = g[i];
-    if (!(c <= a)) b |= 1;
-    if (!(c >= a)) b |= 2;
-  }
-  // one global atomic per block: atomics from every wave to one address serialise in L2
+  for (int64_t i = nchunks * E + tid; i < n; i += stride) cmp(g[i], f[i]);
+  // one global atomic per block, and only for bits not yet set: atomics to one address
+  // serialise (1024 blocks all setting the same bit cost about as much as the whole pass)
   __shared__ int blk;
   if (threadIdx.x == 0) blk = 0;
   __syncthreads();
   const bool gt = __any(b & 1), lt = __any(b & 2);
   if ((threadIdx.x & 63) == 0 && (gt || lt)) atomicOr(&blk, (gt ? 1 : 0) | (lt ? 2 : 0));
   __syncthreads();
-  if (threadIdx.x == 0 && blk) atomicOr(bits, blk);
+  if (threadIdx.x == 0 && blk) {
+    const int seen = __hip_atomic_load(bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((seen | blk) != seen) atomicOr(bits, blk);
+  }
+}
+
+// order bits of (f, g) ORed into device int *bits (zero), no host synchronisation
+template <typename T> static int marker_order_dev(const T *f, const T *g, int64_t n, int *bits, hipStream_t s)
+{
+  hipLaunchKernelGGL((order_kernel<T>), dim3(grid1d(n / (16 / (int64_t)sizeof(T)) / 4 + 1, 256, 1024)), dim3(256), 0,
+                     s, f, g, n, bits);
+  PM_HIP(hipGetLastError());
+  return PM_OK;
+}
+
+template <typename T> struct OrderArgs {
+  const T *f, *g;
+  int64_t n;
+};
+template <typename T> static int marker_order_launch(void *ctx, int *bits, hipStream_t s)
+{
+  const OrderArgs<T> *a = (const OrderArgs<T> *)ctx;
+  return marker_order_dev<T>(a->f, a->g, a->n, bits, s);
 }
 
 template <typename T> static int marker_order(const T *f, const T *g, int64_t n, int *bits, hipStream_t s)
@@ -283,9 +317,7 @@ template <typename T> static int marker_order(const T *f, const T *g, int64_t n,
   Scratch sc;
   PM_TRY(sc.alloc(sizeof(int), s));
   hipLaunchKernelGGL(zero_flag_kernel, dim3(1), dim3(64), 0, s, (int *)sc.ptr);
-  hipLaunchKernelGGL((order_kernel<T>), dim3(grid1d(n / (16 / (int64_t)sizeof(T)) + 1, 256, 1024)), dim3(256), 0, s, f, g, n,
-                     (int *)sc.ptr);
-  PM_HIP(hipGetLastError());
+  PM_TRY(marker_order_dev<T>(f, g, n, (int *)sc.ptr, s));
   static thread_local int *hbits = nullptr;  // pinned, allocated once per host thread
   if (!hbits) PM_HIP(hipHostMalloc((void **)&hbits, sizeof(int), hipHostMallocDefault));
   PM_HIP(hipMemcpyAsync(hbits, sc.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -316,6 +348,18 @@ static int leveling_entry(const pm_image *ref, const pm_image *marker, int se, p
       // With the marker on one side of the reference the closed forms collapse (see the
       // function comment): g <= f gives low(f, g) = leveling(f, g) = R^d_f(g) and
       // high(f, g) = f; g >= f gives high(f, g) = leveling(f, g) = R^e_f(g) and low(f, g) = f.
+      if constexpr (std::is_same<T, uint8_t>::value) {
+        // uint8 2D: assume g <= f (a marker from an opening, the common case) and run its
+        // reconstruction at once, guarded by the order bits on the device, so that no host
+        // round trip sits between the order check and the rounds; on a failed guess fall
+        // through to the general path
+        if (K != LEV_HIGH && rec_u8_applicable(se, d)) {
+          OrderArgs<T> oa{r, g, n};
+          bool aborted = false;
+          PM_TRY(rec_u8_guarded<true>(g, r, dst, d, se, marker_order_launch<T>, &oa, 1, &aborted, s));
+          if (!aborted) return PM_OK;
+        }
+      }
       int bits = 3;
       PM_TRY(marker_order<T>(r, g, n, &bits, s));
       const bool g_le_f = !(bits & 1), g_ge_f = !(bits & 2);

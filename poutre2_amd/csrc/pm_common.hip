@@ -123,7 +123,7 @@ bool take_forced_alloc_failure()
 int pinned_counters(int **h)
 {
   static thread_local int *counters = nullptr;
-  if (!counters) PM_HIP(hipHostMalloc((void **)&counters, sizeof(int) * PM_COUNTERS, hipHostMallocDefault));
+  if (!counters) PM_HIP(hipHostMalloc((void **)&counters, sizeof(int) * (PM_COUNTERS + 1), hipHostMallocDefault));
   *h = counters;
   return PM_OK;
 }

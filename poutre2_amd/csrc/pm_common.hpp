@@ -171,8 +171,9 @@ __device__ __forceinline__ void store2_any(uint8_t *p, uint32_t a) { *reinterpre
 int round_cap(int own_cap);
 bool take_forced_alloc_failure();
 
-// Pinned host memory for PM_COUNTERS round counters, allocated once per host thread (the
-// reconstructions read their convergence counters into it once per batch of rounds).
+// Pinned host memory for PM_COUNTERS round counters and one more int (index PM_COUNTERS: a
+// guard flag read with them), allocated once per host thread (the reconstructions read their
+// convergence counters into it once per batch of rounds).
 constexpr int PM_COUNTERS = 64;
 int pinned_counters(int **h);
 

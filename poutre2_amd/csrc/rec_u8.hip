@@ -149,9 +149,10 @@ __device__ __forceinline__ uint32_t to_other_layout(uint32_t (&v)[64], uint8_t *
 template <bool DIL, int NB>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void rec_u8_kernel(
     RecImage one, const RecImage *__restrict__ many, int nimg, int nt, int round, int *__restrict__ act,
-    int *__restrict__ moved)
+    int *__restrict__ moved, const int *__restrict__ guard, int gmask)
 {
   if (round > 0 && moved[round - 1] == 0) return;  // converged in an earlier round
+  if (guard && (*guard & gmask)) return;            // speculation failed: nothing to do
   // XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin, so XCD x (block
   // x mod 8) takes the contiguous tile range [x * per, (x + 1) * per): horizontally adjacent
   // tiles, which share cache lines, meet in the same L2
@@ -382,20 +383,35 @@ bool rec_u8_applicable(int se, Dims d)
 
 // ints of device scratch a reconstruction over nt tiles uses: one activity tag per tile, one
 // move counter per round
-static size_t rec_scratch_ints(int64_t nt) { return (size_t)nt + REC_MAX_ROUNDS; }
+static size_t rec_scratch_ints(int64_t nt, bool guarded = false) { return (size_t)nt + REC_MAX_ROUNDS + (guarded ? 1 : 0); }
 
 // Rounds driver shared by the single-image, batched and scratch-free entry points.  `sc` is
-// device scratch of rec_scratch_ints(nt) ints (cleared here by a fill kernel), `hmoved` host
-// memory for PM_COUNTERS counters (pinned, or any host memory when the caller cannot allocate).
+// device scratch of rec_scratch_ints(nt, gd != nullptr) ints (cleared here by a fill kernel),
+// `hmoved` host memory for PM_COUNTERS counters, and one more int with a guard (pinned, or any
+// host memory when the caller cannot allocate).
+// Optional guard: after the initialisation, guard->launch(guard->ctx, flag, s) enqueues the
+// work that computes the flag (zeroed by the initialisation); every round exits at once when
+// (flag & guard->mask) != 0, and the host then reports *guard->aborted = true.
+struct RecGuard {
+  int (*launch)(void *ctx, int *flag, hipStream_t s);
+  void *ctx;
+  int mask;
+  bool *aborted;
+};
+
 template <bool DIL>
 static int rec_rounds(const RecImage &one, const RecImage *many, int nimg, int nt, int se, int *sc, int *hmoved,
-                      hipStream_t s)
+                      hipStream_t s, const RecGuard *gd = nullptr)
 {
   int *act = sc;
   int *moved = act + nt;
-  const int ninit = nt > REC_MAX_ROUNDS ? nt : REC_MAX_ROUNDS;
-  hipLaunchKernelGGL(rec_init_kernel, dim3((ninit + 255) / 256), dim3(256), 0, s, act, nt, moved, REC_MAX_ROUNDS);
+  int *guard = gd ? moved + REC_MAX_ROUNDS : nullptr;  // zeroed with the counters
+  const int gmask = gd ? gd->mask : 0;
+  const int nm = REC_MAX_ROUNDS + (gd ? 1 : 0);
+  const int ninit = nt > nm ? nt : nm;
+  hipLaunchKernelGGL(rec_init_kernel, dim3((ninit + 255) / 256), dim3(256), 0, s, act, nt, moved, nm);
   PM_HIP(hipGetLastError());
+  if (gd) PM_TRY(gd->launch(gd->ctx, guard, s));
   int nbk = NBK_SQUARE;
   if (se == PM_SE_CROSS_2D) nbk = NBK_CROSS;
   else if (se == PM_SE_SEGMENT_X_1D || se == PM_SE_SEGMENT_X_2D) nbk = NBK_SEGX;
@@ -409,22 +425,31 @@ static int rec_rounds(const RecImage &one, const RecImage *many, int nimg, int n
       const int R = round + i;
       switch (nbk) {
       case NBK_SQUARE:
-        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SQUARE>), dim3(grid), dim3(64), 0, s, one, many, nimg, nt, R, act, moved);
+        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SQUARE>), dim3(grid), dim3(64), 0, s, one, many, nimg, nt, R, act, moved,
+                           guard, gmask);
         break;
       case NBK_CROSS:
-        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_CROSS>), dim3(grid), dim3(64), 0, s, one, many, nimg, nt, R, act, moved);
+        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_CROSS>), dim3(grid), dim3(64), 0, s, one, many, nimg, nt, R, act, moved,
+                           guard, gmask);
         break;
       case NBK_SEGX:
-        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SEGX>), dim3(grid), dim3(64), 0, s, one, many, nimg, nt, R, act, moved);
+        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SEGX>), dim3(grid), dim3(64), 0, s, one, many, nimg, nt, R, act, moved,
+                           guard, gmask);
         break;
       default:
-        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SEGY>), dim3(grid), dim3(64), 0, s, one, many, nimg, nt, R, act, moved);
+        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SEGY>), dim3(grid), dim3(64), 0, s, one, many, nimg, nt, R, act, moved,
+                           guard, gmask);
         break;
       }
     }
     PM_HIP(hipGetLastError());
     PM_HIP(hipMemcpyAsync(hmoved, moved + round, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
+    if (gd) PM_HIP(hipMemcpyAsync(hmoved + PM_COUNTERS, guard, sizeof(int), hipMemcpyDeviceToHost, s));
     PM_HIP(hipStreamSynchronize(s));
+    if (gd && (hmoved[PM_COUNTERS] & gmask)) {
+      *gd->aborted = true;
+      return PM_OK;
+    }
     for (int i = 0; i < nb; i++)
       if (hmoved[i] == 0) return PM_OK;
     round += nb;
@@ -466,6 +491,22 @@ int rec_u8(const uint8_t *marker, const uint8_t *mask, uint8_t *out, Dims d, int
   return rec_rounds_alloc<DIL>(one, nullptr, 1, one.tiles_x * one.tiles_y, se, s);
 }
 
+template <bool DIL>
+int rec_u8_guarded(const uint8_t *marker, const uint8_t *mask, uint8_t *out, Dims d, int se,
+                   int (*launch)(void *, int *, hipStream_t), void *ctx, int gmask, bool *aborted, hipStream_t s)
+{
+  *aborted = false;
+  RecImage one;
+  PM_TRY(rec_describe(marker, mask, out, d, 0, one));
+  const int nt = one.tiles_x * one.tiles_y;
+  Scratch sc;
+  PM_TRY(sc.alloc(sizeof(int) * rec_scratch_ints(nt, true), s));
+  int *h = nullptr;
+  PM_TRY(pinned_counters(&h));
+  const RecGuard gd{launch, ctx, gmask, aborted};
+  return rec_rounds<DIL>(one, nullptr, 1, nt, se, (int *)sc.ptr, h, s, &gd);
+}
+
 // ---------------------------------------------------------------- scratch-free reconstruction
 // Device scratch of the scratch-free entry point: one array per GPU, held until the process
 // ends.  Host code never uses the array's name as an address; it uses the address that
@@ -494,6 +535,10 @@ static int noalloc_scratch(int **ptr)
 
 template int rec_u8<true>(const uint8_t *, const uint8_t *, uint8_t *, Dims, int, hipStream_t);
 template int rec_u8<false>(const uint8_t *, const uint8_t *, uint8_t *, Dims, int, hipStream_t);
+template int rec_u8_guarded<true>(const uint8_t *, const uint8_t *, uint8_t *, Dims, int,
+                                  int (*)(void *, int *, hipStream_t), void *, int, bool *, hipStream_t);
+template int rec_u8_guarded<false>(const uint8_t *, const uint8_t *, uint8_t *, Dims, int,
+                                   int (*)(void *, int *, hipStream_t), void *, int, bool *, hipStream_t);
 
 }  // namespace pm
 
