@@ -275,6 +275,12 @@ template <typename T, bool DIL>
 int box_pass(const T *in, T *out, Dims d, int64_t rz, int64_t ry, int64_t rx, hipStream_t s)
 {
   if (rz <= 0) return rect_pass<T, DIL>(in, out, d.D, d.H, d.W, ry, rx, s);
+  if constexpr (std::is_same<T, uint8_t>::value) {
+    // a cube of radius <= VOL_CUBE_RMAX: one launch (it reads planes z-r .. z+r while other
+    // workgroups write plane z, so not in place)
+    if (rz == ry && ry == rx && rz <= VOL_CUBE_RMAX && vol_u8_applicable(in, out, d.D, d.H, d.W) && in != out)
+      return cube_u8<DIL>(in, out, d.D, d.H, d.W, (int)rz, s);
+  }
   const int64_t n = d.D * d.H * d.W;
   if (n == 0) return PM_OK;
   Scratch tmp;

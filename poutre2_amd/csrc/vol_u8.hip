@@ -6,6 +6,9 @@
 //    this) and SESegmentZ3D^K (ero_dil_static_se_t.hpp:214-265, 761-839).
 //  * cross3_u8_kernel<DIL>: one pass of the 6-connected cross SECross3D (7 taps,
 //    ero_dil_static_se_t.hpp:841-915); SECross3D^k is k passes.
+//  * cube_u8_kernel<DIL,R>: the (2R+1)^3 box (SESquare3D^R) in one launch, R <= VOL_CUBE_RMAX:
+//    the fused 2D square tile of sq_u8_tile.hpp whose region is loaded as the max / min over
+//    planes z-R .. z+R (the box is separable, and the z window commutes with the 2D square).
 // A thread owns one 16-byte chunk of a plane (4 words, consecutive lanes on consecutive
 // chunks, so every load and store is a coalesced 128-bit access) and walks VOL_ZB planes.
 // The cross keeps the chunk's planes z-1, z, z+1 in registers; the y +- 1 chunks and the two
@@ -16,6 +19,7 @@
 #include <utility>
 
 #include "pm_common.hpp"
+#include "sq_u8_kernel.hpp"
 #include "sq_u8_tile.hpp"
 #include "vol_u8.hpp"
 
@@ -131,6 +135,113 @@ __global__ __launch_bounds__(256) void cross3_u8_kernel(const uint4 *__restrict_
   }
 }
 
+// bytewise max / min of two words (high-byte u16 pairs, see sq_u8_tile.hpp)
+template <bool DIL> __device__ __forceinline__ uint32_t bytes_op(uint32_t a, uint32_t b)
+{
+  return __builtin_amdgcn_perm(pk<DIL>(odd_hi(a), odd_hi(b)), pk<DIL>(even_hi(a), even_hi(b)), 0x07030501u);
+}
+template <bool DIL> __device__ __forceinline__ uint4 bytes_op(const uint4 &a, const uint4 &b)
+{
+  return make_uint4(bytes_op<DIL>(a.x, b.x), bytes_op<DIL>(a.y, b.y), bytes_op<DIL>(a.z, b.z), bytes_op<DIL>(a.w, b.w));
+}
+
+// Region of the 2D tile at (y0, x0) of plane z, as sq_tile_load (aligned rows, R16 = 16),
+// each chunk the max / min over planes z-R .. z+R that lie in the volume: plane z's chunks
+// first, then the other planes', all of a plane's loads in flight at once.
+template <bool DIL, int R, int TW, int TH>
+__device__ __forceinline__ void cube_region_load(uint8_t *smem, const uint8_t *__restrict__ vol, int D, int z, int H,
+                                                 int W, int y0, int x0)
+{
+  constexpr int K = sq_k_of(R), R16 = sq_r16<K>(), AW = TW + 2 * R16, cpr = AW >> 4;
+  constexpr int nchunk = (TH + 2 * R) * cpr;
+  constexpr int MAXC = (nchunk + FTHREADS - 1) / FTHREADS;
+  const uint32_t nv = DIL ? 0u : 0xFFFFFFFFu;
+  const int gx0 = x0 - R16;
+  const int64_t plane = (int64_t)H * W;
+  const int tid = threadIdx.x;
+  int64_t off[MAXC];  // byte offset of the chunk within a plane, -1 outside the image
+#pragma unroll
+  for (int i = 0; i < MAXC; i++) {
+    const int c = tid + i * FTHREADS;
+    const int ri = c / cpr, cc = c - ri * cpr;
+    const int gy = y0 - R + ri, gx = gx0 + (cc << 4);
+    off[i] = (c < nchunk && (unsigned)gy < (unsigned)H && (unsigned)gx < (unsigned)W) ? (int64_t)gy * W + gx : -1;
+  }
+  uint4 v[MAXC];
+  const uint8_t *pz = vol + (int64_t)z * plane;
+#pragma unroll
+  for (int i = 0; i < MAXC; i++)
+    v[i] = off[i] >= 0 ? *reinterpret_cast<const uint4 *>(pz + off[i]) : make_uint4(nv, nv, nv, nv);
+#pragma unroll
+  for (int dz = -R; dz <= R; dz++) {
+    if (dz == 0 || (unsigned)(z + dz) >= (unsigned)D) continue;  // planes outside the volume: neutral
+    const uint8_t *pq = vol + (int64_t)(z + dz) * plane;
+    uint4 u[MAXC];
+#pragma unroll
+    for (int i = 0; i < MAXC; i++)
+      u[i] = off[i] >= 0 ? *reinterpret_cast<const uint4 *>(pq + off[i]) : make_uint4(nv, nv, nv, nv);
+#pragma unroll
+    for (int i = 0; i < MAXC; i++) v[i] = bytes_op<DIL>(v[i], u[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < MAXC; i++) {
+    const int c = tid + i * FTHREADS;
+    if (c < nchunk) {
+      const int ri = c / cpr, cc = c - ri * cpr;
+      *reinterpret_cast<uint4 *>(smem + ri * AW + (cc << 4)) = v[i];
+    }
+  }
+}
+
+// Persistent over the (plane, tile) list with the XCD-aware order of sq_u8_kernel: an XCD
+// walks a contiguous range of planes, so the z-neighbour planes of its tiles are read
+// through its L2.  Rows of W % 16 == 0 bytes, 16-byte aligned buffers.
+template <bool DIL, int R>
+__global__ __launch_bounds__(FTHREADS) void cube_u8_kernel(const uint8_t *__restrict__ in, uint8_t *__restrict__ out,
+                                                           int D, int H, int W, int tiles_x, int tiles_per_img,
+                                                           int ntiles)
+{
+  extern __shared__ __align__(16) uint8_t smem[];
+  constexpr int K = sq_k_of(R), TW = 256, TH = sq_th(R);
+  const TileSched ts = tile_sched(blockIdx.x, gridDim.x, ntiles);
+  for (int tile = ts.first; tile < ts.end; tile += ts.step) {
+    const int z = tile / tiles_per_img;
+    const int t_in = tile - z * tiles_per_img;
+    const int ty = t_in / tiles_x, tx = t_in - ty * tiles_x;
+    const int y0 = ty * TH, x0 = tx * TW;
+    cube_region_load<DIL, R, TW, TH>(smem, in, D, z, H, W, y0, x0);
+    __syncthreads();
+    sq_tile_passes<K, DIL, true, R, TW, TH>(smem, out + (int64_t)z * H * W, H, W, R, y0, x0);
+    __syncthreads();  // A/T are rewritten by the next tile
+  }
+}
+
+typedef void (*cube_kernel_t)(const uint8_t *, uint8_t *, int, int, int, int, int, int);
+
+template <bool DIL> int cube_u8(const uint8_t *in, uint8_t *out, int64_t D, int64_t H, int64_t W, int r, hipStream_t s)
+{
+  static_assert(VOL_CUBE_RMAX == 1, "one kernel per radius");
+  if (r < 1 || r > VOL_CUBE_RMAX) PM_FAIL(PM_E_ARG, "cube radius %d outside [1, %d]", r, VOL_CUBE_RMAX);
+  const cube_kernel_t k = cube_u8_kernel<DIL, 1>;
+  constexpr int TW = 256;
+  const int th = sq_th(r);
+  const int tiles_x = (int)((W + TW - 1) / TW), tiles_y = (int)((H + th - 1) / th);
+  const int64_t per = (int64_t)tiles_x * tiles_y, ntiles = per * D;
+  if (ntiles > INT32_MAX) PM_FAIL(PM_E_ARG, "volume too large for the cube kernel");
+  const size_t lds = sq_tile_lds_bytes(r, TW, th);
+  int dev = 0, cus = 0, occ = 0;
+  PM_HIP(hipGetDevice(&dev));
+  PM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  PM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, FTHREADS, lds));
+  int64_t g = (int64_t)(occ > 0 ? occ : 1) * cus;
+  if (g >= 8) g &= ~(int64_t)7;
+  if (g > ntiles) g = ntiles;
+  hipLaunchKernelGGL(k, dim3((unsigned)g), dim3(FTHREADS), lds, s, in, out, (int)D, (int)H, (int)W, tiles_x, (int)per,
+                     (int)ntiles);
+  PM_HIP(hipGetLastError());
+  return PM_OK;
+}
+
 bool vol_u8_applicable(const void *in, const void *out, int64_t D, int64_t H, int64_t W)
 {
   return D > 0 && H > 0 && W > 0 && W % 16 == 0 && ((uintptr_t)in % 16) == 0 && ((uintptr_t)out % 16) == 0 &&
@@ -171,5 +282,7 @@ template int zwin_u8<true>(const uint8_t *, uint8_t *, int64_t, int64_t, int64_t
 template int zwin_u8<false>(const uint8_t *, uint8_t *, int64_t, int64_t, int64_t, int, hipStream_t);
 template int cross3_u8<true>(const uint8_t *, uint8_t *, int64_t, int64_t, int64_t, hipStream_t);
 template int cross3_u8<false>(const uint8_t *, uint8_t *, int64_t, int64_t, int64_t, hipStream_t);
+template int cube_u8<true>(const uint8_t *, uint8_t *, int64_t, int64_t, int64_t, int, hipStream_t);
+template int cube_u8<false>(const uint8_t *, uint8_t *, int64_t, int64_t, int64_t, int, hipStream_t);
 
 }  // namespace pm
