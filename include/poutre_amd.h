@@ -241,9 +241,9 @@ int pm_label_maxima(const pm_image *in, int se, pm_image *out, void *stream);
 /* ---- batched launches (no reference counterpart: the reference processes one image per call).
  * pm_batch_dilate_open: for each i < n, dil_outs[i] = Dilate(ins[i], SESquare2D, radii[i]) and
  * open_outs[i] = Dilate(Erode(ins[i], SESquare2D, radii[i]), SESquare2D, radii[i]), all images
- * uint8 2D of any sizes, radii in [1, 32]: one persistent launch per distinct radius up to
- * 15 (dilation tiles and fused-opening tiles in one work list), two above, whatever the
- * number of images.
+ * uint8 2D of any sizes, radii in [1, 32]: one persistent launch for all images with radii
+ * up to 15 (dilation tiles and fused-opening tiles of every radius in one work list), two per
+ * distinct radius above, whatever the number of images.
  * pm_batch_reconstruct: n reconstructions (geo::Reconstruction semantics, uint8 2D, 3x3 SEs)
  * sharing one sequence of global rounds. */
 int pm_batch_dilate_open(int n, const pm_image *ins, const int *radii, pm_image *dil_outs, pm_image *open_outs,
