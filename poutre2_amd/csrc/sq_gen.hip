@@ -1,11 +1,40 @@
 // sq_gen.hip -- launcher of the fused square kernels for uint16 / int32 / int64 / float32 /
-// float64 images (kernels: sq_gen.hpp, instantiated in sq_gen_<type>.hip).
+// float64 images (kernels: sq_u16.hpp for uint16, instantiated in sq_u16_k*.hip; sq_gen.hpp
+// for the others, instantiated in sq_gen_<type>.hip).
+#include <type_traits>
+
 #include "sq_gen.hpp"
+#include "sq_u16.hpp"
 
 namespace pm {
 
+static_assert(SQ_U16_RMAX == SQ_GEN_RMAX, "one chaining rule for every type");
+
+template <bool DIL>
+static int sq_u16_once(const uint16_t *in, uint16_t *out, int64_t B, int64_t H, int64_t W, int r, hipStream_t s)
+{
+  const bool aligned = (W % 8) == 0 && ((uintptr_t)in % 16) == 0 && ((uintptr_t)out % 16) == 0;
+  const int th = u16_th(r);
+  const int64_t tiles_x = (W + U16_TW - 1) / U16_TW, tiles_y = (H + th - 1) / th;
+  const int64_t per = tiles_x * tiles_y, ntiles = per * B;
+  if (ntiles > INT32_MAX) PM_FAIL(PM_E_ARG, "image too large for the fused square kernel");
+  const u16_kernel_t k = aligned ? sq_u16_kernel_for<DIL, true>(r) : sq_u16_kernel_for<DIL, false>(r);
+  const size_t lds = sq_u16_lds_bytes(r);
+  int dev = 0, cus = 0, occ = 0;
+  PM_HIP(hipGetDevice(&dev));
+  PM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  PM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, FTHREADS, lds));
+  int64_t g = (int64_t)(occ > 0 ? occ : 1) * cus;
+  if (g >= 8) g &= ~(int64_t)7;
+  if (g > ntiles) g = ntiles;
+  hipLaunchKernelGGL(k, dim3((unsigned)g), dim3(FTHREADS), lds, s, in, out, (int)H, (int)W, (int)tiles_x, (int)per,
+                     (int)ntiles);
+  PM_HIP(hipGetLastError());
+  return PM_OK;
+}
+
 template <typename T, bool DIL>
-static int sq_gen_once(const T *in, T *out, int64_t B, int64_t H, int64_t W, int r, hipStream_t s)
+static int sq_gen_generic(const T *in, T *out, int64_t B, int64_t H, int64_t W, int r, hipStream_t s)
 {
   constexpr int TWE = gen_twe<T>();
   const int64_t tiles_x = (W + TWE - 1) / TWE, tiles_y = (H + GEN_FT - 1) / GEN_FT;
@@ -24,6 +53,13 @@ static int sq_gen_once(const T *in, T *out, int64_t B, int64_t H, int64_t W, int
                      (int)ntiles);
   PM_HIP(hipGetLastError());
   return PM_OK;
+}
+
+template <typename T, bool DIL>
+static int sq_gen_once(const T *in, T *out, int64_t B, int64_t H, int64_t W, int r, hipStream_t s)
+{
+  if constexpr (std::is_same<T, uint16_t>::value) return sq_u16_once<DIL>(in, out, B, H, W, r, s);
+  else return sq_gen_generic<T, DIL>(in, out, B, H, W, r, s);
 }
 
 template <typename T, bool DIL> int sq_gen(const T *in, T *out, int64_t B, int64_t H, int64_t W, int64_t r, hipStream_t s)

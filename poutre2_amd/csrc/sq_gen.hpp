@@ -1,8 +1,9 @@
-// sq_gen.hpp -- fused square erosion / dilation for uint16, int32, int64, float32 and float64
+// sq_gen.hpp -- fused square erosion / dilation for int32, int64, float32 and float64
 // images: Erode/Dilate(img, SESquare2D, r) (ero_dil_static.cpp:229-305, the (2r+1)^2 square
 // as a 2D line pass pair, ero_dil_line_se_t.hpp:49-280) in one launch per radius up to
 // SQ_GEN_RMAX (larger radii chain launches: a square of side 2a+1 followed by one of side
-// 2b+1 is the square of side 2(a+b)+1, also at the image borders).
+// 2b+1 is the square of side 2(a+b)+1, also at the image borders).  uint16 images have a
+// kernel of their own (sq_u16.hpp), which uses the run helpers below on pixel pairs.
 //
 // One workgroup (256 threads) owns an output tile of 64 rows x TWE columns (gen_twe), in a
 // persistent loop over tiles (XCD-aware order, sq_u8_kernel.hpp):
@@ -13,12 +14,10 @@
 //     consecutive rows, so the odd pitch puts them on distinct banks) and computes the
 //     window with a single-split van Herk run (suffix over the run, prefix over the rest:
 //     about three operations per pixel whatever r is); results are held in registers across
-//     a barrier and written over the region (pitch gen_pitch2: TWE + 1, for uint16 TWE + 2);
+//     a barrier and written over the region (pitch gen_pitch2: TWE + 1);
 //  3. vertical pass: a lane owns one column and a run of K output rows (lanes on
 //     consecutive columns: conflict-free reads, and each store instruction writes one
-//     contiguous 256-byte row segment of the tile).  uint16: a lane owns a PAIR of columns
-//     read as one 32-bit word (the horizontal pass's buffer has an even pitch) and reduced
-//     with v_pk_max_u16 / v_pk_min_u16, two pixels per operation and per store lane.
+//     contiguous 256-byte row segment of the tile).
 #pragma once
 #include "pm_common.hpp"
 #include "sq_u8_kernel.hpp"
@@ -31,19 +30,9 @@ constexpr int GEN_FT = 64;       // output tile rows
 // tile columns: 256-byte row segments
 template <typename T> constexpr int gen_twe() { return 256 / (int)sizeof(T); }
 constexpr int gen_k(int r) { return r >= 8 ? 16 : r >= 4 ? 8 : 4; }
-template <typename T> constexpr bool gen_pairs() { return sizeof(T) == 2; }
-// columns of the region left (and right) of the tile: r, or for column pairs r rounded up to
-// even, so the region starts at an even column and every pair load is 4-byte aligned
-template <typename T> constexpr int gen_rl(int r) { return gen_pairs<T>() ? (r + 1) & ~1 : r; }
-// region pitch: odd (elements), or for column pairs an odd number of 32-bit words (the pairs
-// stay word-aligned and lanes on consecutive rows still meet distinct banks)
-template <typename T> constexpr int gen_pitch(int r)
-{
-  return gen_pairs<T>() ? gen_twe<T>() + 2 * gen_rl<T>(r) + 2 : gen_twe<T>() + 2 * r + 1;
-}
-// pitch of the horizontal pass's output: odd, or for column pairs an even number of
-// elements that is an odd number of 32-bit words
-template <typename T> constexpr int gen_pitch2() { return gen_pairs<T>() ? gen_twe<T>() + 2 : gen_twe<T>() + 1; }
+// region pitch and the horizontal pass's output pitch: odd (elements)
+template <typename T> constexpr int gen_pitch(int r) { return gen_twe<T>() + 2 * r + 1; }
+template <typename T> constexpr int gen_pitch2() { return gen_twe<T>() + 1; }
 template <typename T> inline size_t gen_lds_bytes(int r)
 {
   const size_t a = (size_t)(GEN_FT + 2 * r) * gen_pitch<T>(r), b = (size_t)(GEN_FT + 2 * r) * gen_pitch2<T>();
@@ -98,7 +87,7 @@ __global__ __launch_bounds__(256) void sq_gen_kernel(const T *__restrict__ in, T
   using L = Lat<T, DIL>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *S = reinterpret_cast<T *>(smem_raw);
-  constexpr int TWE = gen_twe<T>(), K = gen_k(R), RH = GEN_FT + 2 * R, RL = gen_rl<T>(R), RW = TWE + 2 * RL;
+  constexpr int TWE = gen_twe<T>(), K = gen_k(R), RH = GEN_FT + 2 * R, RW = TWE + 2 * R;
   constexpr int P = gen_pitch<T>(R), P2 = gen_pitch2<T>();
   static_assert(TWE % K == 0 && GEN_FT % K == 0, "runs tile the output");
   constexpr int NEL = RH * RW, NL = (NEL + 255) / 256;
@@ -114,59 +103,23 @@ __global__ __launch_bounds__(256) void sq_gen_kernel(const T *__restrict__ in, T
     const T *src = in + (int64_t)img * H * W;
     T *dst = out + (int64_t)img * H * W;
     // ---- 1. region rows y0-R .., columns x0-R ..
-    if constexpr (gen_pairs<T>()) {
-      // pixel pairs: one 2-byte aligned word per lane where both pixels are in the image
-      typedef uint32_t u32_a2 __attribute__((aligned(2)));
-      constexpr int RW2 = RW / 2, NEL2 = RH * RW2, NL2 = (NEL2 + 255) / 256;
-      const uint32_t nn = (uint32_t)L::neutral() * 0x10001u;
-      uint32_t v[NL2];
+    T v[NL];
 #pragma unroll
-      for (int i = 0; i < NL2; i++) {
-        const int e = tid + i * 256;
-        v[i] = nn;
-        if (e < NEL2) {
-          const int row = e / RW2, c = e - row * RW2;
-          const int gy = y0 - R + row, gx = x0 - RL + 2 * c;
-          if ((unsigned)gy < (unsigned)H) {
-            const T *q = src + (int64_t)gy * W + gx;
-            if (gx >= 0 && gx + 1 < W) {
-              v[i] = *reinterpret_cast<const u32_a2 *>(q);
-            } else {
-              const uint32_t lo = (unsigned)gx < (unsigned)W ? (uint32_t)q[0] : (nn & 0xFFFFu);
-              const uint32_t hi = (unsigned)(gx + 1) < (unsigned)W ? (uint32_t)q[1] : (nn & 0xFFFFu);
-              v[i] = lo | (hi << 16);
-            }
-          }
-        }
+    for (int i = 0; i < NL; i++) {
+      const int e = tid + i * 256;
+      v[i] = L::neutral();
+      if (e < NEL) {
+        const int row = e / RW, col = e - row * RW;
+        const int gy = y0 - R + row, gx = x0 - R + col;
+        if ((unsigned)gy < (unsigned)H && (unsigned)gx < (unsigned)W) v[i] = src[(int64_t)gy * W + gx];
       }
-      uint32_t *S32 = reinterpret_cast<uint32_t *>(S);
+    }
 #pragma unroll
-      for (int i = 0; i < NL2; i++) {
-        const int e = tid + i * 256;
-        if (e < NEL2) {
-          const int row = e / RW2, c = e - row * RW2;
-          S32[row * (P / 2) + c] = v[i];
-        }
-      }
-    } else {
-      T v[NL];
-#pragma unroll
-      for (int i = 0; i < NL; i++) {
-        const int e = tid + i * 256;
-        v[i] = L::neutral();
-        if (e < NEL) {
-          const int row = e / RW, col = e - row * RW;
-          const int gy = y0 - R + row, gx = x0 - R + col;
-          if ((unsigned)gy < (unsigned)H && (unsigned)gx < (unsigned)W) v[i] = src[(int64_t)gy * W + gx];
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < NL; i++) {
-        const int e = tid + i * 256;
-        if (e < NEL) {
-          const int row = e / RW, col = e - row * RW;
-          S[row * P + col] = v[i];
-        }
+    for (int i = 0; i < NL; i++) {
+      const int e = tid + i * 256;
+      if (e < NEL) {
+        const int row = e / RW, col = e - row * RW;
+        S[row * P + col] = v[i];
       }
     }
     __syncthreads();
@@ -178,7 +131,7 @@ __global__ __launch_bounds__(256) void sq_gen_kernel(const T *__restrict__ in, T
         const int task = tid + i * 256;
         if (task < HT) {
           const int run = task / RH, row = task - run * RH;
-          const T *b = S + row * P + run * K + (RL - R);
+          const T *b = S + row * P + run * K;
           vh_run_t<T, K, DIL, R>([&](int t) { return b[t]; }, res[i]);
         }
       }
@@ -195,48 +148,21 @@ __global__ __launch_bounds__(256) void sq_gen_kernel(const T *__restrict__ in, T
     }
     __syncthreads();
     // ---- 3. vertical pass: column `col`, output rows run*K .. run*K+K-1, stored
-    if constexpr (gen_pairs<T>()) {
-      // columns 2c, 2c+1 as one word; rows of any alignment take 2-byte aligned word stores
-      typedef uint32_t u32_a2 __attribute__((aligned(2)));
-      constexpr int TW2 = TWE / 2, P2W = P2 / 2, VT2 = TW2 * NRV;
-      const uint32_t *S2 = reinterpret_cast<const uint32_t *>(S);
-      const bool full = y0 + GEN_FT <= H && x0 + TWE <= W;
-      for (int task = tid; task < VT2; task += 256) {
-        const int run = task / TW2, c2 = task - run * TW2;
-        uint32_t o[K];
-        const uint32_t *b = S2 + (run * K) * P2W + c2;
-        vh_run_l<LatPk16<DIL>, uint32_t, K, R>([&](int t) { return b[t * P2W]; }, o);
-        const int gx = x0 + 2 * c2, gy0 = y0 + run * K;
-        T *d = dst + (int64_t)gy0 * W + gx;
-        if (full) {
+    const bool full = y0 + GEN_FT <= H && x0 + TWE <= W;
+    for (int task = tid; task < VT; task += 256) {
+      const int run = task / TWE, col = task - run * TWE;
+      T o[K];
+      const T *b = S + (run * K) * P2 + col;
+      vh_run_t<T, K, DIL, R>([&](int t) { return b[t * P2]; }, o);
+      const int gx = x0 + col, gy0 = y0 + run * K;
+      T *d = dst + (int64_t)gy0 * W + gx;
+      if (full) {
 #pragma unroll
-          for (int j = 0; j < K; j++) *reinterpret_cast<u32_a2 *>(d + (int64_t)j * W) = o[j];
-        } else {
+        for (int j = 0; j < K; j++) d[(int64_t)j * W] = o[j];
+      } else if (gx < W) {
 #pragma unroll
-          for (int j = 0; j < K; j++)
-            if (gy0 + j < H) {
-              if (gx < W) d[(int64_t)j * W] = (T)(o[j] & 0xFFFFu);
-              if (gx + 1 < W) d[(int64_t)j * W + 1] = (T)(o[j] >> 16);
-            }
-        }
-      }
-    } else {
-      const bool full = y0 + GEN_FT <= H && x0 + TWE <= W;
-      for (int task = tid; task < VT; task += 256) {
-        const int run = task / TWE, col = task - run * TWE;
-        T o[K];
-        const T *b = S + (run * K) * P2 + col;
-        vh_run_t<T, K, DIL, R>([&](int t) { return b[t * P2]; }, o);
-        const int gx = x0 + col, gy0 = y0 + run * K;
-        T *d = dst + (int64_t)gy0 * W + gx;
-        if (full) {
-#pragma unroll
-          for (int j = 0; j < K; j++) d[(int64_t)j * W] = o[j];
-        } else if (gx < W) {
-#pragma unroll
-          for (int j = 0; j < K; j++)
-            if (gy0 + j < H) d[(int64_t)j * W] = o[j];
-        }
+        for (int j = 0; j < K; j++)
+          if (gy0 + j < H) d[(int64_t)j * W] = o[j];
       }
     }
     __syncthreads();  // the next tile's region overwrites S
@@ -250,7 +176,6 @@ template <typename T, bool DIL> gen_kernel_t<T, DIL> sq_gen_kernel_for(int r);
 #define PM_GEN_DECL(T)                                                                                       \
   template <> gen_kernel_t<T, true> sq_gen_kernel_for<T, true>(int r);                                       \
   template <> gen_kernel_t<T, false> sq_gen_kernel_for<T, false>(int r);
-PM_GEN_DECL(uint16_t)
 PM_GEN_DECL(int32_t)
 PM_GEN_DECL(int64_t)
 PM_GEN_DECL(float)

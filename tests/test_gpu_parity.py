@@ -390,30 +390,37 @@ def test_open_close_u8_fused_unaligned_view(ioff, ooff):
 
 
 @pytest.mark.parametrize("dtype", [np.uint16, np.int32, np.int64, np.float32, np.float64])
-@pytest.mark.parametrize("shape", [(64, 64), (65, 70), (200, 130), (3, 5), (130, 257), (1, 300), (300, 1)])
-@pytest.mark.parametrize("r", [1, 2, 3, 4, 7, 8, 15, 16, 17, 40])
+@pytest.mark.parametrize("shape", [(64, 64), (65, 70), (200, 130), (3, 5), (130, 257), (1, 300), (300, 1),
+                                   (72, 136)])
+@pytest.mark.parametrize("r", [1, 2, 3, 4, 7, 8, 9, 15, 16, 17, 40])
 def test_square_generic_fused(dtype, shape, r):
     """uint16 / int32 / int64 / float32 / float64 square (one fused launch per radius up to 16,
-    chained launches above) == the oracle's x line pass then r y segments."""
-    img = rand(shape, dtype, 120 + r)
+    chained launches above) == the oracle's x line pass then r y segments.  (72, 136): rows a
+    multiple of 16 bytes for every type, partial tiles on both axes.  uint16 takes its full
+    range, so that both bytes of every pixel take part."""
+    img = rand(shape, dtype, 120 + r, hi=65536 if dtype == np.uint16 else 256)
     for dil in (True, False):
         f = pm.dilate if dil else pm.erode
         np.testing.assert_array_equal(h(f(g(img), pm.static_nl_se.square2d, r)), _sq_oracle(img, r, dil))
 
 
-@pytest.mark.parametrize("dtype", [np.int32, np.float64])
-def test_square_generic_fused_views(dtype):
-    """the fused generic kernel on views at element offsets (no alignment assumed)"""
-    H, W = 70, 93
-    base = torch.from_numpy(rand((H * W + 5,), dtype, 130)).to(DEV)
+@pytest.mark.parametrize("dtype", [np.uint16, np.int32, np.float64])
+@pytest.mark.parametrize("W,r", [(93, 5), (96, 1), (96, 12), (7, 2)])
+def test_square_generic_fused_views(dtype, W, r):
+    """the fused square kernels on views at element offsets (no alignment assumed; W = 96
+    makes rows a multiple of 16 bytes on an unaligned base, W = 7 rows narrower than 16
+    bytes); the elements around the output view are left untouched"""
+    H = 70
+    base = torch.from_numpy(rand((H * W + 5,), dtype, 130, hi=65536 if dtype == np.uint16 else 256)).to(DEV)
     view = base[3:3 + H * W].view(H, W)
     img = view.cpu().numpy()
-    obuf = torch.zeros((H * W + 5,), dtype=base.dtype, device=DEV)
-    oview = obuf[1:1 + H * W].view(H, W)
-    pm.dilate(view, pm.static_nl_se.square2d, 5, out=oview)
-    np.testing.assert_array_equal(h(oview), _sq_oracle(img, 5, True))
-    rest = obuf.cpu().numpy()
-    assert rest[0] == 0 and (rest[1 + H * W:] == 0).all()
+    for dil in (True, False):
+        obuf = torch.zeros((H * W + 5,), dtype=base.dtype, device=DEV)
+        oview = obuf[1:1 + H * W].view(H, W)
+        (pm.dilate if dil else pm.erode)(view, pm.static_nl_se.square2d, r, out=oview)
+        np.testing.assert_array_equal(h(oview), _sq_oracle(img, r, dil))
+        rest = obuf.cpu().numpy()
+        assert rest[0] == 0 and (rest[1 + H * W:] == 0).all()
 
 
 @pytest.mark.parametrize("it", [1, 2, 5])
