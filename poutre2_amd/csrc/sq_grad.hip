@@ -3,7 +3,11 @@
 // of ero_dil_static.cpp:229-305 with arith_saturated_sub_image; max >= min, so the
 // subtraction never saturates).  One read of the input, one write of the output; the tile
 // code is that of the fused square kernel (sq_u8_tile.hpp) run for max and min on one
-// region with clamped borders.
+// region with clamped borders.  Up to r = 15 the max and min go to two transposed buffers
+// (128-column tiles); from r = 16 (runs of 32 rows) one transposed buffer serves both in
+// turn (sq_tile_grad_passes1), which halves the LDS and allows 192-column tiles: 8192x8192,
+// us, two buffers / 128 columns vs one buffer / 192 columns: r = 16 92.8 / 77.6, r = 24
+// 100.5 / 82.4, r = 31 108.8 / 88.5 (r = 15: 66.0 / 63.8, r = 8: 57.5 / 57.2).
 #include <utility>
 
 #include "pm_common.hpp"
@@ -12,7 +16,12 @@
 
 namespace pm {
 
-constexpr int GRAD_TW = 128;  // tile width: the two transposed buffers double the LDS per workgroup
+// tile width: 128 columns with two transposed buffers (runs of up to 16 rows), 192 with one
+constexpr int grad_tw(int K) { return K == 32 ? 192 : 128; }
+inline size_t grad_lds_bytes(int r)
+{
+  return sq_k_open(r) == 32 ? sq_tile_lds_bytes(r, grad_tw(32)) : sq_grad_lds_bytes(r, grad_tw(16));
+}
 
 struct GradSched {
   int first, step, end;
@@ -30,17 +39,19 @@ __global__ __launch_bounds__(FTHREADS) void sq_grad_kernel(const uint8_t *__rest
     const int x = b & 7, j = b >> 3, gx = G >> 3;
     ts = {(int)((int64_t)x * ntiles / 8) + j, gx, (int)((int64_t)(x + 1) * ntiles / 8)};
   }
+  constexpr int TW = grad_tw(K);
   for (int tile = ts.first; tile < ts.end; tile += ts.step) {
     const int img = tile / tiles_per_img;
     const int t_in = tile - img * tiles_per_img;
     const int ty = t_in / tiles_x, tx = t_in - ty * tiles_x;
-    const int y0 = ty * FT, x0 = tx * GRAD_TW;
+    const int y0 = ty * FT, x0 = tx * TW;
     const int64_t ibase = (int64_t)img * H * W;
-    // K = 16 (r >= 8): indices recomputed per tile (fewer registers) are faster; below, the
+    // K >= 16 (r >= 8): indices recomputed per tile (fewer registers) are faster; below, the
     // hoisted ones (see tile_tid)
-    sq_tile_load_clamped<K, ALIGNED, RC, GRAD_TW, (K >= 16)>(smem, in + ibase, H, W, RC, y0, x0);
+    sq_tile_load_clamped<K, ALIGNED, RC, TW, (K >= 16)>(smem, in + ibase, H, W, RC, y0, x0);
     __syncthreads();
-    sq_tile_grad_passes<K, ALIGNED, RC, GRAD_TW, (K >= 16)>(smem, out + ibase, H, W, RC, y0, x0);
+    if constexpr (K == 32) sq_tile_grad_passes1<K, ALIGNED, RC, TW, true>(smem, out + ibase, H, W, RC, y0, x0);
+    else sq_tile_grad_passes<K, ALIGNED, RC, TW, (K >= 16)>(smem, out + ibase, H, W, RC, y0, x0);
     __syncthreads();
   }
 }
@@ -61,12 +72,13 @@ int sq_u8_gradient(const uint8_t *in, uint8_t *out, int64_t B, int64_t H, int64_
   if (r < 1 || r > SQ_U8_GRAD_RMAX)
     PM_FAIL(PM_E_ARG, "fused gradient radius %lld outside [1, %d]", (long long)r, (int)SQ_U8_GRAD_RMAX);
   if (H > INT32_MAX / 2 || W > INT32_MAX / 2) PM_FAIL(PM_E_ARG, "image too large for the fused gradient");
-  const int tiles_x = (int)((W + GRAD_TW - 1) / GRAD_TW), tiles_y = (int)((H + FT - 1) / FT);
+  const int tw = grad_tw(sq_k_open((int)r));
+  const int tiles_x = (int)((W + tw - 1) / tw), tiles_y = (int)((H + FT - 1) / FT);
   const int64_t per = (int64_t)tiles_x * tiles_y, ntiles = per * B;
   if (ntiles > INT32_MAX) PM_FAIL(PM_E_ARG, "image too large for the fused gradient");
   const bool aligned = (W % 16) == 0 && ((uintptr_t)in % 16) == 0 && ((uintptr_t)out % 16) == 0;
   const grad_kernel_t k = grad_pick((int)r, aligned, std::make_integer_sequence<int, (int)SQ_U8_GRAD_RMAX>{});
-  const size_t lds = sq_grad_lds_bytes((int)r, GRAD_TW);
+  const size_t lds = grad_lds_bytes((int)r);
   int dev = 0, cus = 0, occ = 0;
   PM_HIP(hipGetDevice(&dev));
   PM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));

@@ -670,6 +670,90 @@ __device__ __forceinline__ void sq_tile_grad_passes(uint8_t *smem, uint8_t *__re
   }
 }
 
+// Passes 2 and 3 of the gradient with ONE transposed buffer: the vertical pass's max and min
+// results are held in registers; the max results go to T (reusing the region's storage), the
+// horizontal max pass keeps its rows in registers while the min results replace them in T,
+// and the horizontal min pass subtracts and stores.  Half the LDS of the two-buffer form, for
+// the long runs (K = 32) whose per-thread task counts keep the held rows few.
+template <int K, bool ALIGNED, int RC, int TW, bool OPQ = false>
+__device__ __forceinline__ void sq_tile_grad_passes1(uint8_t *smem, uint8_t *__restrict__ dst_img, int H, int W,
+                                                     int r_rt, int y0, int x0)
+{
+  constexpr int R16 = sq_r16<K>();
+  constexpr int AW = TW + 2 * R16;
+  const int r = RC > 0 ? RC : r_rt;
+  uint8_t *A = smem;
+  uint8_t *T = smem;
+  const int tid = tile_tid<OPQ>();
+  constexpr int MAXT = sq_maxt(K, TW);
+  const int q0 = (R16 - r) >> 2, q1 = (R16 + TW + r + 3) >> 2;
+  const int nq = q1 - q0;
+  const int ntask = nq * (FT / K);
+  uint32_t rmax[MAXT][K], rmin[MAXT][K];
+#pragma unroll
+  for (int sl = 0; sl < MAXT; sl++) {
+    const int task = tid + sl * FTHREADS;
+    if (task < ntask) {
+      const int k = task / nq, q = q0 + (task - k * nq);
+      uint32_t e[K], o[K];
+      const uint8_t *base = A + (k * K) * AW + 4 * q;
+      vh_run<K, true, RC>([&](int t) { return lds_word(base + t * AW); }, r, e, o);
+#pragma unroll
+      for (int g = 0; g < K / 4; g++) transpose4(e + 4 * g, o + 4 * g, &rmax[sl][4 * g]);
+      vh_run<K, false, RC>([&](int t) { return lds_word(base + t * AW); }, r, e, o);
+#pragma unroll
+      for (int g = 0; g < K / 4; g++) transpose4(e + 4 * g, o + 4 * g, &rmin[sl][4 * g]);
+    }
+  }
+  // the transposed results of one operator into T
+  auto put = [&](const uint32_t (&res)[MAXT][K]) {
+#pragma unroll
+    for (int sl = 0; sl < MAXT; sl++) {
+      const int task = tid + sl * FTHREADS;
+      if (task < ntask) {
+        const int k = task / nq, q = q0 + (task - k * nq);
+#pragma unroll
+        for (int g = 0; g < K / 4; g++)
+#pragma unroll
+          for (int c = 0; c < 4; c++)
+            *reinterpret_cast<uint32_t *>(T + t_off(4 * q + c, (k * K >> 2) + g)) = res[sl][4 * g + c];
+      }
+    }
+  };
+  constexpr int NR = TW / K;
+  constexpr int ntask3 = (FT / 4) * NR, MAXH = (ntask3 + FTHREADS - 1) / FTHREADS;
+  __syncthreads();
+  put(rmax);
+  __syncthreads();
+  uint32_t hi[MAXH][4][K / 4];
+#pragma unroll
+  for (int i = 0; i < MAXH; i++) {
+    const int task = tid + i * FTHREADS;
+    if (task < ntask3) {
+      const int q2 = task / NR, k2 = task - q2 * NR;
+      sq_pass3_rows<K, true, RC>(T, r, R16 - r, q2, k2, hi[i]);
+    }
+  }
+  __syncthreads();
+  put(rmin);
+  __syncthreads();
+  const bool full = (y0 + FT <= H) && (x0 + TW <= W);
+#pragma unroll
+  for (int i = 0; i < MAXH; i++) {
+    const int task = tid + i * FTHREADS;
+    if (task < ntask3) {
+      const int q2 = task / NR, k2 = task - q2 * NR;
+      uint32_t lo[4][K / 4];
+      sq_pass3_rows<K, false, RC>(T, r, R16 - r, q2, k2, lo);
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int g = 0; g < K / 4; g++) hi[i][c][g] -= lo[c][g];
+      sq_store_rows<K, ALIGNED, TW>(hi[i], dst_img, H, W, y0, x0, q2, k2, full);
+    }
+  }
+}
+
 constexpr int SQ_TILE_RMAX = 32;
 // LDS bytes of one workgroup for radius r, tile width TW and height TH (region A and
 // transposed buffer T alias)
