@@ -92,9 +92,11 @@ def test_static_3d(dtype, se, it):
 @pytest.mark.parametrize("se", ["SECross3D", "SESquare3D", "SESegmentZ3D"])
 @pytest.mark.parametrize("shape,it", [((11, 13, 48), 1), ((11, 13, 48), 2), ((7, 9, 32), 3), ((20, 5, 16), 8),
                                       ((20, 5, 16), 9), ((3, 17, 64), 4), ((1, 6, 16), 2), ((4, 2, 32), 2),
-                                      ((2, 3, 16), 5)])
+                                      ((2, 3, 16), 5), ((40, 70, 272), 2), ((40, 70, 272), 3), ((33, 65, 144), 4),
+                                      ((17, 40, 256), 6), ((16, 32, 128), 3)])
 def test_static_3d_u8_volume_kernels(se, shape, it):
-    """uint8 volumes with W % 16 == 0 take the z-streaming kernels (z radius <= 8; 9 falls back)."""
+    """uint8 volumes with W % 16 == 0 take the z-streaming kernels (z radius <= 8; 9 falls back);
+    the larger shapes cover several 2D tiles and partial ones on every axis."""
     img = rand(shape, np.uint8, 5 + it)
     for dil in (True, False):
         f = pm.dilate if dil else pm.erode
