@@ -117,6 +117,20 @@ def bench_dilate_8k(pm, torch, steps, warmup, dev):
                 "kernel": f"sq_u8_kernel<{kk}, true, true, {r_dom}, {pm.tuning()['aligned'][r_dom - 1]}> "
                           f"(side {dom}) on 8192x8192 uint8",
                 "kernel_ms": t_dom, "bytes_per_launch": 2 * npx, "side": int(dom)}
+    # practical ceiling: a plain elementwise kernel moving the same bytes (64 MiB read + 64 MiB
+    # written) over the same rotating pairs, queued back to back
+    def copy_one():
+        a, b = pairs[state["i"] % 3]
+        state["i"] += 1
+        torch.bitwise_not(a, out=b)
+    copy_one()
+    reps = max(10, steps)
+    t_copy = timed(copy_one, reps, stream, ahead_us=60 * reps + 300) / reps
+    roofline["copy_ceiling"] = {
+        "kernel": "torch.bitwise_not on the same 8192x8192 uint8 pairs (64 MiB read + 64 MiB written)",
+        "ms": round(t_copy, 4), "achieved_gbs": round(2 * npx / (t_copy * 1e-3) / 1e9, 1),
+        "frac_of_peak": round(2 * npx / (t_copy * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "dominant_kernel_frac_of_copy": round(t_copy / t_dom, 4)}
     # op-level fraction of HBM peak over the whole step (2 B/px algorithmic)
     pct = 100 * 2 * npx * len(SIDES) * steps / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS
     del pairs, base
