@@ -302,16 +302,6 @@ template <typename T> static int marker_order_dev(const T *f, const T *g, int64_
   return PM_OK;
 }
 
-template <typename T> struct OrderArgs {
-  const T *f, *g;
-  int64_t n;
-};
-template <typename T> static int marker_order_launch(void *ctx, int *bits, hipStream_t s)
-{
-  const OrderArgs<T> *a = (const OrderArgs<T> *)ctx;
-  return marker_order_dev<T>(a->f, a->g, a->n, bits, s);
-}
-
 template <typename T> static int marker_order(const T *f, const T *g, int64_t n, int *bits, hipStream_t s)
 {
   Scratch sc;
@@ -350,13 +340,13 @@ static int leveling_entry(const pm_image *ref, const pm_image *marker, int se, p
       // high(f, g) = f; g >= f gives high(f, g) = leveling(f, g) = R^e_f(g) and low(f, g) = f.
       if constexpr (std::is_same<T, uint8_t>::value) {
         // uint8 2D: assume g <= f (a marker from an opening, the common case) and run its
-        // reconstruction at once, guarded by the order bits on the device, so that no host
-        // round trip sits between the order check and the rounds; on a failed guess fall
-        // through to the general path
+        // reconstruction at once; its round 0, which reads every pixel of g and f, also checks
+        // the assumption and flags a pixel with g > f, after which the later rounds exit at
+        // once.  No separate order pass and no host round trip before the rounds; on a failed
+        // guess fall through to the general path.
         if (K != LEV_HIGH && rec_u8_applicable(se, d)) {
-          OrderArgs<T> oa{r, g, n};
           bool aborted = false;
-          PM_TRY(rec_u8_guarded<true>(g, r, dst, d, se, marker_order_launch<T>, &oa, 1, &aborted, s));
+          PM_TRY(rec_u8_guarded<true>(g, r, dst, d, se, nullptr, nullptr, 1, &aborted, s));
           if (!aborted) return PM_OK;
         }
       }

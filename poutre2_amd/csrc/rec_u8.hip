@@ -149,7 +149,7 @@ __device__ __forceinline__ uint32_t to_other_layout(uint32_t (&v)[64], uint8_t *
 template <bool DIL, int NB>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void rec_u8_kernel(
     RecImage one, const RecImage *__restrict__ many, int nimg, int nt, int round, int *__restrict__ act,
-    int *__restrict__ moved, const int *__restrict__ guard, int gmask)
+    int *__restrict__ moved, int *__restrict__ guard, int gmask, int chk)
 {
   if (round > 0 && moved[round - 1] == 0) return;  // converged in an earlier round
   if (guard && (*guard & gmask)) return;            // speculation failed: nothing to do
@@ -250,7 +250,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
   }
   // round 0 starts from marker ^ mask (marker v mask for erosion): the reference's
   // select_marker step; halo pixels (rows 0 / 63 = lanes 0 / 63, columns 0 / 63) get
-  // mask = value, which pins them
+  // mask = value, which pins them.  With chk, round 0 also notes whether some interior pixel
+  // of the marker lies beyond the mask (above it for a dilation), which a guarded caller
+  // assumed not to happen (pixels outside the image hold the neutral value in both).
+  bool beyond = false;
 #pragma unroll
   for (int k = 0; k < 16; k++) {
     uint32_t mw = 0;
@@ -258,12 +261,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
     for (int b = 0; b < 4; b++) {
       const int j = 4 * k + b;
       uint32_t mk = (mB[k] >> (8 * b)) & 0xffu, val = v[j];
+      if (round == 0 && chk && j != 0 && j != 63) beyond |= DIL ? val > mk : val < mk;
       if (round == 0) val = DIL ? min(val, mk) : max(val, mk);
       if (lane_halo || j == 0 || j == 63) mk = val;
       v[j] = val;
       mw |= mk << (8 * b);
     }
     mB[k] = mw;
+  }
+  if (round == 0 && chk) {
+    const bool b = __any(beyond && !lane_halo);
+    if (b && lane == 0) atomicOr(guard, gmask);
   }
 
   // border values at the start: rows 1 / 62 (lanes 1 / 62: their row sums), columns 1 / 62
@@ -390,8 +398,10 @@ static size_t rec_scratch_ints(int64_t nt, bool guarded = false) { return (size_
 // `hmoved` host memory for PM_COUNTERS counters, and one more int with a guard (pinned, or any
 // host memory when the caller cannot allocate).
 // Optional guard: after the initialisation, guard->launch(guard->ctx, flag, s) enqueues the
-// work that computes the flag (zeroed by the initialisation); every round exits at once when
-// (flag & guard->mask) != 0, and the host then reports *guard->aborted = true.
+// work that computes the flag (zeroed by the initialisation), or, with launch == nullptr,
+// round 0 ORs guard->mask into the flag when some marker pixel lies beyond the mask (above
+// it for a dilation); every round exits at once when (flag & guard->mask) != 0, and the host
+// then reports *guard->aborted = true.
 struct RecGuard {
   int (*launch)(void *ctx, int *flag, hipStream_t s);
   void *ctx;
@@ -411,7 +421,8 @@ static int rec_rounds(const RecImage &one, const RecImage *many, int nimg, int n
   const int ninit = nt > nm ? nt : nm;
   hipLaunchKernelGGL(rec_init_kernel, dim3((ninit + 255) / 256), dim3(256), 0, s, act, nt, moved, nm);
   PM_HIP(hipGetLastError());
-  if (gd) PM_TRY(gd->launch(gd->ctx, guard, s));
+  if (gd && gd->launch) PM_TRY(gd->launch(gd->ctx, guard, s));
+  const int chk = (gd && !gd->launch) ? 1 : 0;  // round 0 computes the guard itself
   int nbk = NBK_SQUARE;
   if (se == PM_SE_CROSS_2D) nbk = NBK_CROSS;
   else if (se == PM_SE_SEGMENT_X_1D || se == PM_SE_SEGMENT_X_2D) nbk = NBK_SEGX;
@@ -426,19 +437,19 @@ static int rec_rounds(const RecImage &one, const RecImage *many, int nimg, int n
       switch (nbk) {
       case NBK_SQUARE:
         hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SQUARE>), dim3(grid), dim3(64), 0, s, one, many, nimg, nt, R, act, moved,
-                           guard, gmask);
+                           guard, gmask, chk);
         break;
       case NBK_CROSS:
         hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_CROSS>), dim3(grid), dim3(64), 0, s, one, many, nimg, nt, R, act, moved,
-                           guard, gmask);
+                           guard, gmask, chk);
         break;
       case NBK_SEGX:
         hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SEGX>), dim3(grid), dim3(64), 0, s, one, many, nimg, nt, R, act, moved,
-                           guard, gmask);
+                           guard, gmask, chk);
         break;
       default:
         hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SEGY>), dim3(grid), dim3(64), 0, s, one, many, nimg, nt, R, act, moved,
-                           guard, gmask);
+                           guard, gmask, chk);
         break;
       }
     }

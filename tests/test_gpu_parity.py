@@ -288,6 +288,34 @@ def test_leveling_ordered_marker(op, dtype, order):
     np.testing.assert_array_equal(got, getattr(oracle, op)(ref, marker, "SESquare2D"))
 
 
+@pytest.mark.parametrize("pos,vertical", [((0, 61), True), ((0, 62), True), ((149, 123), True), ((61, 124), True),
+                                          ((62, 1), True), ((149, 168), True), ((75, 61), True), ((61, 0), False),
+                                          ((62, 169), False), ((123, 0), False), ((124, 169), False), ((1, 0), False),
+                                          ((148, 169), False)])
+def test_leveling_marker_above_at_one_pixel(pos, vertical):
+    """The uint8 leveling assumes a marker below the reference and lets the reconstruction's
+    first round check that on every pixel: here the marker exceeds the reference at ONE pixel
+    (tile corners and edges, image borders), a gap in a low wall of the reference through which
+    the true leveling (low(f, g) = f ^ R_{f v g}(g)) floods the far side, while the assumed form
+    R_f(g) cannot: a missed pixel gives a different image."""
+    H, W = 150, 170
+    y, x = pos
+    f = np.full((H, W), 100, np.uint8)
+    m = np.zeros((H, W), np.uint8)
+    if vertical:
+        f[:, x] = 10
+        m[:, :x] = 100
+    else:
+        f[y, :] = 10
+        m[:y, :] = 100
+    m[y, x] = 100
+    expected = oracle.leveling(f, m, "SESquare2D")
+    assert (expected != oracle.reconstruct("dilate", m, f, "SESquare2D")).any()
+    for op in ("leveling", "low_leveling"):
+        np.testing.assert_array_equal(h(getattr(pm, op)(g(f), g(m), PM_SE["SESquare2D"])),
+                                      getattr(oracle, op)(f, m, "SESquare2D"))
+
+
 @pytest.mark.parametrize("op", ["h_maxima", "h_minima", "h_convex", "h_concave", "dynamic_pseudo_opening",
                                 "dynamic_pseudo_closing"])
 @pytest.mark.parametrize("dtype", [np.uint8, np.int32])
