@@ -4,6 +4,8 @@
 // uint8 (ero_dil_static.cpp:229-305, ero_dil_line_se_t.hpp:49-280): one launch reads
 // every input pixel once from HBM and writes every output once.
 //
+// (The 3 x 3 square on images with 16-byte aligned rows runs the register-streamed kernel of
+// sq_u8_rs.hip instead.)
 // One workgroup (256 threads) owns an output tile of TH = 64 (r <= 3) or 128 rows and TW = 128,
 // 192 or 256 columns
 // (per device and radius: the table pm_autotune fills, else the rule sq_tw).
@@ -75,6 +77,7 @@ static int sq_launch(bool dil, bool aligned, int tw, const uint8_t *in, uint8_t 
 template <bool DIL>
 static int sq_u8_once(const uint8_t *in, uint8_t *out, int64_t B, int64_t H, int64_t W, int r, hipStream_t s)
 {
+  if (r == 1 && sq_u8_rs_applicable(in, out, H, W)) return sq_u8_rs<DIL>(in, out, B, H, W, s);
   const bool aligned = (W % 16) == 0 && ((uintptr_t)in % 16) == 0 && ((uintptr_t)out % 16) == 0;
   int dev = 0, cus = 0;
   PM_HIP(hipGetDevice(&dev));

@@ -12,6 +12,11 @@ bool sq_u8_applicable(const void *in, const void *out, int64_t H, int64_t W);
 // (2r+1)^2 square on B images of H x W; r > SQ_U8_RMAX runs several launches.
 template <bool DIL> int sq_u8(const uint8_t *in, uint8_t *out, int64_t B, int64_t H, int64_t W, int64_t r, hipStream_t s);
 
+// The 3 x 3 square (r = 1) on B images whose rows are 16-byte aligned, streamed through
+// registers (sq_u8_rs.hip); sq_u8 takes it for such images.
+bool sq_u8_rs_applicable(const void *in, const void *out, int64_t H, int64_t W);
+template <bool DIL> int sq_u8_rs(const uint8_t *in, uint8_t *out, int64_t B, int64_t H, int64_t W, hipStream_t s);
+
 // Dilate - Erode by the (2r+1)^2 square on B images of H x W in one launch,
 // 1 <= r <= SQ_U8_GRAD_RMAX
 constexpr int64_t SQ_U8_GRAD_RMAX = 32;

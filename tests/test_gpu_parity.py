@@ -419,6 +419,19 @@ def test_open_close_u8_fused_unaligned_view(ioff, ooff):
         assert (rest[:ooff] == 5).all() and (rest[ooff + H * W:] == 5).all()
 
 
+@pytest.mark.parametrize("shape", [(1, 16), (2, 32), (3, 1024), (9, 1040), (17, 2064), (70, 4096), (257, 160),
+                                   (8, 1008), (16, 1024), (25, 3072)])
+def test_square_u8_streamed_3x3(shape):
+    """uint8 3x3 square on rows of a multiple of 16 bytes (the register-streamed kernel: bands of
+    8 rows, 64 chunks of 16 bytes per wave): partial bands, partial and single-chunk wave
+    column groups, the words beside each wave's chunks, image borders"""
+    img = rand(shape, np.uint8, 140 + shape[0])
+    for dil in (True, False):
+        f = pm.dilate if dil else pm.erode
+        np.testing.assert_array_equal(h(f(g(img), pm.static_nl_se.square2d, 1)),
+                                      oracle.static_iter(img, "SESquare2D", 1, dil))
+
+
 @pytest.mark.parametrize("dtype", [np.uint16, np.int32, np.int64, np.float32, np.float64])
 @pytest.mark.parametrize("shape", [(64, 64), (65, 70), (200, 130), (3, 5), (130, 257), (1, 300), (300, 1),
                                    (72, 136)])
