@@ -114,7 +114,9 @@ __global__ __launch_bounds__(RT) void rec_tile_kernel(T *__restrict__ cur, const
     const int gy = y0 + yy, gx = x0 + xx;
     if (gy < H && gx < W) cur[(int64_t)gy * W + gx] = c[(yy + 1) * RP + xx + 1];
   }
-  if (__ballot(border) != 0 && lane == 0) atomicOr(border_moved, 1);
+  // a plain store of 1 from every workgroup whose border moved (the flag only has to become
+  // non-zero; atomics to the one address would serialise)
+  if (__ballot(border) != 0 && lane == 0) *border_moved = 1;
 }
 
 template <typename T, bool DIL>
@@ -144,7 +146,7 @@ __global__ __launch_bounds__(256) void rec_nl_kernel(T *__restrict__ cur, const 
       chg = true;
     }
   }
-  if (__ballot(chg) != 0 && (threadIdx.x & 63) == 0) atomicOr(moved, 1);
+  if (__ballot(chg) != 0 && (threadIdx.x & 63) == 0) *moved = 1;  // plain store, as above
 }
 
 // Reconstruction of marker under mask into out (all device pointers, out distinct from both).
