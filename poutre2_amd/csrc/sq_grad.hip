@@ -72,6 +72,8 @@ int sq_u8_gradient(const uint8_t *in, uint8_t *out, int64_t B, int64_t H, int64_
   if (r < 1 || r > SQ_U8_GRAD_RMAX)
     PM_FAIL(PM_E_ARG, "fused gradient radius %lld outside [1, %d]", (long long)r, (int)SQ_U8_GRAD_RMAX);
   if (H > INT32_MAX / 2 || W > INT32_MAX / 2) PM_FAIL(PM_E_ARG, "image too large for the fused gradient");
+  // the 3 x 3 square on rows of a multiple of 16 bytes: streamed through registers
+  if (r == 1 && sq_u8_rs_applicable(in, out, H, W)) return sq_u8_rs_compose(2, in, out, B, H, W, s);
   const int tw = grad_tw(sq_k_open((int)r));
   const int tiles_x = (int)((W + tw - 1) / tw), tiles_y = (int)((H + FT - 1) / FT);
   const int64_t per = (int64_t)tiles_x * tiles_y, ntiles = per * B;

@@ -1,6 +1,7 @@
 // sq_open.hip -- launcher of the fused uint8 opening / closing (kernels: sq_open.hpp,
 // instantiated in sq_open_{o,c}{a,u}.hip).
 #include "sq_open.hpp"
+#include "sq_u8.hpp"
 
 namespace pm {
 
@@ -12,6 +13,8 @@ int sq_u8_open_close(bool open, const uint8_t *in, uint8_t *out, int64_t B, int6
   if (r < 1 || r > SQ_U8_OPEN_RMAX)
     PM_FAIL(PM_E_ARG, "fused opening/closing radius %lld outside [1, %d]", (long long)r, (int)SQ_U8_OPEN_RMAX);
   if (H > INT32_MAX / 2 || W > INT32_MAX / 2) PM_FAIL(PM_E_ARG, "image too large for the fused opening/closing");
+  // the 3 x 3 square on rows of a multiple of 16 bytes: streamed through registers
+  if (r == 1 && sq_u8_rs_applicable(in, out, H, W)) return sq_u8_rs_compose(open ? 0 : 1, in, out, B, H, W, s);
   const int fo = open_fo((int)r);
   const int64_t tiles_x = (W + OPEN_TW - 1) / OPEN_TW, tiles_y = (H + fo - 1) / fo;
   const int64_t per = tiles_x * tiles_y, ntiles = per * B;

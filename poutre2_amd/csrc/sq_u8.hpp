@@ -16,6 +16,8 @@ template <bool DIL> int sq_u8(const uint8_t *in, uint8_t *out, int64_t B, int64_
 // registers (sq_u8_rs.hip); sq_u8 takes it for such images.
 bool sq_u8_rs_applicable(const void *in, const void *out, int64_t H, int64_t W);
 template <bool DIL> int sq_u8_rs(const uint8_t *in, uint8_t *out, int64_t B, int64_t H, int64_t W, hipStream_t s);
+// The 3 x 3 opening (op 0), closing (1) or gradient (2) the same way, on such images.
+int sq_u8_rs_compose(int op, const uint8_t *in, uint8_t *out, int64_t B, int64_t H, int64_t W, hipStream_t s);
 
 // Dilate - Erode by the (2r+1)^2 square on B images of H x W in one launch,
 // 1 <= r <= SQ_U8_GRAD_RMAX

@@ -422,14 +422,22 @@ def test_open_close_u8_fused_unaligned_view(ioff, ooff):
 @pytest.mark.parametrize("shape", [(1, 16), (2, 32), (3, 1024), (9, 1040), (17, 2064), (70, 4096), (257, 160),
                                    (8, 1008), (16, 1024), (25, 3072)])
 def test_square_u8_streamed_3x3(shape):
-    """uint8 3x3 square on rows of a multiple of 16 bytes (the register-streamed kernel: bands of
-    8 rows, 64 chunks of 16 bytes per wave): partial bands, partial and single-chunk wave
-    column groups, the words beside each wave's chunks, image borders"""
+    """uint8 3x3 square, opening, closing and gradient on rows of a multiple of 16 bytes (the
+    register-streamed kernels: bands of 8 rows, 64 chunks of 16 bytes per wave): partial bands,
+    partial and single-chunk wave column groups, the words beside each wave's chunks, image
+    borders (where the first operator of a composition is padded with the second's neutral)"""
     img = rand(shape, np.uint8, 140 + shape[0])
+    d = oracle.static_iter(img, "SESquare2D", 1, True)
+    e = oracle.static_iter(img, "SESquare2D", 1, False)
     for dil in (True, False):
         f = pm.dilate if dil else pm.erode
-        np.testing.assert_array_equal(h(f(g(img), pm.static_nl_se.square2d, 1)),
-                                      oracle.static_iter(img, "SESquare2D", 1, dil))
+        np.testing.assert_array_equal(h(f(g(img), pm.static_nl_se.square2d, 1)), d if dil else e)
+    # the compositions on the same streams (two operator stages, or max and min at once)
+    np.testing.assert_array_equal(h(pm.opening(g(img), pm.static_nl_se.square2d, 1)),
+                                  oracle.static_iter(e, "SESquare2D", 1, True))
+    np.testing.assert_array_equal(h(pm.closing(g(img), pm.static_nl_se.square2d, 1)),
+                                  oracle.static_iter(d, "SESquare2D", 1, False))
+    np.testing.assert_array_equal(h(pm.gradient(g(img), pm.static_nl_se.square2d, 1)), d - e)
 
 
 @pytest.mark.parametrize("dtype", [np.uint16, np.int32, np.int64, np.float32, np.float64])
