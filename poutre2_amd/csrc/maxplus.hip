@@ -47,6 +47,44 @@ __device__ __forceinline__ void load_region(float *dst, int rows, int cols, cons
   }
 }
 
+// load_region for a region whose first column x0 - R lies R <= 4 floats left of a 16-byte
+// aligned image column x0 (rows 16-byte aligned: W % 4 == 0 and src aligned, x0 % 4 == 0):
+// the region rows are read as aligned float4 chunks starting at image column x0 - 4, one
+// 16-byte load per chunk inside the image (element loads with padding for chunks crossing
+// an image edge), every element landing in LDS column (image column) - (x0 - R) when that
+// lies in [0, cols).  A quarter of the loads and index computations of load_region.
+template <int R>
+__device__ __forceinline__ void load_region4(float *dst, int rows, int cols, const float *__restrict__ src, int H,
+                                             int W, int y0, int x0, float pad)
+{
+  static_assert(R >= 0 && R <= 4, "the region starts at most one chunk left of x0");
+  const int nk = (cols - R + 3) / 4 + 1;  // chunk k: image columns x0 - 4 + 4k .. x0 - 1 + 4k
+  const int n = rows * nk;
+  for (int e = threadIdx.x; e < n; e += MTHREADS) {
+    const int i = e / nk, k = e - i * nk;
+    const int gy = y0 + i, g0 = x0 - 4 + 4 * k;
+    float v[4];
+    if ((unsigned)gy < (unsigned)H && g0 >= 0 && g0 + 4 <= W) {
+      const float4 f = *reinterpret_cast<const float4 *>(src + (int64_t)gy * W + g0);
+      v[0] = f.x;
+      v[1] = f.y;
+      v[2] = f.z;
+      v[3] = f.w;
+    } else {
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int gx = g0 + u;
+        v[u] = ((unsigned)gy < (unsigned)H && (unsigned)gx < (unsigned)W) ? src[(int64_t)gy * W + gx] : pad;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int c = 4 * k - 4 + R + u;
+      if (c >= 0 && c < cols) dst[i * cols + c] = v[u];
+    }
+  }
+}
+
 // selected tap (raster index dy*K + dx) and value of output q whose window starts at row
 // pointer p (region pitch pitch); out-of-image taps hold +-inf and are never selected
 template <int K, bool DIL, bool FLAT>
@@ -87,7 +125,7 @@ __device__ __forceinline__ int select_tap(const float *p, int pitch, const float
 template <int K, bool DIL, bool FLAT, bool SEL>
 __global__ __launch_bounds__(MTHREADS) void mp_fwd_kernel(const float *__restrict__ x, const float *__restrict__ wg,
                                                           float *__restrict__ y, uint8_t *__restrict__ selo, int H,
-                                                          int W, int tiles_x, int tiles_per_img)
+                                                          int W, int tiles_x, int tiles_per_img, int al)
 {
   constexpr int R = (K - 1) / 2, RR = MTH_F + K - 1;
   constexpr int RCP = (MTW + K - 1 + 3) & ~3;  // region pitch, a multiple of 4 floats
@@ -99,7 +137,8 @@ __global__ __launch_bounds__(MTHREADS) void mp_fwd_kernel(const float *__restric
   const int ty = t / tiles_x, tx = t - ty * tiles_x;
   const int y0 = ty * MTH_F, x0 = tx * MTW;
   const int64_t ib = (int64_t)img * H * W;
-  load_region(X, RR, RCP, x + ib, H, W, y0 - R, x0 - R, pad_value<DIL>());
+  if (al) load_region4<R>(X, RR, RCP, x + ib, H, W, y0 - R, x0, pad_value<DIL>());  // 16-byte aligned rows
+  else load_region(X, RR, RCP, x + ib, H, W, y0 - R, x0 - R, pad_value<DIL>());
   float wv[K * K];  // uniform: scalar loads
 #pragma unroll
   for (int k = 0; k < K * K; k++) wv[k] = FLAT ? 0.f : wg[k];
@@ -464,20 +503,17 @@ __global__ __launch_bounds__(MTHREADS) void mp_bwd_flat_kernel(const uint8_t *__
         bad |= (int)(hit & (ad >= 0) & (ad != (int)d));
         ad = hit ? (int)d : ad;
       }
-      float acc = 0.0f;
+      // stage B: lane l + k holds output column ox + k - R, whose tap column 2R - k lands on
+      // ox.  With part(dx) the lane's sum for tap column dx, gx = sum over dx ascending of
+      // part(dx) from lane l + 2R - dx, accumulated Horner-wise (h = h from lane l + 1 +
+      // part(dx), one DPP add per step): lane-wise the same additions in the same order as
+      // shifting every part(dx) into place and summing (no part(dx) is -0, so the leading
+      // 0 + part(0) of that sum changes nothing).
+      float acc;
       if (!__any(bad)) {
-        // stage B: lane l + k holds output column ox + k - R, whose tap column 2R - k lands on ox
-        float gs[2 * R + 1];
-        int as[2 * R + 1];
-        gs[0] = GR;
-        as[0] = ad;
+        acc = (ad == 0) ? GR : 0.0f;
 #pragma unroll
-        for (int k = 1; k <= 2 * R; k++) {
-          gs[k] = from_right(gs[k - 1]);
-          as[k] = from_right(as[k - 1]);
-        }
-#pragma unroll
-        for (int dx = 0; dx < K; dx++) acc += (as[2 * R - dx] == dx) ? gs[2 * R - dx] : 0.0f;
+        for (int dx = 1; dx < K; dx++) acc = from_right(acc) + ((ad == dx) ? GR : 0.0f);
       } else {
         float part[K];
 #pragma unroll
@@ -488,13 +524,9 @@ __global__ __launch_bounds__(MTHREADS) void mp_bwd_flat_kernel(const uint8_t *__
 #pragma unroll
           for (int dx = 0; dx < K; dx++) part[dx] += (d == dx) ? gv[2 * R - dy] : 0.0f;
         }
+        acc = part[0];
 #pragma unroll
-        for (int dx = 0; dx < K; dx++) {
-          float v = part[dx];
-#pragma unroll
-          for (int k = 0; k < 2 * R - dx; k++) v = from_right(v);
-          acc += v;
-        }
+        for (int dx = 1; dx < K; dx++) acc = from_right(acc) + part[dx];
       }
       if (out_lane) gx[ib + (int64_t)r * W + ws.ox] = acc;
       if (GWL) {
@@ -651,16 +683,17 @@ static int maxplus_fwd(const float *x, int64_t B, int64_t H, int64_t W, const fl
   if (nb > INT32_MAX) PM_FAIL(PM_E_SIZE, "maxplus: batch too large");
   hipStream_t s = as_stream(stream);
   const bool flat = (w == nullptr);
+  const int al = (W % 4 == 0 && ((uintptr_t)x & 15) == 0) ? 1 : 0;  // rows of 16-byte aligned float4
   const int64_t nbs = mp_strips(B, H, W, K, WNR, strips_x, sper, nwaves);
   if (flat && nbs < 0) PM_FAIL(PM_E_SIZE, "maxplus: batch too large");
 #define L(KK, SELV)                                                                                          \
   do {                                                                                                       \
     if (dilate) {                                                                                            \
       if (flat) hipLaunchKernelGGL((mp_fwd_flat_kernel<KK, true, SELV>), dim3((unsigned)nbs), dim3(MTHREADS), 0, s, x, y, selo, (int)H, (int)W, strips_x, sper, nwaves); \
-      else hipLaunchKernelGGL((mp_fwd_kernel<KK, true, false, SELV>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, x, w, y, selo, (int)H, (int)W, tiles_x, per); \
+      else hipLaunchKernelGGL((mp_fwd_kernel<KK, true, false, SELV>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, x, w, y, selo, (int)H, (int)W, tiles_x, per, al); \
     } else {                                                                                                 \
       if (flat) hipLaunchKernelGGL((mp_fwd_flat_kernel<KK, false, SELV>), dim3((unsigned)nbs), dim3(MTHREADS), 0, s, x, y, selo, (int)H, (int)W, strips_x, sper, nwaves); \
-      else hipLaunchKernelGGL((mp_fwd_kernel<KK, false, false, SELV>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, x, w, y, selo, (int)H, (int)W, tiles_x, per); \
+      else hipLaunchKernelGGL((mp_fwd_kernel<KK, false, false, SELV>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, x, w, y, selo, (int)H, (int)W, tiles_x, per, al); \
     }                                                                                                        \
   } while (0)
 #define LS(KK) L(KK, true)
