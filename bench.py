@@ -457,6 +457,17 @@ def bench_autograd(pm, torch, dev, reps=3):
     tp = timed(plain, 1, stream)
     res["dilate3"]["plain_per_image_fwd_ms"] = round(tp, 3)
     res["dilate3"]["fwd_speedup_vs_per_image"] = round(tp / res["dilate3"]["fwd_ms"], 2)
+    # the plain float32 square kernel on one image of the same pixel count (8192 x 16384, the
+    # same 512 MiB viewed as one image), 3x3 and 7x7: the kernel-level comparison
+    big = x.view(8192, 16384)
+    outb = torch.empty_like(big)
+    for name, it in (("dilate3", 1), ("dilate7", 3)):
+        def plain_big(it=it):
+            pm.dilate(big, pm.static_nl_se.square2d, it, out=outb)
+        plain_big()
+        tq = timed(plain_big, reps, stream) / reps
+        res[name]["plain_square_kernel_same_px_ms"] = round(tq, 3)
+        res[name]["fwd_vs_plain_square_kernel"] = round(tq / res[name]["fwd_ms"], 3)
     return res
 
 

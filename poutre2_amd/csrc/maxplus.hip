@@ -689,10 +689,12 @@ static int maxplus_fwd(const float *x, int64_t B, int64_t H, int64_t W, const fl
 #define L(KK, SELV)                                                                                          \
   do {                                                                                                       \
     if (dilate) {                                                                                            \
-      if (flat) hipLaunchKernelGGL((mp_fwd_flat_kernel<KK, true, SELV>), dim3((unsigned)nbs), dim3(MTHREADS), 0, s, x, y, selo, (int)H, (int)W, strips_x, sper, nwaves); \
+      if (flat && !al) hipLaunchKernelGGL((mp_fwd_flat_kernel<KK, true, SELV>), dim3((unsigned)nbs), dim3(MTHREADS), 0, s, x, y, selo, (int)H, (int)W, strips_x, sper, nwaves); \
+      else if (flat) hipLaunchKernelGGL((mp_fwd_kernel<KK, true, true, SELV>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, x, w, y, selo, (int)H, (int)W, tiles_x, per, al); \
       else hipLaunchKernelGGL((mp_fwd_kernel<KK, true, false, SELV>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, x, w, y, selo, (int)H, (int)W, tiles_x, per, al); \
     } else {                                                                                                 \
-      if (flat) hipLaunchKernelGGL((mp_fwd_flat_kernel<KK, false, SELV>), dim3((unsigned)nbs), dim3(MTHREADS), 0, s, x, y, selo, (int)H, (int)W, strips_x, sper, nwaves); \
+      if (flat && !al) hipLaunchKernelGGL((mp_fwd_flat_kernel<KK, false, SELV>), dim3((unsigned)nbs), dim3(MTHREADS), 0, s, x, y, selo, (int)H, (int)W, strips_x, sper, nwaves); \
+      else if (flat) hipLaunchKernelGGL((mp_fwd_kernel<KK, false, true, SELV>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, x, w, y, selo, (int)H, (int)W, tiles_x, per, al); \
       else hipLaunchKernelGGL((mp_fwd_kernel<KK, false, false, SELV>), dim3((unsigned)nb), dim3(MTHREADS), 0, s, x, w, y, selo, (int)H, (int)W, tiles_x, per, al); \
     }                                                                                                        \
   } while (0)
