@@ -348,7 +348,7 @@ static int leveling_entry(const pm_image *ref, const pm_image *marker, int se, p
         // guess fall through to the general path.
         if (K != LEV_HIGH && rec_u8_applicable(se, d)) {
           bool aborted = false;
-          PM_TRY(rec_u8_guarded<true>(g, r, dst, d, se, nullptr, nullptr, 1, &aborted, s));
+          PM_TRY(rec_u8_guarded<true>(g, r, dst, d, se, &aborted, s));
           if (!aborted) return PM_OK;
         }
       }

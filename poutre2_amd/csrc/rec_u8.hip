@@ -18,9 +18,12 @@
 //    with lanes = rows;
 //  * the tile iterates vertical/horizontal sweeps until a full iteration changes nothing.
 // Global rounds: a round launches one workgroup per tile; a tile runs in round R only if its
-// tag act[t] == R; when a tile's border moves it tags its neighbours R+1 and sets moved[R].  Rounds are enqueued in batches without host synchronisation; round R exits
-// at once if moved[R-1] == 0.  Round 0 reads marker ^ mask (marker v mask for erosion)
-// instead of the current image, fusing the reference's select_marker step.
+// tag act[t] equals round R's tag; when a tile's border moves it gives its neighbours round
+// R+1's tag and stores round R's tag in moved[R].  Rounds are enqueued in batches without host
+// synchronisation; round R exits at once if moved[R-1] does not hold round R-1's tag.  A
+// call's tags are base + R with a base above every value its scratch holds from earlier calls,
+// so the scratch needs no clearing between calls.  Round 0 reads marker ^ mask (marker v mask
+// for erosion) instead of the current image, fusing the reference's select_marker step.
 #include <mutex>
 #include <vector>
 
@@ -147,11 +150,12 @@ __device__ __forceinline__ uint32_t to_other_layout(uint32_t (&v)[64], uint8_t *
 
 template <bool DIL, int NB>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void rec_u8_kernel(
-    RecImage one, const RecImage *__restrict__ many, int nimg, int nt, int round, int *__restrict__ act,
-    int *__restrict__ moved, int *__restrict__ guard, int gmask, int chk)
+    RecImage one, const RecImage *__restrict__ many, int nimg, int nt, int round, int base, int *__restrict__ act,
+    int *__restrict__ moved, int *__restrict__ guard)
 {
-  if (round > 0 && moved[round - 1] == 0) return;  // converged in an earlier round
-  if (guard && (*guard & gmask)) return;            // speculation failed: nothing to do
+  const int tag = base + round;                          // this round's tag
+  if (round > 0 && moved[round - 1] != tag - 1) return;  // converged in an earlier round
+  if (guard && *guard == base) return;                   // speculation failed: nothing to do
   // XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin, so XCD x (block
   // x mod 8) takes the contiguous tile range [x * per, (x + 1) * per): horizontally adjacent
   // tiles, which share cache lines, meet in the same L2
@@ -160,7 +164,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
   if (t >= nt) return;
   // every tile runs in round 0 (it initialises its part of cur); later rounds run the tiles
   // tagged by a neighbour whose border moved in the previous round
-  if (round > 0 && act[t] != round) return;
+  if (round > 0 && act[t] != tag) return;
   RecImage im = many ? many[rec_find(many, nimg, t)] : one;
   // the descriptor is the same for the whole wave: keep it (and everything derived from it)
   // in scalar registers
@@ -249,8 +253,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
   }
   // round 0 starts from marker ^ mask (marker v mask for erosion): the reference's
   // select_marker step; halo pixels (rows 0 / 63 = lanes 0 / 63, columns 0 / 63) get
-  // mask = value, which pins them.  With chk, round 0 also notes whether some interior pixel
-  // of the marker lies beyond the mask (above it for a dilation), which a guarded caller
+  // mask = value, which pins them.  With a guard, round 0 also notes whether some interior
+  // pixel of the marker lies beyond the mask (above it for a dilation), which a guarded caller
   // assumed not to happen (pixels outside the image hold the neutral value in both).
   bool beyond = false;
 #pragma unroll
@@ -260,7 +264,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
     for (int b = 0; b < 4; b++) {
       const int j = 4 * k + b;
       uint32_t mk = (mB[k] >> (8 * b)) & 0xffu, val = v[j];
-      if (round == 0 && chk && j != 0 && j != 63) beyond |= DIL ? val > mk : val < mk;
+      if (round == 0 && guard && j != 0 && j != 63) beyond |= DIL ? val > mk : val < mk;
       if (round == 0) val = DIL ? min(val, mk) : max(val, mk);
       if (lane_halo || j == 0 || j == 63) mk = val;
       v[j] = val;
@@ -268,9 +272,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
     }
     mB[k] = mw;
   }
-  if (round == 0 && chk) {
+  if (round == 0 && guard) {
     const bool b = __any(beyond && !lane_halo);
-    if (b && lane == 0) atomicOr(guard, gmask);
+    if (b && lane == 0) *guard = base;  // the call's tag: every writer stores the same value
   }
 
   // border values at the start: rows 1 / 62 (lanes 1 / 62: their row sums), columns 1 / 62
@@ -361,7 +365,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
   const bool right = __any(!lane_halo && v[62] != right0);
   if (lane == 0 && (top || bot || left || right)) {
     const int tiles_y = im.tiles_y;
-    const int nr = round + 1;
+    const int nr = tag + 1;  // round R+1's tag
     for (int dy = -1; dy <= 1; dy++)
       for (int dx = -1; dx <= 1; dx++) {
         if (dx == 0 && dy == 0) continue;
@@ -370,11 +374,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
         if (need && (unsigned)ny < (unsigned)tiles_y && (unsigned)nx < (unsigned)tiles_x)
           act[im.tile0 + ny * tiles_x + nx] = nr;
       }
-    // moved[R] only has to become non-zero: a plain store of 1 (the same value from every
+    // moved[R] only has to take round R's tag: a plain store (the same value from every
     // writer; the kernel's end publishes it to the next round and to the host copy).  An
     // atomicAdd from every moved tile serialised on the one address: ~4000 device-scope
     // atomics per round bounded rounds 0 and 1 at ~75-80 us each on 4096^2.
-    moved[round] = 1;
+    moved[round] = tag;
   }
 }
 
@@ -393,83 +397,164 @@ bool rec_u8_applicable(int se, Dims d)
 }
 
 // ints of device scratch a reconstruction over nt tiles uses: one activity tag per tile, one
-// move counter per round
-static size_t rec_scratch_ints(int64_t nt, bool guarded = false) { return (size_t)nt + REC_MAX_ROUNDS + (guarded ? 1 : 0); }
+// move tag per round, and with gslot the guard of speculative calls
+static size_t rec_scratch_ints(int64_t nt, bool gslot) { return (size_t)nt + REC_MAX_ROUNDS + (gslot ? 1 : 0); }
 
 // Rounds driver shared by the single-image, batched and scratch-free entry points.  `sc` is
-// device scratch of rec_scratch_ints(nt, gd != nullptr) ints (cleared here by a fill kernel),
-// `hmoved` host memory for PM_COUNTERS counters, and one more int with a guard (pinned, or any
-// host memory when the caller cannot allocate).
-// Optional guard: after the initialisation, guard->launch(guard->ctx, flag, s) enqueues the
-// work that computes the flag (zeroed by the initialisation), or, with launch == nullptr,
-// round 0 ORs guard->mask into the flag when some marker pixel lies beyond the mask (above
-// it for a dilation); every round exits at once when (flag & guard->mask) != 0, and the host
-// then reports *guard->aborted = true.
-struct RecGuard {
-  int (*launch)(void *ctx, int *flag, hipStream_t s);
-  void *ctx;
-  int mask;
-  bool *aborted;
-};
-
+// device scratch of rec_scratch_ints(nt, gslot) ints and *hi the largest tag it may hold from
+// earlier calls (-1: unknown contents; then it is cleared by a fill kernel and tags start
+// again), updated here; `hmoved` is host memory for PM_COUNTERS move tags and the guard.
+// aborted != nullptr (needs gslot): the call is speculative.  Round 0 then stores the call's base tag in the
+// guard when some marker pixel lies beyond the mask (above it for a dilation), every round
+// exits at once when the guard holds it, and the host reports *aborted = true.
 template <bool DIL>
-static int rec_rounds(const RecImage &one, const RecImage *many, int nimg, int nt, int se, int *sc, int *hmoved,
-                      hipStream_t s, const RecGuard *gd = nullptr)
+static int rec_rounds(const RecImage &one, const RecImage *many, int nimg, int nt, int se, int *sc, bool gslot,
+                      int *hi, int *hmoved, hipStream_t s, bool *aborted = nullptr)
 {
+  if (aborted && !gslot) PM_FAIL(PM_E_ARG, "reconstruction: a speculative call needs a guard slot");
   int *act = sc;
   int *moved = act + nt;
-  int *guard = gd ? moved + REC_MAX_ROUNDS : nullptr;  // zeroed with the counters
-  const int gmask = gd ? gd->mask : 0;
-  const int nm = REC_MAX_ROUNDS + (gd ? 1 : 0);
-  const int ninit = nt > nm ? nt : nm;
-  hipLaunchKernelGGL(rec_init_kernel, dim3((ninit + 255) / 256), dim3(256), 0, s, act, nt, moved, nm);
-  PM_HIP(hipGetLastError());
-  if (gd && gd->launch) PM_TRY(gd->launch(gd->ctx, guard, s));
-  const int chk = (gd && !gd->launch) ? 1 : 0;  // round 0 computes the guard itself
+  int *guard = aborted ? moved + REC_MAX_ROUNDS : nullptr;
+  const int cap = round_cap(REC_MAX_ROUNDS);
+  if (*hi < 0 || *hi > INT32_MAX - cap - 2) {
+    const int nm = REC_MAX_ROUNDS + (gslot ? 1 : 0), ninit = nt > nm ? nt : nm;
+    hipLaunchKernelGGL(rec_init_kernel, dim3((ninit + 255) / 256), dim3(256), 0, s, act, nt, moved, nm);
+    PM_HIP(hipGetLastError());
+    *hi = 0;
+  }
+  const int base = *hi + 1;  // tags base .. base + rounds: above every value sc holds
+  *hi = -1;                  // until the rounds below are known to have ended
   int nbk = NBK_SQUARE;
   if (se == PM_SE_CROSS_2D) nbk = NBK_CROSS;
   else if (se == PM_SE_SEGMENT_X_1D || se == PM_SE_SEGMENT_X_2D) nbk = NBK_SEGX;
   else if (se == PM_SE_SEGMENT_Y_2D) nbk = NBK_SEGY;
   const unsigned grid = 8u * (unsigned)((nt + 7) / 8);  // every tile index below 8 * ceil(nt / 8)
   int round = 0, batch = 8;
-  const int cap = round_cap(REC_MAX_ROUNDS);
   while (round < cap) {
     const int nb = (round + batch <= cap) ? batch : cap - round;
     for (int i = 0; i < nb; i++) {
       const int R = round + i;
       switch (nbk) {
       case NBK_SQUARE:
-        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SQUARE>), dim3(grid), dim3(64), 0, s, one, many, nimg, nt, R, act, moved,
-                           guard, gmask, chk);
+        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SQUARE>), dim3(grid), dim3(64), 0, s, one, many, nimg, nt, R, base,
+                           act, moved, guard);
         break;
       case NBK_CROSS:
-        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_CROSS>), dim3(grid), dim3(64), 0, s, one, many, nimg, nt, R, act, moved,
-                           guard, gmask, chk);
+        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_CROSS>), dim3(grid), dim3(64), 0, s, one, many, nimg, nt, R, base,
+                           act, moved, guard);
         break;
       case NBK_SEGX:
-        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SEGX>), dim3(grid), dim3(64), 0, s, one, many, nimg, nt, R, act, moved,
-                           guard, gmask, chk);
+        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SEGX>), dim3(grid), dim3(64), 0, s, one, many, nimg, nt, R, base,
+                           act, moved, guard);
         break;
       default:
-        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SEGY>), dim3(grid), dim3(64), 0, s, one, many, nimg, nt, R, act, moved,
-                           guard, gmask, chk);
+        hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SEGY>), dim3(grid), dim3(64), 0, s, one, many, nimg, nt, R, base,
+                           act, moved, guard);
         break;
       }
     }
     PM_HIP(hipGetLastError());
     PM_HIP(hipMemcpyAsync(hmoved, moved + round, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
-    if (gd) PM_HIP(hipMemcpyAsync(hmoved + PM_COUNTERS, guard, sizeof(int), hipMemcpyDeviceToHost, s));
+    if (guard) PM_HIP(hipMemcpyAsync(hmoved + PM_COUNTERS, guard, sizeof(int), hipMemcpyDeviceToHost, s));
     PM_HIP(hipStreamSynchronize(s));
-    if (gd && (hmoved[PM_COUNTERS] & gmask)) {
-      *gd->aborted = true;
+    *hi = base + round + nb;  // the largest tag written so far (activity tags of round round+nb)
+    if (guard && hmoved[PM_COUNTERS] == base) {
+      *aborted = true;
       return PM_OK;
     }
     for (int i = 0; i < nb; i++)
-      if (hmoved[i] == 0) return PM_OK;
+      if (hmoved[i] != base + round + i) return PM_OK;
     round += nb;
     batch = batch * 2 > PM_COUNTERS ? PM_COUNTERS : batch * 2;
   }
   PM_FAIL(PM_E_NOCONV, "reconstruction did not converge");
+}
+
+// Device scratch of the rounds, reused across calls.  Every call synchronises its stream
+// before it returns, after which its buffer is idle, so a per-process list of idle buffers
+// (under a mutex, for callers on several host threads) replaces a stream-ordered allocation
+// and free per call: hipFreeAsync alone took 23-36 us of host time per 4096^2 call, after the
+// synchronisation, i.e. between the GPU's end of one call and the start of the next.  Each
+// buffer carries the largest tag it holds (rec_rounds), so it is never cleared again.
+struct RecBuf {
+  int dev = 0;
+  int *ptr = nullptr;
+  size_t ints = 0;
+  int hi = -1;
+};
+constexpr size_t REC_BUF_KEEP = 8;  // idle buffers kept (the smallest beyond this are freed)
+static std::mutex rec_buf_mu;
+static std::vector<RecBuf> rec_buf_idle;
+
+struct RecScratch {
+  RecBuf b;
+  bool idle = false;  // set when the call's rounds ended with a synchronisation
+  RecScratch() = default;
+  RecScratch(const RecScratch &) = delete;
+  RecScratch &operator=(const RecScratch &) = delete;
+  int take(size_t ints)
+  {
+    if (take_forced_alloc_failure())
+      PM_FAIL(PM_E_HIP, "scratch allocation of %zu bytes failed: out of device memory (forced by the testing hook)",
+              ints * sizeof(int));
+    int dev = 0;
+    PM_HIP(hipGetDevice(&dev));
+    {
+      std::lock_guard<std::mutex> lock(rec_buf_mu);
+      int best = -1;
+      for (int i = 0; i < (int)rec_buf_idle.size(); i++) {
+        const RecBuf &c = rec_buf_idle[i];
+        if (c.dev == dev && c.ints >= ints && (best < 0 || c.ints < rec_buf_idle[best].ints)) best = i;
+      }
+      if (best >= 0) {
+        b = rec_buf_idle[best];
+        rec_buf_idle.erase(rec_buf_idle.begin() + best);
+        return PM_OK;
+      }
+    }
+    void *p = nullptr;
+    const hipError_t e = hipMalloc(&p, ints * sizeof(int));
+    if (e != hipSuccess) PM_FAIL(PM_E_HIP, "scratch allocation of %zu bytes failed: %s", ints * sizeof(int), hipGetErrorString(e));
+    b.dev = dev;
+    b.ptr = (int *)p;
+    b.ints = ints;
+    b.hi = -1;
+    return PM_OK;
+  }
+  ~RecScratch()
+  {
+    if (!b.ptr) return;
+    if (!idle) {  // an error left work queued: hipFree waits for the device first
+      (void)hipFree(b.ptr);
+      return;
+    }
+    std::vector<RecBuf> drop;
+    {
+      std::lock_guard<std::mutex> lock(rec_buf_mu);
+      rec_buf_idle.push_back(b);
+      while (rec_buf_idle.size() > REC_BUF_KEEP) {
+        int small = 0;
+        for (int i = 1; i < (int)rec_buf_idle.size(); i++)
+          if (rec_buf_idle[i].ints < rec_buf_idle[small].ints) small = i;
+        drop.push_back(rec_buf_idle[small]);
+        rec_buf_idle.erase(rec_buf_idle.begin() + small);
+      }
+    }
+    for (const RecBuf &d : drop) (void)hipFree(d.ptr);  // idle buffers: nothing in flight uses them
+  }
+};
+
+template <bool DIL>
+static int rec_rounds_cached(const RecImage &one, const RecImage *many, int nimg, int nt, int se, hipStream_t s,
+                             bool *aborted = nullptr)
+{
+  RecScratch sc;
+  PM_TRY(sc.take(rec_scratch_ints(nt, true)));  // every cached buffer has the guard slot
+  int *h = nullptr;
+  PM_TRY(pinned_counters(&h));
+  const int rc = rec_rounds<DIL>(one, many, nimg, nt, se, sc.b.ptr, true, &sc.b.hi, h, s, aborted);
+  sc.idle = (rc == PM_OK || rc == PM_E_NOCONV);  // both return after the stream synchronisation
+  return rc;
 }
 
 static int rec_describe(const uint8_t *marker, const uint8_t *mask, uint8_t *out, Dims d, int64_t tile0, RecImage &r)
@@ -488,37 +573,21 @@ static int rec_describe(const uint8_t *marker, const uint8_t *mask, uint8_t *out
 }
 
 template <bool DIL>
-static int rec_rounds_alloc(const RecImage &one, const RecImage *many, int nimg, int nt, int se, hipStream_t s)
-{
-  Scratch sc;
-  PM_TRY(sc.alloc(sizeof(int) * rec_scratch_ints(nt), s));
-  int *h = nullptr;
-  PM_TRY(pinned_counters(&h));
-  return rec_rounds<DIL>(one, many, nimg, nt, se, (int *)sc.ptr, h, s);
-}
-
-template <bool DIL>
 int rec_u8(const uint8_t *marker, const uint8_t *mask, uint8_t *out, Dims d, int se, hipStream_t s)
 {
   RecImage one;
   PM_TRY(rec_describe(marker, mask, out, d, 0, one));
-  return rec_rounds_alloc<DIL>(one, nullptr, 1, one.tiles_x * one.tiles_y, se, s);
+  return rec_rounds_cached<DIL>(one, nullptr, 1, one.tiles_x * one.tiles_y, se, s);
 }
 
 template <bool DIL>
-int rec_u8_guarded(const uint8_t *marker, const uint8_t *mask, uint8_t *out, Dims d, int se,
-                   int (*launch)(void *, int *, hipStream_t), void *ctx, int gmask, bool *aborted, hipStream_t s)
+int rec_u8_guarded(const uint8_t *marker, const uint8_t *mask, uint8_t *out, Dims d, int se, bool *aborted,
+                   hipStream_t s)
 {
   *aborted = false;
   RecImage one;
   PM_TRY(rec_describe(marker, mask, out, d, 0, one));
-  const int nt = one.tiles_x * one.tiles_y;
-  Scratch sc;
-  PM_TRY(sc.alloc(sizeof(int) * rec_scratch_ints(nt, true), s));
-  int *h = nullptr;
-  PM_TRY(pinned_counters(&h));
-  const RecGuard gd{launch, ctx, gmask, aborted};
-  return rec_rounds<DIL>(one, nullptr, 1, nt, se, (int *)sc.ptr, h, s, &gd);
+  return rec_rounds_cached<DIL>(one, nullptr, 1, one.tiles_x * one.tiles_y, se, s, aborted);
 }
 
 // ---------------------------------------------------------------- scratch-free reconstruction
@@ -528,10 +597,13 @@ int rec_u8_guarded(const uint8_t *marker, const uint8_t *mask, uint8_t *out, Dim
 constexpr size_t NOALLOC_SCRATCH_INTS = (size_t)1 << 18;  // 1 MiB
 static __device__ __align__(256) int scratch[NOALLOC_SCRATCH_INTS];
 
-static int noalloc_scratch(int **ptr)
+// The array's address on the current device and the largest tag it holds (see rec_rounds;
+// -1 until its first use on that device).
+static int noalloc_scratch(int **ptr, int **hi)
 {
   static std::mutex mu;
   static int *addr[PM_MAX_DEVICES] = {};
+  static int tags[PM_MAX_DEVICES];
   int dev = 0;
   PM_HIP(hipGetDevice(&dev));
   if (dev < 0 || dev >= PM_MAX_DEVICES) PM_FAIL(PM_E_ARG, "device %d out of range", dev);
@@ -542,17 +614,17 @@ static int noalloc_scratch(int **ptr)
     if (e != hipSuccess || !p)
       PM_FAIL(PM_E_HIP, "scratch-free reconstruction: hipGetSymbolAddress failed: %s", hipGetErrorString(e));
     addr[dev] = (int *)p;
+    tags[dev] = -1;
   }
   *ptr = addr[dev];
+  *hi = &tags[dev];
   return PM_OK;
 }
 
 template int rec_u8<true>(const uint8_t *, const uint8_t *, uint8_t *, Dims, int, hipStream_t);
 template int rec_u8<false>(const uint8_t *, const uint8_t *, uint8_t *, Dims, int, hipStream_t);
-template int rec_u8_guarded<true>(const uint8_t *, const uint8_t *, uint8_t *, Dims, int,
-                                  int (*)(void *, int *, hipStream_t), void *, int, bool *, hipStream_t);
-template int rec_u8_guarded<false>(const uint8_t *, const uint8_t *, uint8_t *, Dims, int,
-                                   int (*)(void *, int *, hipStream_t), void *, int, bool *, hipStream_t);
+template int rec_u8_guarded<true>(const uint8_t *, const uint8_t *, uint8_t *, Dims, int, bool *, hipStream_t);
+template int rec_u8_guarded<false>(const uint8_t *, const uint8_t *, uint8_t *, Dims, int, bool *, hipStream_t);
 
 }  // namespace pm
 
@@ -582,8 +654,8 @@ extern "C" int pm_batch_reconstruct(int n, int rec_type, const pm_image *markers
   Scratch dsc;
   PM_TRY(dsc.alloc(sizeof(RecImage) * (size_t)n, s));
   PM_HIP(hipMemcpyAsync(dsc.ptr, host.data(), sizeof(RecImage) * (size_t)n, hipMemcpyHostToDevice, s));
-  if (rec_type == PM_REC_DILATE) return rec_rounds_alloc<true>(host[0], (const RecImage *)dsc.ptr, n, (int)tiles, se, s);
-  return rec_rounds_alloc<false>(host[0], (const RecImage *)dsc.ptr, n, (int)tiles, se, s);
+  if (rec_type == PM_REC_DILATE) return rec_rounds_cached<true>(host[0], (const RecImage *)dsc.ptr, n, (int)tiles, se, s);
+  return rec_rounds_cached<false>(host[0], (const RecImage *)dsc.ptr, n, (int)tiles, se, s);
 }
 
 // pm_reconstruct for uint8 2D images and 3x3-class SEs, allocating nothing: the activity tags
@@ -605,13 +677,13 @@ extern "C" int pm_reconstruct_noalloc(int rec_type, const pm_image *marker, cons
   RecImage one;
   PM_TRY(rec_describe((const uint8_t *)marker->data, (const uint8_t *)mask->data, (uint8_t *)out->data, d, 0, one));
   const int64_t nt = (int64_t)one.tiles_x * one.tiles_y;
-  if (rec_scratch_ints(nt) > NOALLOC_SCRATCH_INTS)
+  if (rec_scratch_ints(nt, false) > NOALLOC_SCRATCH_INTS)
     PM_FAIL(PM_E_ARG, "scratch-free reconstruction: %lld tiles of %dx%d exceed the %zu-tile limit", (long long)nt, RW,
             RW, NOALLOC_SCRATCH_INTS - REC_MAX_ROUNDS);
-  int *sc = nullptr;
-  PM_TRY(noalloc_scratch(&sc));
-  int hmoved[PM_COUNTERS];
+  int *sc = nullptr, *hi = nullptr;
+  PM_TRY(noalloc_scratch(&sc, &hi));
+  int hmoved[PM_COUNTERS + 1];
   hipStream_t s = as_stream(stream);
-  if (rec_type == PM_REC_DILATE) return rec_rounds<true>(one, nullptr, 1, (int)nt, se, sc, hmoved, s);
-  return rec_rounds<false>(one, nullptr, 1, (int)nt, se, sc, hmoved, s);
+  if (rec_type == PM_REC_DILATE) return rec_rounds<true>(one, nullptr, 1, (int)nt, se, sc, false, hi, hmoved, s);
+  return rec_rounds<false>(one, nullptr, 1, (int)nt, se, sc, false, hi, hmoved, s);
 }

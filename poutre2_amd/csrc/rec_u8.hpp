@@ -8,15 +8,11 @@ namespace pm {
 bool rec_u8_applicable(int se, Dims d);
 // Reconstruction of marker under mask into out (out distinct from both), by dilation (DIL) or erosion.
 template <bool DIL> int rec_u8(const uint8_t *marker, const uint8_t *mask, uint8_t *out, Dims d, int se, hipStream_t s);
-// The same, run speculatively under a device flag: after zeroing its counters and the flag,
-// it calls launch(ctx, flag, s) to enqueue the work that computes the flag, or, with launch ==
-// nullptr, its round 0 ORs gmask into the flag when some marker pixel lies beyond the mask
-// (above it for a dilation, below it for an erosion); every round exits at once when
-// (flag & gmask) != 0.  The host reads the flag with the round counters and then sets
-// *aborted and leaves out unspecified.
+// The same, run speculatively: its round 0 also checks whether some marker pixel lies beyond
+// the mask (above it for a dilation, below it for an erosion), which the caller assumed not to
+// happen; if one does, every round exits at once, *aborted is set and out is left unspecified.
 template <bool DIL>
-int rec_u8_guarded(const uint8_t *marker, const uint8_t *mask, uint8_t *out, Dims d, int se,
-                   int (*launch)(void *ctx, int *flag, hipStream_t s), void *ctx, int gmask, bool *aborted,
+int rec_u8_guarded(const uint8_t *marker, const uint8_t *mask, uint8_t *out, Dims d, int se, bool *aborted,
                    hipStream_t s);
 
 }  // namespace pm
