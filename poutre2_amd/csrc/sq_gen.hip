@@ -49,8 +49,10 @@ static int sq_gen_generic(const T *in, T *out, int64_t B, int64_t H, int64_t W, 
   int64_t g = (int64_t)(occ > 0 ? occ : 1) * cus;
   if (g >= 8) g &= ~(int64_t)7;
   if (g > ntiles) g = ntiles;
+  // rows of whole, 16-byte aligned chunks: the region is read chunk by chunk
+  const int al = (W % (16 / (int64_t)sizeof(T)) == 0 && ((uintptr_t)in & 15) == 0) ? 1 : 0;
   hipLaunchKernelGGL(k, dim3((unsigned)g), dim3(256), lds, s, in, out, (int)H, (int)W, (int)tiles_x, (int)per,
-                     (int)ntiles);
+                     (int)ntiles, al);
   PM_HIP(hipGetLastError());
   return PM_OK;
 }

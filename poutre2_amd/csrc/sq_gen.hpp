@@ -7,9 +7,12 @@
 //
 // One workgroup (256 threads) owns an output tile of 64 rows x TWE columns (gen_twe), in a
 // persistent loop over tiles (XCD-aware order, sq_u8_kernel.hpp):
-//  1. the region (64+2r rows x TWE+2r columns) is read element by element -- consecutive
-//     lanes on consecutive columns, all of a thread's loads in flight -- into LDS with an
-//     odd row pitch; pixels outside the image hold the operator's neutral value;
+//  1. the region (64+2r rows x TWE+2r columns) is read into LDS with an odd row pitch, all of
+//     a thread's loads in flight; pixels outside the image hold the operator's neutral value.
+//     Rows of 16-byte aligned chunks (al: W a multiple of 16 / sizeof(T), 16-byte aligned
+//     buffer) are read as aligned 16-byte chunks, one load per chunk inside the image;
+//     otherwise element by element, consecutive lanes on consecutive columns (4096^2 side 15
+//     with chunks: float32 43.1 -> 40.9 us, int32 40.4 -> 38.1 us; side 3 unchanged);
 //  2. horizontal pass: a lane owns one region row and a run of K output columns (lanes on
 //     consecutive rows, so the odd pitch puts them on distinct banks) and computes the
 //     window with a single-split van Herk run (suffix over the run, prefix over the rest:
@@ -82,7 +85,7 @@ template <typename T, int K, bool DIL, int R, typename LD> __device__ __forceinl
 
 template <typename T, bool DIL, int R>
 __global__ __launch_bounds__(256) void sq_gen_kernel(const T *__restrict__ in, T *__restrict__ out, int H, int W,
-                                                     int tiles_x, int tiles_per_img, int ntiles)
+                                                     int tiles_x, int tiles_per_img, int ntiles, int al)
 {
   using L = Lat<T, DIL>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -103,23 +106,65 @@ __global__ __launch_bounds__(256) void sq_gen_kernel(const T *__restrict__ in, T
     const T *src = in + (int64_t)img * H * W;
     T *dst = out + (int64_t)img * H * W;
     // ---- 1. region rows y0-R .., columns x0-R ..
-    T v[NL];
+    if (al) {
+      // aligned 16-byte chunks from image column x0 - CA (x0 and CA are multiples of EPC);
+      // element u of chunk k lands in region column EPC*k + u + R - CA when inside [0, RW)
+      constexpr int EPC = 16 / (int)sizeof(T), CA = (R + EPC - 1) / EPC * EPC;
+      constexpr int NK = (RW + CA - R + EPC - 1) / EPC, NCH = RH * NK, NC = (NCH + 255) / 256;
+      static_assert(TWE % EPC == 0, "tile columns start on chunk boundaries");
+      T v[NC][EPC];
 #pragma unroll
-    for (int i = 0; i < NL; i++) {
-      const int e = tid + i * 256;
-      v[i] = L::neutral();
-      if (e < NEL) {
-        const int row = e / RW, col = e - row * RW;
-        const int gy = y0 - R + row, gx = x0 - R + col;
-        if ((unsigned)gy < (unsigned)H && (unsigned)gx < (unsigned)W) v[i] = src[(int64_t)gy * W + gx];
+      for (int i = 0; i < NC; i++) {
+        const int e = tid + i * 256;
+#pragma unroll
+        for (int u = 0; u < EPC; u++) v[i][u] = L::neutral();
+        if (e < NCH) {
+          const int row = e / NK, k = e - row * NK;
+          const int gy = y0 - R + row, g0 = x0 - CA + EPC * k;
+          if ((unsigned)gy < (unsigned)H) {
+            const T *p = src + (int64_t)gy * W + g0;
+            if (g0 >= 0 && g0 + EPC <= W) {
+              const uint4 c = *reinterpret_cast<const uint4 *>(p);
+              __builtin_memcpy(v[i], &c, 16);
+            } else {
+#pragma unroll
+              for (int u = 0; u < EPC; u++)
+                if ((unsigned)(g0 + u) < (unsigned)W) v[i][u] = p[u];
+            }
+          }
+        }
       }
-    }
 #pragma unroll
-    for (int i = 0; i < NL; i++) {
-      const int e = tid + i * 256;
-      if (e < NEL) {
-        const int row = e / RW, col = e - row * RW;
-        S[row * P + col] = v[i];
+      for (int i = 0; i < NC; i++) {
+        const int e = tid + i * 256;
+        if (e < NCH) {
+          const int row = e / NK, k = e - row * NK;
+#pragma unroll
+          for (int u = 0; u < EPC; u++) {
+            const int c = EPC * k + u + R - CA;
+            if (c >= 0 && c < RW) S[row * P + c] = v[i][u];
+          }
+        }
+      }
+    } else {
+      T v[NL];
+#pragma unroll
+      for (int i = 0; i < NL; i++) {
+        const int e = tid + i * 256;
+        v[i] = L::neutral();
+        if (e < NEL) {
+          const int row = e / RW, col = e - row * RW;
+          const int gy = y0 - R + row, gx = x0 - R + col;
+          if ((unsigned)gy < (unsigned)H && (unsigned)gx < (unsigned)W) v[i] = src[(int64_t)gy * W + gx];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < NL; i++) {
+        const int e = tid + i * 256;
+        if (e < NEL) {
+          const int row = e / RW, col = e - row * RW;
+          S[row * P + col] = v[i];
+        }
       }
     }
     __syncthreads();
@@ -169,7 +214,7 @@ __global__ __launch_bounds__(256) void sq_gen_kernel(const T *__restrict__ in, T
   }
 }
 
-template <typename T, bool DIL> using gen_kernel_t = void (*)(const T *, T *, int, int, int, int, int);
+template <typename T, bool DIL> using gen_kernel_t = void (*)(const T *, T *, int, int, int, int, int, int);
 
 // kernel of radius r (1 .. SQ_GEN_RMAX), instantiated per type in sq_gen_<type>.hip
 template <typename T, bool DIL> gen_kernel_t<T, DIL> sq_gen_kernel_for(int r);
