@@ -151,7 +151,7 @@ __device__ __forceinline__ uint32_t to_other_layout(uint32_t (&v)[64], uint8_t *
 template <bool DIL, int NB>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void rec_u8_kernel(
     RecImage one, const RecImage *__restrict__ many, int nimg, int nt, int round, int base, int *__restrict__ act,
-    int *__restrict__ moved, int *__restrict__ guard)
+    int *__restrict__ moved, int *__restrict__ guard, int *__restrict__ hrep, int rep_round)
 {
   const int tag = base + round;                          // this round's tag
   if (round > 0 && moved[round - 1] != tag - 1) return;  // converged in an earlier round
@@ -274,7 +274,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
   }
   if (round == 0 && guard) {
     const bool b = __any(beyond && !lane_halo);
-    if (b && lane == 0) *guard = base;  // the call's tag: every writer stores the same value
+    if (b && lane == 0) {
+      *guard = base;  // the call's tag: every writer stores the same value
+      if (hrep) hrep[0] = base;
+    }
   }
 
   // border values at the start: rows 1 / 62 (lanes 1 / 62: their row sums), columns 1 / 62
@@ -379,6 +382,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
     // atomicAdd from every moved tile serialised on the one address: ~4000 device-scope
     // atomics per round bounded rounds 0 and 1 at ~75-80 us each on 4096^2.
     moved[round] = tag;
+    if (hrep && round == rep_round) hrep[1] = tag;  // the host's copy (pinned host memory)
   }
 }
 
@@ -403,13 +407,16 @@ static size_t rec_scratch_ints(int64_t nt, bool gslot) { return (size_t)nt + REC
 // Rounds driver shared by the single-image, batched and scratch-free entry points.  `sc` is
 // device scratch of rec_scratch_ints(nt, gslot) ints and *hi the largest tag it may hold from
 // earlier calls (-1: unknown contents; then it is cleared by a fill kernel and tags start
-// again), updated here; `hmoved` is host memory for PM_COUNTERS move tags and the guard.
+// again), updated here.  `hmoved`: with pinned (device-visible) host memory the kernels write
+// the host's report into it themselves (hmoved[0]: the guard, hmoved[1]: the tag of the
+// batch's last round if that round moved), which saves a copy per batch; otherwise it takes
+// PM_COUNTERS move tags and the guard by copies.
 // aborted != nullptr (needs gslot): the call is speculative.  Round 0 then stores the call's base tag in the
 // guard when some marker pixel lies beyond the mask (above it for a dilation), every round
 // exits at once when the guard holds it, and the host reports *aborted = true.
 template <bool DIL>
 static int rec_rounds(const RecImage &one, const RecImage *many, int nimg, int nt, int se, int *sc, bool gslot,
-                      int *hi, int *hmoved, hipStream_t s, bool *aborted = nullptr)
+                      int *hi, int *hmoved, bool pinned, hipStream_t s, bool *aborted = nullptr)
 {
   if (aborted && !gslot) PM_FAIL(PM_E_ARG, "reconstruction: a speculative call needs a guard slot");
   int *act = sc;
@@ -432,38 +439,44 @@ static int rec_rounds(const RecImage &one, const RecImage *many, int nimg, int n
   int round = 0, batch = 8;
   while (round < cap) {
     const int nb = (round + batch <= cap) ? batch : cap - round;
+    int *hrep = pinned ? hmoved : nullptr;
+    const int rep_round = round + nb - 1;
+    if (pinned) hmoved[0] = hmoved[1] = 0;  // no tag is 0; the previous batch has ended
     for (int i = 0; i < nb; i++) {
       const int R = round + i;
       switch (nbk) {
       case NBK_SQUARE:
         hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SQUARE>), dim3(grid), dim3(64), 0, s, one, many, nimg, nt, R, base,
-                           act, moved, guard);
+                           act, moved, guard, hrep, rep_round);
         break;
       case NBK_CROSS:
         hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_CROSS>), dim3(grid), dim3(64), 0, s, one, many, nimg, nt, R, base,
-                           act, moved, guard);
+                           act, moved, guard, hrep, rep_round);
         break;
       case NBK_SEGX:
         hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SEGX>), dim3(grid), dim3(64), 0, s, one, many, nimg, nt, R, base,
-                           act, moved, guard);
+                           act, moved, guard, hrep, rep_round);
         break;
       default:
         hipLaunchKernelGGL((rec_u8_kernel<DIL, NBK_SEGY>), dim3(grid), dim3(64), 0, s, one, many, nimg, nt, R, base,
-                           act, moved, guard);
+                           act, moved, guard, hrep, rep_round);
         break;
       }
     }
     PM_HIP(hipGetLastError());
-    PM_HIP(hipMemcpyAsync(hmoved, moved + round, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
-    if (guard) PM_HIP(hipMemcpyAsync(hmoved + PM_COUNTERS, guard, sizeof(int), hipMemcpyDeviceToHost, s));
+    if (!pinned) {
+      PM_HIP(hipMemcpyAsync(hmoved, moved + round, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
+      if (guard) PM_HIP(hipMemcpyAsync(hmoved + PM_COUNTERS, guard, sizeof(int), hipMemcpyDeviceToHost, s));
+    }
     PM_HIP(hipStreamSynchronize(s));
     *hi = base + round + nb;  // the largest tag written so far (activity tags of round round+nb)
-    if (guard && hmoved[PM_COUNTERS] == base) {
+    if (guard && (pinned ? hmoved[0] : hmoved[PM_COUNTERS]) == base) {
       *aborted = true;
       return PM_OK;
     }
-    for (int i = 0; i < nb; i++)
-      if (hmoved[i] != base + round + i) return PM_OK;
+    // rounds after the first that moved nothing exit at once, so the batch's last round moved
+    // iff every round of the batch did
+    if ((pinned ? hmoved[1] : hmoved[nb - 1]) != base + rep_round) return PM_OK;
     round += nb;
     batch = batch * 2 > PM_COUNTERS ? PM_COUNTERS : batch * 2;
   }
@@ -552,7 +565,7 @@ static int rec_rounds_cached(const RecImage &one, const RecImage *many, int nimg
   PM_TRY(sc.take(rec_scratch_ints(nt, true)));  // every cached buffer has the guard slot
   int *h = nullptr;
   PM_TRY(pinned_counters(&h));
-  const int rc = rec_rounds<DIL>(one, many, nimg, nt, se, sc.b.ptr, true, &sc.b.hi, h, s, aborted);
+  const int rc = rec_rounds<DIL>(one, many, nimg, nt, se, sc.b.ptr, true, &sc.b.hi, h, true, s, aborted);
   sc.idle = (rc == PM_OK || rc == PM_E_NOCONV);  // both return after the stream synchronisation
   return rc;
 }
@@ -684,6 +697,6 @@ extern "C" int pm_reconstruct_noalloc(int rec_type, const pm_image *marker, cons
   PM_TRY(noalloc_scratch(&sc, &hi));
   int hmoved[PM_COUNTERS + 1];
   hipStream_t s = as_stream(stream);
-  if (rec_type == PM_REC_DILATE) return rec_rounds<true>(one, nullptr, 1, (int)nt, se, sc, false, hi, hmoved, s);
-  return rec_rounds<false>(one, nullptr, 1, (int)nt, se, sc, false, hi, hmoved, s);
+  if (rec_type == PM_REC_DILATE) return rec_rounds<true>(one, nullptr, 1, (int)nt, se, sc, false, hi, hmoved, false, s);
+  return rec_rounds<false>(one, nullptr, 1, (int)nt, se, sc, false, hi, hmoved, false, s);
 }
