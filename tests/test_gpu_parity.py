@@ -237,6 +237,35 @@ def test_reconstruction_2d(dtype, se, shape, rec):
     np.testing.assert_array_equal(got, oracle.reconstruct(rec, marker, mask, se))
 
 
+def test_reconstruction_scratch_reuse_across_calls():
+    """uint8 reconstructions reuse their device scratch from call to call, its round tags
+    offset per call instead of cleared: alternating sizes (the larger buffer kept, a smaller
+    call reusing it, a larger one taking a new one), both operators, speculative leveling
+    calls (whose first round may abort) and the scratch-free entry point in between, every
+    result equal to the oracle's"""
+    import ctypes
+
+    from poutre2_amd import _capi, synth
+    seq = [((300, 310), "dilate", "SESquare2D"), ((64, 70), "erode", "SECross2D"),
+           ((300, 310), "erode", "SESquare2D"), ((520, 130), "dilate", "SECross2D"),
+           ((64, 70), "dilate", "SESquare2D"), ((300, 310), "dilate", "SECross2D")]
+    for k, (shape, rec, se) in enumerate(seq):
+        marker, mask = _recon_inputs(shape, np.uint8, 40 + k, sparse=(rec == "dilate"))
+        got = h(pm.reconstruction(pm.RecType[rec], g(marker), g(mask), PM_SE[se]))
+        np.testing.assert_array_equal(got, oracle.reconstruct(rec, marker, mask, se))
+        f = synth.blurred_noise_mask(shape[0], shape[1], 50 + k, sigma=3.0, radius=10)
+        for marker_img in (oracle.static_iter(f, "SESquare2D", 2, False), f ^ 1):  # below f; neither
+            got = h(pm.leveling(g(f), g(marker_img), PM_SE[se]))
+            np.testing.assert_array_equal(got, oracle.leveling(f, marker_img, se))
+        tm, tk = g(marker), g(mask)  # the descriptors hold raw pointers: keep the tensors alive
+        out = torch.empty_like(tk)
+        dm, dk, do = pm._desc(tm), pm._desc(tk), pm._desc(out)
+        s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        _capi.check(_capi.lib().pm_reconstruct_noalloc(int(pm.RecType[rec]), ctypes.byref(dm), ctypes.byref(dk),
+                                                       int(PM_SE[se]), ctypes.byref(do), s))
+        np.testing.assert_array_equal(h(out), oracle.reconstruct(rec, marker, mask, se))
+
+
 def test_reconstruction_long_geodesic():
     """A serpentine corridor forces propagation through many tiles and rounds."""
     H, W = 256, 256
