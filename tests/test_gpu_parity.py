@@ -570,6 +570,26 @@ def test_label_binary_large_blobs(shape):
         assert n == m
 
 
+def test_label_bench_workload_4096():
+    """the bench's labelling workload at its size (4096^2 blurred-noise mask, seed 4,
+    thresholded at 128 and quantised to 8 levels): binary by SESquare2D and SECross2D and flat
+    zones, equal to the oracle, each three times (the union-find's plain loads race with its
+    atomicMin updates by design; every run must give the same labels)"""
+    from poutre2_amd import synth
+    mask = h(synth.blurred_noise_mask(4096, 4096, 4, device=DEV))
+    binary = (mask > 127).astype(np.uint8)
+    zones = (mask // 32).astype(np.uint8)
+    for img, fn, se, flat in ((binary, pm.label_binary, "SESquare2D", False),
+                              (binary, pm.label_binary, "SECross2D", False),
+                              (zones, pm.label_flat_zones, "SESquare2D", True)):
+        exp, m = oracle.label(img, se, flat=flat)
+        gi = g(img)
+        for _ in range(3):
+            out, n = fn(gi, PM_SE[se])
+            assert n == m, (se, flat, n, m)
+            np.testing.assert_array_equal(h(out), exp)
+
+
 def _serpentine(H, W):
     """one 1-pixel-wide path through every row pair: rows 0, 2, 4, ... are foreground and
     joined alternately at the right and the left end"""
