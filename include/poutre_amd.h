@@ -124,7 +124,9 @@ int pm_gradient(const pm_image *in, int se, int iter, pm_image *out, void *strea
 
 /* ---- geodesy
  * poutre::geo::Reconstruction, include/poutre/geodesy/mreconstruct.hpp:37-41 ->
- * src/geodesy/mreconstruct.cpp:48-132 (mreconstruct_t.hpp:75-512). */
+ * src/geodesy/mreconstruct.cpp:48-132 (mreconstruct_t.hpp:75-512).  Returns after
+ * synchronising the stream (it checks convergence on the host).  For uint8 2D images its
+ * device scratch is kept between calls (up to 8 idle buffers per process, thread-safe). */
 int pm_reconstruct(int rec_type, const pm_image *marker, const pm_image *mask, int se, pm_image *out,
                    void *stream);
 /* ---- testing hooks (no reference counterpart): force the error paths from the host, so that
