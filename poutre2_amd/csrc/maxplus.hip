@@ -241,10 +241,12 @@ __global__ __launch_bounds__(MTHREADS) void mp_fwd_kernel(const float *__restric
 // lane l holding input column (first output column) - R + l: a row's K taps are lanes l..l+2R,
 // reached by DPP shifts (no LDS, no barrier); the vertical window is a ring of K rows in
 // registers.  Lanes 64 - 2R.. only feed their left neighbours.
-// WNR: strip rows (the max-plus backward with gw, K^2 work per row, takes 2 * WNR: fewer
-// halo rows per output row; measured on [64,32,256,256] 5x5: 1.18 -> 1.11 ms forward +
-// backward, while 64-row strips slowed the flat 3x3 backward, 0.65 -> 0.71 ms).  WCH: rows
-// whose loads a wave issues together (16 gained nothing over 8).
+// WNR: strip rows (the max-plus backward with gw takes 2 * WNR: fewer halo rows per output
+// row and fewer per-strip gw partials; measured on [64,32,256,256] 5x5 with the partials in
+// LDS bins, backward alone: 32 / 64 / 128 rows 381 / 351-358 / 362 us, while 64-row strips
+// slowed the flat 3x3 backward, 0.65 -> 0.71 ms).  WCH: rows whose loads a wave issues
+// together (16 gained nothing over 8; 16 in the backward alone: 3x3 slightly faster, 5x5
+// and 7x7 slower).
 constexpr int WNR = 32, WCH = 8;
 
 // lane i receives lane i + 1's value (lane 63 receives 0; no output lane reads it)
